@@ -1,0 +1,175 @@
+"""hybrid3_amd -- tiled Householder QR engine for one MI355X (gfx950).
+
+The product is the C-ABI shared library ``libhybrid3.so`` built from
+``csrc/`` (HIP kernels + host runtime); its entry points are declared in
+``include/*.h``.  This module is the Python-side binding used by the tests
+and ``bench.py``: it loads the library with ctypes and mirrors the
+reference's interface names (``mat_init``, ``Hybrid_tile_QR`` ...).
+
+There is no fallback: if the library is missing, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhybrid3.so")
+
+_dp = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+
+class mat_stat_t(C.Structure):
+    """Same fields as the reference's mat_stat_t (mat_prod.h:12-25)."""
+    _fields_ = [
+        ("mat_M", C.c_ulong), ("mat_N", C.c_ulong),
+        ("my_mat_M", C.c_ulong), ("my_mat_N", C.c_ulong),
+        ("tile_M", C.c_uint), ("tile_N", C.c_uint),
+        ("glob_tile_M", C.c_uint), ("glob_tile_N", C.c_uint),
+        ("my_tile_M", C.c_uint), ("my_tile_N", C.c_uint),
+    ]
+
+
+# name -> (restype, argtypes); every symbol declared in include/*.h
+SIGNATURES = {
+    # mat_prod.h
+    "mat_init": (None, [C.POINTER(mat_stat_t), C.c_long, C.c_long, C.c_int, C.c_int, C.c_int, C.c_int]),
+    # Hybrid_tile_QR.h
+    "Hybrid_tile_QR": (None, [C.POINTER(mat_stat_t), _dp, _dp, C.c_int, C.c_int, C.c_int]),
+    "Hybrid_make_Q": (None, [C.POINTER(mat_stat_t), _dp, _dp, _dp, C.c_int, C.c_int, C.c_int]),
+    "Hybrid_dorgqr": (None, [C.POINTER(mat_stat_t), _dp, _dp, _dp, C.c_int, C.c_int]),
+    "Hybrid_dormqr": (None, [C.POINTER(mat_stat_t), _dp, _dp, _dp, C.c_int, C.c_int]),
+    "Hybrid_make_I": (None, [C.POINTER(mat_stat_t), _dp, C.c_int, C.c_int]),
+    "Hybrid_dgemm": (None, [C.POINTER(mat_stat_t), _dp, _dp, _dp, C.c_int, C.c_int]),
+    # tile_kernel.h
+    "tile_init": (_vp, [C.c_char, C.c_int, C.c_int]),
+    "tile_dgeqrt": (None, [C.c_int, C.c_int, C.c_int, _dp, C.c_int, _dp, _dp, _dp, C.c_int, _dp]),
+    "tile_dtsqrt": (None, [C.c_int, C.c_int, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _dp, C.c_int, _dp]),
+    "tile_dlarfb": (None, [C.c_int, C.c_int, _dp, C.c_int, _dp, _dp, C.c_int, _dp]),
+    "tile_dssrfb": (None, [C.c_int, C.c_int, _dp, _dp, C.c_int, _dp, _dp, C.c_int, _dp]),
+    "tile_dlarft": (None, [C.c_int, C.c_int, C.c_int, _dp, C.c_int, _dp, _dp, C.c_int, _dp]),
+    # hybrid3_dev.h
+    "hq_last_error": (C.c_int, []),
+    "hq_last_error_message": (C.c_char_p, []),
+    "hq_version": (C.c_char_p, []),
+    "hq_tile_qr_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp]),
+    "hq_tile_qr_dev_serial": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp]),
+    "hq_schedule_info": (C.c_int, [C.c_int] * 4 + [C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "hq_dgeqrt_dev": (C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
+    "hq_dtsqrt_dev": (C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp]),
+    "hq_dormqr_dev": (C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
+    "hq_dtsmqr_dev": (C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp]),
+    "hq_fill_uniform_dev": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_uint64, _vp, _vp]),
+    "hq_sync": (C.c_int, [_vp]),
+    "hq_finalize": (None, []),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libhybrid3.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+                " (there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            if not hasattr(L, name):
+                continue
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def build(verbose: bool = False) -> str:
+    from . import _build
+    return _build.build(verbose=verbose)
+
+
+class HQError(RuntimeError):
+    pass
+
+
+def check(rc: int):
+    if rc != 0:
+        msg = lib().hq_last_error_message()
+        raise HQError(f"hybrid3 error {rc}: {msg.decode() if msg else ''}")
+
+
+def last_error():
+    L = lib()
+    return L.hq_last_error(), (L.hq_last_error_message() or b"").decode()
+
+
+def mat_init(M, N, nb, my_rank=0, proc_num=1) -> mat_stat_t:
+    st = mat_stat_t()
+    lib().mat_init(C.byref(st), M, N, nb, nb, my_rank, proc_num)
+    return st
+
+
+def _p(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def Hybrid_tile_QR(st: mat_stat_t, A: np.ndarray, TAU: np.ndarray, my_rank=0, proc_num=1, ib=32):
+    """Host-pointer drop-in (Hybrid_tile_QR.h:26-30) on flat tile-layout arrays."""
+    lib().Hybrid_tile_QR(C.byref(st), _p(A), _p(TAU), my_rank, proc_num, ib)
+    return last_error()
+
+
+# ---------------------------------------------------------------- device API
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream(stream=None):
+    if stream is not None:
+        return C.c_void_p(stream)
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def tile_qr_dev(MT, NT, nb, ib, A, TAU, T=None, stream=None, serial=False):
+    """Factor the device tile-layout tensor A in place (torch float64 tensors)."""
+    fn = lib().hq_tile_qr_dev_serial if serial else lib().hq_tile_qr_dev
+    check(fn(MT, NT, nb, ib, _ptr(A), _ptr(TAU), _ptr(T), _stream(stream)))
+
+
+def fill_uniform_dev(MT, NT, nb, seed, A, stream=None):
+    check(lib().hq_fill_uniform_dev(MT, NT, nb, seed, _ptr(A), _stream(stream)))
+
+
+def schedule_info(MT, NT, nb, ib):
+    n, g = C.c_int(0), C.c_int(0)
+    check(lib().hq_schedule_info(MT, NT, nb, ib, C.byref(n), C.byref(g)))
+    return n.value, g.value
+
+
+def dgeqrt_dev(nb, ib, A, T, TAU, stream=None):
+    check(lib().hq_dgeqrt_dev(nb, ib, _ptr(A), _ptr(T), _ptr(TAU), _stream(stream)))
+
+
+def dtsqrt_dev(nb, ib, A1, A2, T, TAU, stream=None):
+    check(lib().hq_dtsqrt_dev(nb, ib, _ptr(A1), _ptr(A2), _ptr(T), _ptr(TAU), _stream(stream)))
+
+
+def dormqr_dev(nb, ib, V, T, Cm, stream=None):
+    check(lib().hq_dormqr_dev(nb, ib, _ptr(V), _ptr(T), _ptr(Cm), _stream(stream)))
+
+
+def dtsmqr_dev(nb, ib, A1, A2, V, T, stream=None):
+    check(lib().hq_dtsmqr_dev(nb, ib, _ptr(A1), _ptr(A2), _ptr(V), _ptr(T), _stream(stream)))
+
+
+def sync(stream=None):
+    """Wait for the stream and raise if the device scheduler reported a timeout."""
+    check(lib().hq_sync(_stream(stream)))

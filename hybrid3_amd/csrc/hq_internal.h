@@ -1,0 +1,45 @@
+// hq_internal.h -- types shared by the HIP kernels and the host runtime of
+// the tiled QR engine (not part of the public C-ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+// Task kinds of the flat-tree tile QR DAG (Hybrid_tile_QR.c:960,1123,1254,1332)
+// plus the Q-application kinds used by dorgqr / dormqr.
+enum hq_task_kind : int32_t {
+  HQ_GEQRT = 0,   // QR of diagonal tile (k,k)
+  HQ_TSQRT = 1,   // QR of [R(k,k); A(r,k)]
+  HQ_LARFB = 2,   // A(k,j) <- Q(k,k)^T A(k,j)            (one column slab)
+  HQ_SSRFB = 3,   // [A(k,j); A(r,j)] <- Q(r,k)^T [...]   (one column slab)
+};
+
+#define HQ_MAXDEP 6
+
+// One node of the device DAG.  Dependencies always have smaller indices in
+// the task array (the array is a topological order), which is what makes
+// in-order claiming by a persistent kernel deadlock-free.
+struct hq_task {
+  int32_t kind;
+  int32_t k;      // elimination step (panel column)
+  int32_t r;      // tile row (TSQRT / SSRFB)
+  int32_t j;      // tile column (LARFB / SSRFB)
+  int32_t slab;   // 64-column slab of tile column j (LARFB / SSRFB)
+  int32_t ndep;
+  int32_t dep[HQ_MAXDEP];
+};
+
+// Geometry + base pointers of a tile-layout factorization.
+//   tile (i,j):  A   + (i + j*MT) * nb*nb      (column-major, ld = nb)
+//   TAU  (i,j):  TAU + (i + j*MT) * nb
+//   T    (i,j):  T   + (i + j*MT) * ib*nb      (ld = ib; block b at column b*ib)
+struct hq_geom {
+  double *A;
+  double *TAU;
+  double *T;
+  int MT, NT, nb, ib;
+};
+
+#define HQ_THREADS 256
+#define HQ_SLAB 64
+#define HQ_IBMAX 32

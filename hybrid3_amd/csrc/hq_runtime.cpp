@@ -1,0 +1,513 @@
+// hq_runtime.cpp -- host side of the tiled QR engine: the C-ABI entry points,
+// the task-DAG builder that replaces the reference's dependency table
+// (Hybrid_tile_QR.c:36-40,496-601) and TileSch FIFO (TileSch.c:10-61), and the
+// per-device buffers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <queue>
+#include <tuple>
+#include <vector>
+
+#include "../../include/Hybrid_tile_QR.h"
+#include "../../include/hybrid3_dev.h"
+#include "hq_internal.h"
+
+// launch wrappers (hq_factor.hip)
+hipError_t hq_launch_sched(const hq_geom &G, const hq_task *tasks, int ntasks, int *flags, int epoch,
+                           int *counter, int grid, hipStream_t st);
+hipError_t hq_launch_task(const hq_geom &G, const hq_task &tk, hipStream_t st);
+hipError_t hq_launch_geqrt(int nb, int ib, double *A, double *T, double *TAU, hipStream_t st);
+hipError_t hq_launch_tsqrt(int nb, int ib, double *A1, double *A2, double *T, double *TAU,
+                           hipStream_t st);
+hipError_t hq_launch_larfb(int nb, int ib, const double *V, const double *T, double *C,
+                           hipStream_t st);
+hipError_t hq_launch_ssrfb(int nb, int ib, double *A1, double *A2, const double *V, const double *T,
+                           hipStream_t st);
+hipError_t hq_launch_fill_uniform(double *A, int MT, int NT, int nb, uint64_t seed, hipStream_t st);
+
+namespace {
+
+// ---------------------------------------------------------------- errors
+thread_local int g_err = HQ_OK;
+thread_local char g_msg[512] = "";
+
+int set_error(int code, const char *fmt, ...) {
+  g_err = code;
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_msg, sizeof g_msg, fmt, ap);
+  va_end(ap);
+  return code;
+}
+int ok() {
+  g_err = HQ_OK;
+  g_msg[0] = 0;
+  return HQ_OK;
+}
+#define HQ_HIP(call)                                                                      \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return set_error(HQ_ERR_DEVICE, "%s failed: %s", #call, hipGetErrorString(e_));     \
+  } while (0)
+
+bool geometry_ok(int MT, int NT, int nb, int ib) {
+  if (MT < 1 || NT < 1) return set_error(HQ_ERR_ARG, "tile grid %d x %d is empty", MT, NT), false;
+  if (nb != 64 && nb != 128 && nb != 256)
+    return set_error(HQ_ERR_ARG, "nb=%d unsupported (64, 128 or 256)", nb), false;
+  if (ib < 1 || ib > HQ_IBMAX)
+    return set_error(HQ_ERR_ARG, "ib=%d unsupported (1..%d)", ib, HQ_IBMAX), false;
+  return true;
+}
+
+// ---------------------------------------------------------------- DAG
+// Canonical task enumeration (step-major, a topological order):
+//   for k: GEQRT(k); LARFB(k,j,s) for j>k; for r>k: TSQRT(k,r); SSRFB(k,j,r,s) for j>k
+// Dependencies (the reference's MEM / LIN / YTM bits, Hybrid_tile_QR.c:36-40):
+//   GEQRT(k)      MEM: SSRFB(k-1,k,k,*)
+//   TSQRT(k,r)    LIN: TSQRT(k,r-1) | GEQRT(k);  MEM: SSRFB(k-1,k,r,*)
+//   LARFB(k,j,s)  YTM: GEQRT(k);                 MEM: SSRFB(k-1,j,k,s)
+//   SSRFB(k,j,r,s) YTM: TSQRT(k,r); LIN: SSRFB(k,j,r-1,s) | LARFB(k,j,s); MEM: SSRFB(k-1,j,r,s)
+struct Schedule {
+  std::vector<hq_task> tasks;  // in claim order
+  int grid = 0;
+};
+
+struct DagBuilder {
+  int MT, NT, nb, ib, S, KT;
+  std::vector<hq_task> t;
+  std::vector<int> idg, idt, idl, ids;
+  DagBuilder(int MT_, int NT_, int nb_, int ib_) : MT(MT_), NT(NT_), nb(nb_), ib(ib_) {
+    S = (nb + HQ_SLAB - 1) / HQ_SLAB;
+    KT = std::min(MT, NT);
+    idg.assign(KT, -1);
+    idt.assign((size_t)KT * MT, -1);
+    idl.assign((size_t)KT * NT * S, -1);
+    ids.assign((size_t)KT * NT * MT * S, -1);
+  }
+  int &G(int k) { return idg[k]; }
+  int &Tq(int k, int r) { return idt[(size_t)k * MT + r]; }
+  int &L(int k, int j, int s) { return idl[((size_t)k * NT + j) * S + s]; }
+  int &Sr(int k, int j, int r, int s) { return ids[(((size_t)k * NT + j) * MT + r) * S + s]; }
+  int add(int kind, int k, int r, int j, int s) {
+    hq_task x{};
+    x.kind = kind;
+    x.k = k;
+    x.r = r;
+    x.j = j;
+    x.slab = s;
+    x.ndep = 0;
+    t.push_back(x);
+    return (int)t.size() - 1;
+  }
+  void dep(int id, int d) {
+    hq_task &x = t[id];
+    if (d < 0) return;
+    if (x.ndep >= HQ_MAXDEP) {
+      fprintf(stderr, "hybrid3: internal error: too many dependencies\n");
+      return;
+    }
+    x.dep[x.ndep++] = d;
+  }
+  void build() {
+    for (int k = 0; k < KT; k++) {
+      int g = G(k) = add(HQ_GEQRT, k, k, k, 0);
+      if (k > 0)
+        for (int s = 0; s < S; s++) dep(g, Sr(k - 1, k, k, s));
+      for (int j = k + 1; j < NT; j++)
+        for (int s = 0; s < S; s++) {
+          int l = L(k, j, s) = add(HQ_LARFB, k, k, j, s);
+          dep(l, g);
+          if (k > 0) dep(l, Sr(k - 1, j, k, s));
+        }
+      for (int r = k + 1; r < MT; r++) {
+        int q = Tq(k, r) = add(HQ_TSQRT, k, r, k, 0);
+        dep(q, r == k + 1 ? g : Tq(k, r - 1));
+        if (k > 0)
+          for (int s = 0; s < S; s++) dep(q, Sr(k - 1, k, r, s));
+        for (int j = k + 1; j < NT; j++)
+          for (int s = 0; s < S; s++) {
+            int x = Sr(k, j, r, s) = add(HQ_SSRFB, k, r, j, s);
+            dep(x, q);
+            dep(x, r == k + 1 ? L(k, j, s) : Sr(k, j, r - 1, s));
+            if (k > 0) dep(x, Sr(k - 1, j, r, s));
+          }
+      }
+    }
+  }
+  // Estimated durations (us) of each task kind for the priority computation.
+  double cost(const hq_task &x) const {
+    double n3 = (double)nb * nb * nb;
+    double slabw = std::min(HQ_SLAB, nb);
+    switch (x.kind) {
+      case HQ_GEQRT: return (4.0 / 3.0) * n3 / 2.0e4;  // ~20 GFLOP/s per CU
+      case HQ_TSQRT: return 2.0 * n3 / 3.0e4;
+      case HQ_LARFB: return 2.0 * nb * nb * slabw / 1.5e5;
+      default: return 4.0 * nb * nb * slabw / 1.5e5;
+    }
+  }
+  // List-scheduling simulation on P workers with bottom-level priorities;
+  // the dispatch order is a topological order in which tasks are claimed
+  // roughly when they become ready.
+  std::vector<int> order(int P) {
+    const int n = (int)t.size();
+    std::vector<std::vector<int>> succ(n);
+    std::vector<int> indeg(n, 0);
+    for (int i = 0; i < n; i++)
+      for (int d = 0; d < t[i].ndep; d++) {
+        succ[t[i].dep[d]].push_back(i);
+        indeg[i]++;
+      }
+    std::vector<double> bl(n, 0.0);
+    for (int i = n - 1; i >= 0; i--) {
+      double mx = 0.0;
+      for (int s : succ[i]) mx = std::max(mx, bl[s]);
+      bl[i] = cost(t[i]) + mx;
+    }
+    typedef std::pair<double, int> PI;
+    std::priority_queue<PI> ready;  // max bottom level first
+    std::priority_queue<PI, std::vector<PI>, std::greater<PI>> running;  // (finish, id)
+    for (int i = 0; i < n; i++)
+      if (indeg[i] == 0) ready.push(PI(bl[i], -i));
+    std::vector<int> out;
+    out.reserve(n);
+    double now = 0.0;
+    int busy = 0;
+    while ((int)out.size() < n) {
+      while (busy < P && !ready.empty()) {
+        int id = -ready.top().second;
+        ready.pop();
+        out.push_back(id);
+        running.push(PI(now + cost(t[id]), id));
+        busy++;
+      }
+      if (running.empty()) break;  // cannot happen for a DAG
+      PI f = running.top();
+      running.pop();
+      now = f.first;
+      busy--;
+      for (int s : succ[f.second])
+        if (--indeg[s] == 0) ready.push(PI(bl[s], -s));
+    }
+    return out;
+  }
+};
+
+// ---------------------------------------------------------------- device context
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  int reserve(size_t n) {
+    if (n <= bytes) return HQ_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (hipMalloc(&p, n) != hipSuccess) {
+      (void)hipGetLastError();
+      return set_error(HQ_ERR_NOMEM, "hipMalloc(%zu) failed", n);
+    }
+    bytes = n;
+    return HQ_OK;
+  }
+};
+
+struct CachedSched {
+  DevBuf tasks;
+  int ntasks = 0;
+  int grid = 0;
+};
+
+struct Context {
+  int device = -1;
+  int num_cu = 0;
+  hipStream_t stream = nullptr;
+  DevBuf flags, counter, T, A, TAU;
+  int epoch = 0;
+  std::map<std::tuple<int, int, int, int>, std::unique_ptr<CachedSched>> scheds;
+};
+
+std::mutex g_mu;
+std::map<int, std::unique_ptr<Context>> g_ctx;
+
+int get_ctx(Context **out) {
+  int dev = 0;
+  HQ_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto &c = g_ctx[dev];
+  if (!c) {
+    c.reset(new Context());
+    c->device = dev;
+    HQ_HIP(hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    HQ_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    if (c->counter.reserve(256) != HQ_OK) return g_err;
+    HQ_HIP(hipMemset(c->counter.p, 0, 256));
+  }
+  *out = c.get();
+  return HQ_OK;
+}
+
+hipStream_t pick_stream(Context *c, void *s) { return s ? (hipStream_t)s : c->stream; }
+
+int get_schedule(Context *c, int MT, int NT, int nb, int ib, CachedSched **out) {
+  auto key = std::make_tuple(MT, NT, nb, ib);
+  auto it = c->scheds.find(key);
+  if (it != c->scheds.end()) {
+    *out = it->second.get();
+    return HQ_OK;
+  }
+  DagBuilder b(MT, NT, nb, ib);
+  b.build();
+  int grid = c->num_cu > 0 ? c->num_cu : 256;
+  std::vector<int> ord = b.order(grid);
+  const int n = (int)b.t.size();
+  std::vector<int> pos(n);
+  for (int i = 0; i < n; i++) pos[ord[i]] = i;
+  std::vector<hq_task> tl(n);
+  for (int i = 0; i < n; i++) {
+    hq_task x = b.t[ord[i]];
+    for (int d = 0; d < x.ndep; d++) x.dep[d] = pos[x.dep[d]];
+    tl[i] = x;
+  }
+  std::unique_ptr<CachedSched> cs(new CachedSched());
+  if (cs->tasks.reserve(sizeof(hq_task) * (size_t)n) != HQ_OK) return g_err;
+  HQ_HIP(hipMemcpy(cs->tasks.p, tl.data(), sizeof(hq_task) * (size_t)n, hipMemcpyHostToDevice));
+  cs->ntasks = n;
+  cs->grid = std::min(grid, n);
+  // flags: one int per task, zero-initialised; epochs start at 1
+  size_t fbytes = sizeof(int) * (size_t)n;
+  if (c->flags.bytes < fbytes) {
+    if (c->flags.reserve(fbytes) != HQ_OK) return g_err;
+    HQ_HIP(hipMemset(c->flags.p, 0, c->flags.bytes));
+    c->epoch = 0;
+  }
+  *out = cs.get();
+  c->scheds[key] = std::move(cs);
+  return HQ_OK;
+}
+
+int host_schedule(int MT, int NT, int nb, int ib, int P, std::vector<hq_task> &tl) {
+  DagBuilder b(MT, NT, nb, ib);
+  b.build();
+  std::vector<int> ord = b.order(P);
+  tl.resize(ord.size());
+  for (size_t i = 0; i < ord.size(); i++) tl[i] = b.t[ord[i]];
+  return HQ_OK;
+}
+
+int ensure_T(Context *c, int MT, int NT, int nb, int ib, double **T) {
+  if (*T) return HQ_OK;
+  size_t bytes = sizeof(double) * (size_t)MT * NT * ib * nb;
+  if (c->T.reserve(bytes) != HQ_OK) return g_err;
+  *T = (double *)c->T.p;
+  return HQ_OK;
+}
+
+int tile_qr_dev_impl(int MT, int NT, int nb, int ib, double *A, double *TAU, double *T,
+                     void *stream, bool serial) {
+  if (!geometry_ok(MT, NT, nb, ib)) return g_err;
+  if (!A || !TAU) return set_error(HQ_ERR_ARG, "A and TAU must be device pointers");
+  Context *c;
+  if (get_ctx(&c) != HQ_OK) return g_err;
+  hipStream_t st = pick_stream(c, stream);
+  if (ensure_T(c, MT, NT, nb, ib, &T) != HQ_OK) return g_err;
+  hq_geom G{A, TAU, T, MT, NT, nb, ib};
+  if (serial) {
+    std::vector<hq_task> tl;
+    host_schedule(MT, NT, nb, ib, c->num_cu, tl);
+    for (const hq_task &tk : tl) HQ_HIP(hq_launch_task(G, tk, st));
+    return ok();
+  }
+  CachedSched *cs;
+  if (get_schedule(c, MT, NT, nb, ib, &cs) != HQ_OK) return g_err;
+  c->epoch++;
+  HQ_HIP(hq_launch_sched(G, (const hq_task *)cs->tasks.p, cs->ntasks, (int *)c->flags.p, c->epoch,
+                         (int *)c->counter.p, cs->grid, st));
+  return ok();
+}
+
+}  // namespace
+
+// =====================================================================
+// C-ABI
+// =====================================================================
+extern "C" {
+
+int hq_last_error(void) { return g_err; }
+const char *hq_last_error_message(void) { return g_msg; }
+const char *hq_version(void) { return "hybrid3-mi355x 0.1 (gfx950)"; }
+
+void mat_init(mat_stat_t *status, long int mat_m, long int mat_n, int tile_m, int tile_n,
+              int my_rank, int proc_num) {
+  long int mt = mat_m / tile_m;
+  long int nt = mat_n / tile_n;
+  long int mine = nt / proc_num + ((nt % proc_num) > my_rank ? 1 : 0);
+  status->mat_M = mat_m;
+  status->mat_N = mat_n;
+  status->my_mat_M = mat_m;
+  status->my_tile_M = (unsigned)mt;
+  status->glob_tile_M = (unsigned)mt;
+  status->glob_tile_N = (unsigned)nt;
+  status->my_mat_N = (unsigned long)tile_n * mine;
+  status->my_tile_N = (unsigned)mine;
+  status->tile_M = tile_m;
+  status->tile_N = tile_n;
+}
+
+int hq_tile_qr_dev(int MT, int NT, int nb, int ib, double *A, double *TAU, double *T,
+                   void *stream) {
+  return tile_qr_dev_impl(MT, NT, nb, ib, A, TAU, T, stream, false);
+}
+
+int hq_tile_qr_dev_serial(int MT, int NT, int nb, int ib, double *A, double *TAU, double *T,
+                          void *stream) {
+  return tile_qr_dev_impl(MT, NT, nb, ib, A, TAU, T, stream, true);
+}
+
+int hq_schedule_info(int MT, int NT, int nb, int ib, int *ntasks, int *grid) {
+  if (!geometry_ok(MT, NT, nb, ib)) return g_err;
+  int KT = std::min(MT, NT), S = (nb + HQ_SLAB - 1) / HQ_SLAB;
+  long n = 0;
+  for (int k = 0; k < KT; k++)
+    n += 1 + (MT - k - 1) + (long)(NT - k - 1) * S + (long)(NT - k - 1) * (MT - k - 1) * S;
+  if (ntasks) *ntasks = (int)n;
+  if (grid) {
+    int dev = 0, cu = 256;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
+      *grid = (int)std::min<long>(cu, n);
+    else
+      *grid = (int)std::min<long>(256, n);
+  }
+  return ok();
+}
+
+#define HQ_TILE_ARGS_OK(nb, ib)                                                      \
+  if ((nb != 64 && nb != 128 && nb != 256) || ib < 1 || ib > HQ_IBMAX)              \
+    return set_error(HQ_ERR_ARG, "unsupported nb=%d / ib=%d", nb, ib);
+
+int hq_dgeqrt_dev(int nb, int ib, double *A, double *T, double *TAU, void *stream) {
+  HQ_TILE_ARGS_OK(nb, ib);
+  Context *c;
+  if (get_ctx(&c) != HQ_OK) return g_err;
+  HQ_HIP(hq_launch_geqrt(nb, ib, A, T, TAU, pick_stream(c, stream)));
+  return ok();
+}
+int hq_dtsqrt_dev(int nb, int ib, double *A1, double *A2, double *T, double *TAU, void *stream) {
+  HQ_TILE_ARGS_OK(nb, ib);
+  Context *c;
+  if (get_ctx(&c) != HQ_OK) return g_err;
+  HQ_HIP(hq_launch_tsqrt(nb, ib, A1, A2, T, TAU, pick_stream(c, stream)));
+  return ok();
+}
+int hq_dormqr_dev(int nb, int ib, const double *V, const double *T, double *C, void *stream) {
+  HQ_TILE_ARGS_OK(nb, ib);
+  Context *c;
+  if (get_ctx(&c) != HQ_OK) return g_err;
+  HQ_HIP(hq_launch_larfb(nb, ib, V, T, C, pick_stream(c, stream)));
+  return ok();
+}
+int hq_dtsmqr_dev(int nb, int ib, double *A1, double *A2, const double *V, const double *T,
+                  void *stream) {
+  HQ_TILE_ARGS_OK(nb, ib);
+  Context *c;
+  if (get_ctx(&c) != HQ_OK) return g_err;
+  HQ_HIP(hq_launch_ssrfb(nb, ib, A1, A2, V, T, pick_stream(c, stream)));
+  return ok();
+}
+
+int hq_fill_uniform_dev(int MT, int NT, int nb, uint64_t seed, double *A, void *stream) {
+  if (MT < 1 || NT < 1 || nb < 1 || !A) return set_error(HQ_ERR_ARG, "bad fill arguments");
+  Context *c;
+  if (get_ctx(&c) != HQ_OK) return g_err;
+  HQ_HIP(hq_launch_fill_uniform(A, MT, NT, nb, seed, pick_stream(c, stream)));
+  return ok();
+}
+
+int hq_sync(void *stream) {
+  Context *c;
+  if (get_ctx(&c) != HQ_OK) return g_err;
+  HQ_HIP(hipStreamSynchronize(pick_stream(c, stream)));
+  // counter[1] is raised by the scheduler when a dependency wait timed out
+  int st[2] = {0, 0};
+  HQ_HIP(hipMemcpy(st, c->counter.p, sizeof st, hipMemcpyDeviceToHost));
+  if (st[1] != 0) {
+    HQ_HIP(hipMemset((int *)c->counter.p + 1, 0, sizeof(int)));
+    return set_error(HQ_ERR_DEVICE, "scheduler dependency wait timed out at task %d", st[1] - 1);
+  }
+  return ok();
+}
+
+void hq_finalize(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (auto &kv : g_ctx) {
+    if (kv.second && kv.second->stream) (void)hipStreamSynchronize(kv.second->stream);
+  }
+  g_ctx.clear();
+}
+
+// ------------------------------------------------------------ host-pointer API
+static int check_rank(int my_rank, int Proc_num) {
+  if (my_rank != 0 || Proc_num != 1)
+    return set_error(HQ_ERR_RANK,
+                     "the engine runs on one GPU without ranks: my_rank=%d Proc_num=%d "
+                     "(need 0 and 1)",
+                     my_rank, Proc_num);
+  return HQ_OK;
+}
+
+static int report(const char *fn) {
+  if (g_err != HQ_OK) fprintf(stderr, "hybrid3: %s: %s\n", fn, g_msg);
+  return g_err;
+}
+
+void Hybrid_tile_QR(mat_stat_t *status, double *A, double *TAU, int my_rank, int Proc_num,
+                    int ib) {
+  if (check_rank(my_rank, Proc_num) != HQ_OK) {
+    report("Hybrid_tile_QR");
+    return;
+  }
+  if (!status || !A || !TAU || status->tile_M != status->tile_N) {
+    set_error(HQ_ERR_ARG, "need square tiles and non-null A / TAU");
+    report("Hybrid_tile_QR");
+    return;
+  }
+  const int MT = (int)status->glob_tile_M, NT = (int)status->glob_tile_N;
+  const int nb = (int)status->tile_M;
+  if (!geometry_ok(MT, NT, nb, ib)) {
+    report("Hybrid_tile_QR");
+    return;
+  }
+  Context *c;
+  if (get_ctx(&c) != HQ_OK) {
+    report("Hybrid_tile_QR");
+    return;
+  }
+  const size_t nA = (size_t)MT * NT * nb * nb, nT = (size_t)MT * NT * nb;
+  auto run = [&]() -> int {
+    if (c->A.reserve(nA * sizeof(double)) != HQ_OK) return g_err;
+    if (c->TAU.reserve(nT * sizeof(double)) != HQ_OK) return g_err;
+    HQ_HIP(hipMemcpyAsync(c->A.p, A, nA * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HQ_HIP(hipMemcpyAsync(c->TAU.p, TAU, nT * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (tile_qr_dev_impl(MT, NT, nb, ib, (double *)c->A.p, (double *)c->TAU.p, nullptr, c->stream,
+                         false) != HQ_OK)
+      return g_err;
+    HQ_HIP(hipMemcpyAsync(A, c->A.p, nA * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HQ_HIP(hipMemcpyAsync(TAU, c->TAU.p, nT * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HQ_HIP(hipStreamSynchronize(c->stream));
+    return ok();
+  };
+  run();
+  report("Hybrid_tile_QR");
+}
+
+}  // extern "C"

@@ -1,0 +1,83 @@
+/*
+ * hybrid3_dev.h -- device-pointer entry points and error reporting of the
+ * tiled QR engine (libhybrid3.so).
+ *
+ * These have no counterpart in the reference API (whose functions take host
+ * pointers and return void); they are the measured path and the per-kernel
+ * boundary used by the parity tests.  Every function returns 0 on success
+ * or a nonzero hq error code (also kept in hq_last_error()).  `stream` is a
+ * hipStream_t passed as void* (NULL = the library's own stream); work is
+ * queued on it and the call returns without synchronising unless stated.
+ *
+ * Tile layout of A / TAU / T (all device memory):
+ *   tile (i,j): A   + (i + j*MT)*nb*nb   column-major, ld = nb
+ *   TAU  (i,j): TAU + (i + j*MT)*nb
+ *   T    (i,j): T   + (i + j*MT)*ib*nb   ld = ib, block b at column b*ib
+ */
+#ifndef HQ_HYBRID3_DEV_H_
+#define HQ_HYBRID3_DEV_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  HQ_OK = 0,
+  HQ_ERR_ARG = 1,      /* invalid or unsupported argument */
+  HQ_ERR_RANK = 2,     /* my_rank != 0 or Proc_num != 1 */
+  HQ_ERR_DEVICE = 3,   /* HIP runtime error */
+  HQ_ERR_NOMEM = 4     /* device allocation failed */
+};
+
+int hq_last_error(void);
+const char *hq_last_error_message(void);
+
+/* Version string of the library build. */
+const char *hq_version(void);
+
+/* Flat-tree tile QR of an MT x NT tile grid on the device with the
+ * persistent DAG scheduler (one launch).  T may be NULL (the library keeps
+ * its own); otherwise it receives every tile's ib x nb T factor. */
+int hq_tile_qr_dev(int MT, int NT, int nb, int ib, double *A, double *TAU,
+                   double *T, void *stream);
+
+/* Same factorisation, one kernel launch per task in the scheduler's order
+ * (stream order replaces the device dependency flags). */
+int hq_tile_qr_dev_serial(int MT, int NT, int nb, int ib, double *A,
+                          double *TAU, double *T, void *stream);
+
+/* Schedule of the DAG for a geometry: number of tasks and the number of
+ * persistent workgroups a launch uses. */
+int hq_schedule_info(int MT, int NT, int nb, int ib, int *ntasks, int *grid);
+
+/* PLASMA-style single-tile kernels on device pointers (nb in {64,128,256},
+ * 1 <= ib <= 32, every leading dimension = nb, T leading dimension = ib):
+ *   GEQRT = CORE_dgeqrt(nb,nb,ib,A,nb,T,ib,TAU)        (Hybrid_tile_QR.c:985)
+ *   TSQRT = CORE_dtsqrt(nb,nb,ib,A1,nb,A2,nb,T,ib,TAU) (Hybrid_tile_QR.c:1135)
+ *   LARFB = CORE_dormqr(Left,Trans,nb,nb,nb,ib,V,nb,T,ib,C,nb) (:1276)
+ *   SSRFB = CORE_dtsmqr(Left,Trans,nb,nb,nb,nb,nb,ib,A1,nb,A2,nb,V,nb,T,ib) (:1356) */
+int hq_dgeqrt_dev(int nb, int ib, double *A, double *T, double *TAU, void *stream);
+int hq_dtsqrt_dev(int nb, int ib, double *A1, double *A2, double *T,
+                  double *TAU, void *stream);
+int hq_dormqr_dev(int nb, int ib, const double *V, const double *T, double *C,
+                  void *stream);
+int hq_dtsmqr_dev(int nb, int ib, double *A1, double *A2, const double *V,
+                  const double *T, void *stream);
+
+/* A (tile layout, MT x NT tiles of nb) <- uniform[-1,1) from SplitMix64 of
+ * (seed, column-major global index). */
+int hq_fill_uniform_dev(int MT, int NT, int nb, uint64_t seed, double *A,
+                        void *stream);
+
+/* Synchronise the library's stream (or `stream`). */
+int hq_sync(void *stream);
+
+/* Release every device buffer the library holds. */
+void hq_finalize(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
