@@ -25,6 +25,15 @@ __device__ __forceinline__ d4 mfma(double a, double b, d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// v_mfma_f64_16x16x4f64 lane layout (gfx950): A(i,k) at lane i + 16k,
+// B(k,j) at lane j + 16k, D(i,j) at lane j + 16*(i%4), register i/4.
+// Feeding A-operand row i from matrix row arow(i) = 4*(i%4) + i/4 makes
+// register v of lane group g hold matrix row 4g + v, so an accumulator
+// strip of a column is four consecutive doubles per lane (32-byte loads),
+// and the same registers serve as the B operand with k index g, register s
+// <-> row 4g + s.
+__device__ __forceinline__ int arow(int i) { return 4 * (i & 3) + (i >> 2); }
+
 // Butterfly sum: every lane ends with the bitwise-identical total (each level
 // adds the same two operands, only in swapped order).
 __device__ __forceinline__ double wave_sum(double v) {
@@ -93,6 +102,7 @@ __device__ __forceinline__ void apply_block(d4 (&acc)[NB / 16], int rb0, const S
                                             double *a1col, int kb, bool ok) {
   constexpr int LDV = Smem<NB>::LDV, LDT = Smem<NB>::LDT;
   const int lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4;
+  const int pm = arow(m);
   d4 Wa[WB], Wb[WB], A1v[WB];
 #pragma unroll
   for (int wb = 0; wb < WB; wb++) {
@@ -115,7 +125,7 @@ __device__ __forceinline__ void apply_block(d4 (&acc)[NB / 16], int rb0, const S
     for (int s = 0; s < 4; s++) {
 #pragma unroll
       for (int wb = 0; wb < WB; wb++) {
-        double a = S.V[(16 * wb + m) * LDV + 16 * rb + 4 * g + s];
+        double a = S.V[(16 * wb + pm) * LDV + 16 * rb + 4 * g + s];
         if (rb & 1)
           Wb[wb] = mfma(a, acc[rb][s], Wb[wb]);
         else
@@ -140,7 +150,7 @@ __device__ __forceinline__ void apply_block(d4 (&acc)[NB / 16], int rb0, const S
     for (int s = 0; s < 4; s++)
 #pragma unroll
       for (int wo = wi; wo < WB; wo++) {
-        double a = S.TT[(16 * wo + m) * LDT + 16 * wi + 4 * g + s];
+        double a = S.TT[(16 * wo + pm) * LDT + 16 * wi + 4 * g + s];
         W2[wo] = mfma(a, W[wi][s], W2[wo]);
       }
   if (HAS_A1 && ok) {
@@ -162,7 +172,7 @@ __device__ __forceinline__ void apply_block(d4 (&acc)[NB / 16], int rb0, const S
     for (int wb = 0; wb < WB; wb++)
 #pragma unroll
       for (int s = 0; s < 4; s++) {
-        double a = S.V[(16 * wb + 4 * g + s) * LDV + 16 * rb + m];
+        double a = S.V[(16 * wb + 4 * g + s) * LDV + 16 * rb + pm];
         acc[rb] = mfma(a, W2[wb][s], acc[rb]);
       }
     __builtin_amdgcn_sched_group_barrier(0x100, 4 * WB, 1);
@@ -270,6 +280,7 @@ template <int NB, int WB>
 __device__ __forceinline__ void gram(Smem<NB> &S, int rb0) {
   constexpr int LDV = Smem<NB>::LDV, LDT = Smem<NB>::LDT;
   const int lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4, wave = threadIdx.x >> 6;
+  const int pm = arow(m);
   constexpr int NBLK = WB * (WB + 1) / 2;
   if (wave >= NBLK) return;
   int pb = 0, cb = 0, t = wave;
@@ -284,7 +295,7 @@ __device__ __forceinline__ void gram(Smem<NB> &S, int rb0) {
   for (int rb = rb0; rb < NB / 16; rb++)
 #pragma unroll
     for (int s = 0; s < 4; s++) {
-      double a = S.V[(16 * pb + m) * LDV + 16 * rb + 4 * g + s];
+      double a = S.V[(16 * pb + pm) * LDV + 16 * rb + 4 * g + s];
       double b = S.V[(16 * cb + m) * LDV + 16 * rb + 4 * g + s];
       acc = mfma(a, b, acc);
     }
@@ -511,41 +522,48 @@ __global__ __launch_bounds__(HQ_THREADS) void sched_kernel(hq_geom G, const hq_t
                                                           int ntasks, int *flags, int epoch,
                                                           int *counter) {
   __shared__ Smem<NB> S;
+  int done = -1;  // task finished by this workgroup in the previous iteration
   for (;;) {
+    // Thread 0 publishes the previous task and claims the next one in ONE
+    // divergent region.  (Two separate thread-0 regions at the end and the
+    // start of the loop body get merged across the back edge by the CFG
+    // structurizer into an inner loop that the other lanes never leave.)
     if (threadIdx.x == 0) {
+      if (done >= 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_store(&flags[done], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
       int t = atomicAdd(counter, 1);
       // counter[1] != 0: a wait timed out somewhere; drain without work
       if (__hip_atomic_load(&counter[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
         t = ntasks;
-      S.task = t;
       if (t < ntasks) {
         const hq_task &tk = tasks[t];
         for (int i = 0; i < tk.ndep && t < ntasks; i++) {
           const int d = tk.dep[i];
           unsigned spins = 0;
-          while (__hip_atomic_load(&flags[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+          while (t < ntasks &&
+                 __hip_atomic_load(&flags[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
             __builtin_amdgcn_s_sleep(2);
             if (++spins > (1u << 22)) {  // seconds: a dependency never completed
               __hip_atomic_store(&counter[1], 1 + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
               t = ntasks;
-              break;
             }
           }
         }
-        S.task = t;
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       }
+      S.task = t;
     }
     __syncthreads();
-    const int t = S.task;
+    // readfirstlane: the task index, the loop exit and every barrier after it
+    // are wave-uniform for the compiler
+    const int t = __builtin_amdgcn_readfirstlane(S.task);
     if (t >= ntasks) break;
     const hq_task tk = tasks[t];
     run_task<NB, WB>(S, G, tk);
     __syncthreads();
-    if (threadIdx.x == 0) {
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      __hip_atomic_store(&flags[t], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    done = t;
   }
 }
 

@@ -256,7 +256,9 @@ int get_ctx(Context **out) {
   return HQ_OK;
 }
 
-hipStream_t pick_stream(Context *c, void *s) { return s ? (hipStream_t)s : c->stream; }
+// NULL is HIP's default (null) stream, as for every HIP API; the host-pointer
+// entry points pass the library's own stream explicitly.
+hipStream_t pick_stream(Context *, void *s) { return (hipStream_t)s; }
 
 int get_schedule(Context *c, int MT, int NT, int nb, int ib, CachedSched **out) {
   auto key = std::make_tuple(MT, NT, nb, ib);

@@ -6,7 +6,7 @@
  * pointers and return void); they are the measured path and the per-kernel
  * boundary used by the parity tests.  Every function returns 0 on success
  * or a nonzero hq error code (also kept in hq_last_error()).  `stream` is a
- * hipStream_t passed as void* (NULL = the library's own stream); work is
+ * hipStream_t passed as void* (NULL = HIP's default stream); work is
  * queued on it and the call returns without synchronising unless stated.
  *
  * Tile layout of A / TAU / T (all device memory):
@@ -71,7 +71,7 @@ int hq_dtsmqr_dev(int nb, int ib, double *A1, double *A2, const double *V,
 int hq_fill_uniform_dev(int MT, int NT, int nb, uint64_t seed, double *A,
                         void *stream);
 
-/* Synchronise the library's stream (or `stream`). */
+/* Synchronise `stream`; reports a scheduler timeout raised on the device. */
 int hq_sync(void *stream);
 
 /* Release every device buffer the library holds. */
