@@ -63,6 +63,9 @@ SIGNATURES = {
     "hq_dormqr_dev": (C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
     "hq_dtsmqr_dev": (C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp]),
     "hq_fill_uniform_dev": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_uint64, _vp, _vp]),
+    "hq_tile_qr_dev_traced": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp, _vp]),
+    "hq_schedule_tasks": (C.c_int, [C.c_int] * 4 + [C.POINTER(C.c_int), C.c_int]),
+    "hq_wall_clock_khz": (C.c_int, [C.POINTER(C.c_int)]),
     "hq_sync": (C.c_int, [_vp]),
     "hq_finalize": (None, []),
 }

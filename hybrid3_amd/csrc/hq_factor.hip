@@ -53,12 +53,26 @@ __device__ __forceinline__ double dlapy2(double x, double y) {
 
 template <int NB>
 struct Smem {
-  static constexpr int LDV = NB + 1;        // odd stride: spreads LDS banks
+  // column stride NB+2: each column starts 16-byte aligned (direct-to-LDS loads
+  // write 64 lanes x 16 bytes contiguously) and consecutive columns fall on
+  // different LDS banks (2*c mod 32 distinct for the 16 columns of an MFMA)
+  static constexpr int LDV = NB + 2;
   static constexpr int LDT = HQ_IBMAX + 1;
-  double V[HQ_IBMAX * LDV];   // V block / panel, column-major V[c*LDV + r]
-  double TT[HQ_IBMAX * LDT];  // T^T of the block, TT[c*LDT + p] = T(p,c)
-  double A1[HQ_IBMAX * LDT];  // TSQRT triangle block, A1[c*LDT + p] = R(p,c)
-  double G[HQ_IBMAX * LDT];   // Gram V^T V, G[p*LDT + c]
+  static constexpr int VSZ = HQ_IBMAX * LDV;
+  static constexpr int TSZ = HQ_IBMAX * LDT;
+  // buffer 0: V block / panel, column-major V[c*LDV + r]; T^T, TT[c*LDT + p] = T(p,c)
+  double V[VSZ];
+  double TT[TSZ];
+  union {
+    struct {  // buffer 1 of the LARFB / SSRFB pipeline
+      double V1[VSZ];
+      double TT1[TSZ];
+    };
+    struct {  // panel tasks only
+      double A1[TSZ];  // TSQRT triangle block, A1[c*LDT + p] = R(p,c)
+      double G[TSZ];   // Gram V^T V, G[p*LDT + c]
+    };
+  };
   double tau[HQ_IBMAX];
   int task;
 };
@@ -97,26 +111,42 @@ __device__ __forceinline__ void store_strip(const d4 (&acc)[NB / 16], double *ti
 //   A1 -= W ;  X -= V W
 // X is the strip held in `acc` (rows >= 16*rb0 take part).  WB = IBP/16.
 // --------------------------------------------------------------------------
-template <int NB, int WB, bool HAS_A1>
-__device__ __forceinline__ void apply_block(d4 (&acc)[NB / 16], int rb0, const Smem<NB> &S,
-                                            double *a1col, int kb, bool ok) {
-  constexpr int LDV = Smem<NB>::LDV, LDT = Smem<NB>::LDT;
-  const int lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4;
-  const int pm = arow(m);
-  d4 Wa[WB], Wb[WB], A1v[WB];
+// UNIT: Vs holds the raw tile (R above the diagonal): the operand is masked
+// to the unit lower trapezoid with its diagonal at row i0 + c.  Columns
+// c >= kb of Vs are treated as zero.
+template <int WB>
+__device__ __forceinline__ void load_a1(d4 (&A1v)[WB], const double *a1col, int kb, bool ok) {
+  const int g = (threadIdx.x & 63) >> 4;
 #pragma unroll
   for (int wb = 0; wb < WB; wb++) {
     A1v[wb] = d4{0.0, 0.0, 0.0, 0.0};
-    if (HAS_A1 && ok) {
+    if (ok) {
 #pragma unroll
       for (int v = 0; v < 4; v++) {
         int row = 16 * wb + 4 * g + v;
         if (row < kb) A1v[wb][v] = a1col[row];
       }
     }
-    Wa[wb] = A1v[wb];
-    Wb[wb] = d4{0.0, 0.0, 0.0, 0.0};
   }
+}
+
+template <int NB, int WB, bool HAS_A1, bool UNIT = false>
+__device__ __forceinline__ void apply_block(d4 (&acc)[NB / 16], int rb0, const double *Vs,
+                                            const double *TTs, const d4 (&A1v)[WB], double *a1col,
+                                            int kb, bool ok, int i0 = 0) {
+  constexpr int LDV = Smem<NB>::LDV, LDT = Smem<NB>::LDT;
+  const int lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4;
+  const int pm = arow(m);
+  auto vmask = [&](double val, int r, int c) -> double {
+    if (UNIT) {
+      const int d = i0 + c;
+      val = r > d ? val : (r == d ? 1.0 : 0.0);
+    }
+    return c < kb ? val : 0.0;
+  };
+  d4 W[WB];
+#pragma unroll
+  for (int wb = 0; wb < WB; wb++) W[wb] = HAS_A1 ? A1v[wb] : d4{0.0, 0.0, 0.0, 0.0};
   // W = A1 + V^T X: the k index of MFMA (rb, s) at lane group g is row 16rb+4g+s.
 #pragma unroll
   for (int rb = 0; rb < NB / 16; rb++) {
@@ -125,11 +155,9 @@ __device__ __forceinline__ void apply_block(d4 (&acc)[NB / 16], int rb0, const S
     for (int s = 0; s < 4; s++) {
 #pragma unroll
       for (int wb = 0; wb < WB; wb++) {
-        double a = S.V[(16 * wb + pm) * LDV + 16 * rb + 4 * g + s];
-        if (rb & 1)
-          Wb[wb] = mfma(a, acc[rb][s], Wb[wb]);
-        else
-          Wa[wb] = mfma(a, acc[rb][s], Wa[wb]);
+        double a = vmask(Vs[(16 * wb + pm) * LDV + 16 * rb + 4 * g + s], 16 * rb + 4 * g + s,
+                         16 * wb + pm);
+        W[wb] = mfma(a, acc[rb][s], W[wb]);
       }
     }
     // keep the operand loads of one row block next to its MFMAs (stops the
@@ -137,12 +165,9 @@ __device__ __forceinline__ void apply_block(d4 (&acc)[NB / 16], int rb0, const S
     __builtin_amdgcn_sched_group_barrier(0x100, 4 * WB, 0);
     __builtin_amdgcn_sched_group_barrier(0x008, 4 * WB, 0);
   }
-  d4 W[WB], W2[WB];
+  d4 W2[WB];
 #pragma unroll
-  for (int wb = 0; wb < WB; wb++) {
-    W[wb] = Wa[wb] + Wb[wb];
-    W2[wb] = d4{0.0, 0.0, 0.0, 0.0};
-  }
+  for (int wb = 0; wb < WB; wb++) W2[wb] = d4{0.0, 0.0, 0.0, 0.0};
   // W2 = T^T W (T^T lower triangular)
 #pragma unroll
   for (int wi = 0; wi < WB; wi++)
@@ -150,10 +175,10 @@ __device__ __forceinline__ void apply_block(d4 (&acc)[NB / 16], int rb0, const S
     for (int s = 0; s < 4; s++)
 #pragma unroll
       for (int wo = wi; wo < WB; wo++) {
-        double a = S.TT[(16 * wo + pm) * LDT + 16 * wi + 4 * g + s];
+        double a = TTs[(16 * wo + pm) * LDT + 16 * wi + 4 * g + s];
         W2[wo] = mfma(a, W[wi][s], W2[wo]);
       }
-  if (HAS_A1 && ok) {
+  if (HAS_A1 && ok) {  // A1 -= W2 (store only: no load in flight behind the prefetch)
 #pragma unroll
     for (int wb = 0; wb < WB; wb++)
 #pragma unroll
@@ -172,7 +197,8 @@ __device__ __forceinline__ void apply_block(d4 (&acc)[NB / 16], int rb0, const S
     for (int wb = 0; wb < WB; wb++)
 #pragma unroll
       for (int s = 0; s < 4; s++) {
-        double a = S.V[(16 * wb + 4 * g + s) * LDV + 16 * rb + pm];
+        double a = vmask(Vs[(16 * wb + 4 * g + s) * LDV + 16 * rb + pm], 16 * rb + pm,
+                         16 * wb + 4 * g + s);
         acc[rb] = mfma(a, W2[wb][s], acc[rb]);
       }
     __builtin_amdgcn_sched_group_barrier(0x100, 4 * WB, 1);
@@ -365,7 +391,8 @@ __device__ __forceinline__ void task_geqrt(Smem<NB> &S, double *A, double *T, do
       bool ok = col < NB;
       d4 acc[NB / 16];
       load_strip<NB>(acc, A, col, ok, ii / 16);
-      apply_block<NB, WB, false>(acc, ii / 16, S, nullptr, sb, ok);
+      d4 A1v[WB];
+      apply_block<NB, WB, false>(acc, ii / 16, S.V, S.TT, A1v, nullptr, sb, ok);
       store_strip<NB>(acc, A, col, ok, ii / 16);
     }
     __syncthreads();
@@ -410,73 +437,146 @@ __device__ __forceinline__ void task_tsqrt(Smem<NB> &S, double *A1, double *A2, 
       bool ok = col < NB;
       d4 acc[NB / 16];
       load_strip<NB>(acc, A2, col, ok, 0);
-      apply_block<NB, WB, true>(acc, 0, S, A1 + (size_t)col * NB + ii, sb, ok);
+      d4 A1v[WB];
+      load_a1<WB>(A1v, A1 + (size_t)col * NB + ii, sb, ok);
+      apply_block<NB, WB, true>(acc, 0, S.V, S.TT, A1v, A1 + (size_t)col * NB + ii, sb, ok);
       store_strip<NB>(acc, A2, col, ok, 0);
     }
     __syncthreads();
   }
 }
 
-// stage block b of a reflector tile: unit lower (LARFB) or full (SSRFB), and T^T
-template <int NB, int WB, bool UNIT>
-__device__ __forceinline__ void stage_block(Smem<NB> &S, const double *V, const double *T, int ib,
-                                            int i0, int kb) {
-  constexpr int LDV = Smem<NB>::LDV, LDT = Smem<NB>::LDT;
+// Staging of one ib block of a reflector tile and its T^T for the LARFB /
+// SSRFB pipeline.  V (raw tile columns i0..i0+kb) goes straight from global
+// memory into LDS with direct-to-LDS 16-byte loads (one wave instruction per
+// 128 doubles of a column; no registers involved); nb = 64 uses ordinary
+// loads.  T^T (at most 32 x 32) goes through registers: each thread holds
+// IBP*IBP/256 entries between the load and the LDS store.
+template <int NB, int WB>
+struct BlockRegs {
+  static constexpr int IBP = 16 * WB;
+  static constexpr int NPT = (IBP * IBP + HQ_THREADS - 1) / HQ_THREADS;
+  static constexpr int NPV = NB >= 128 ? 1 : IBP * NB / HQ_THREADS;
+  double t[NPT];
+  double v[NPV];
+};
+
+typedef __attribute__((address_space(3))) char lds_char;
+
+template <int NB, int WB>
+__device__ __forceinline__ void fetch_block(BlockRegs<NB, WB> &R, double *Vs_next, const double *V,
+                                            const double *T, int ib, int i0, int kb) {
   constexpr int IBP = 16 * WB;
-  const int tid = threadIdx.x;
-  for (int e = tid; e < IBP * NB; e += HQ_THREADS) {
-    int c = e / NB, r = e - c * NB;
-    double val = 0.0;
-    if (c < kb) {
-      if (UNIT)
-        val = (r > i0 + c) ? V[(size_t)(i0 + c) * NB + r] : (r == i0 + c ? 1.0 : 0.0);
-      else
-        val = V[(size_t)(i0 + c) * NB + r];
+  constexpr int LDV = Smem<NB>::LDV;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if constexpr (NB >= 128) {
+    constexpr int HALVES = NB / 128;
+    const int nq = kb * HALVES;
+    for (int q = wave; q < nq; q += HQ_THREADS / 64) {
+      const int c = q / HALVES, h = q - c * HALVES;
+      const double *src = V + (size_t)(i0 + c) * NB + 128 * h + 2 * lane;
+      lds_char *dst = (lds_char *)(Vs_next + c * LDV + 128 * h);
+      __builtin_amdgcn_global_load_lds((const void *)src, (void __attribute__((address_space(3))) *)dst,
+                                       16, 0, 0);
     }
-    S.V[c * LDV + r] = val;
+  } else {
+#pragma unroll
+    for (int i = 0; i < BlockRegs<NB, WB>::NPV; i++) {
+      const int e = tid + HQ_THREADS * i;
+      const int c = e / NB, r = e - c * NB;
+      R.v[i] = c < kb ? V[(size_t)(i0 + c) * NB + r] : 0.0;
+    }
   }
-  for (int e = tid; e < IBP * IBP; e += HQ_THREADS) {
-    int c = e / IBP, p = e - c * IBP;
-    S.TT[c * LDT + p] = (c < kb && p <= c) ? T[(size_t)(i0 + c) * ib + p] : 0.0;
+#pragma unroll
+  for (int i = 0; i < BlockRegs<NB, WB>::NPT; i++) {
+    const int e = tid + HQ_THREADS * i;
+    const int c = e / IBP, p = e - c * IBP;
+    R.t[i] = (e < IBP * IBP && c < kb && p <= c) ? T[(size_t)(i0 + c) * ib + p] : 0.0;
   }
 }
 
-// LARFB = CORE_dormqr(Left, Trans, nb, nb, nb, ib, V, nb, T, ib, C, nb), one slab
 template <int NB, int WB>
-__device__ __forceinline__ void task_larfb(Smem<NB> &S, const double *V, const double *T, double *C, int ib,
-                           int slab) {
-  const int lane = threadIdx.x & 63, m = lane & 15, wave = threadIdx.x >> 6;
-  const int col = slab * HQ_SLAB + 16 * wave + m;
-  const bool ok = col < NB;
-  d4 acc[NB / 16];
-  load_strip<NB>(acc, C, col, ok, 0);
-  for (int i0 = 0; i0 < NB; i0 += ib) {
-    const int kb = min(ib, NB - i0);
-    stage_block<NB, WB, true>(S, V, T, ib, i0, kb);
-    __syncthreads();
-    apply_block<NB, WB, false>(acc, i0 / 16, S, nullptr, kb, ok);
-    __syncthreads();
+__device__ __forceinline__ void store_block(const BlockRegs<NB, WB> &R, double *Vs, double *TTs) {
+  constexpr int IBP = 16 * WB;
+  constexpr int LDV = Smem<NB>::LDV, LDT = Smem<NB>::LDT;
+  const int tid = threadIdx.x;
+  if constexpr (NB < 128) {
+#pragma unroll
+    for (int i = 0; i < BlockRegs<NB, WB>::NPV; i++) {
+      const int e = tid + HQ_THREADS * i;
+      const int c = e / NB, r = e - c * NB;
+      Vs[c * LDV + r] = R.v[i];
+    }
   }
-  store_strip<NB>(acc, C, col, ok, 0);
+#pragma unroll
+  for (int i = 0; i < BlockRegs<NB, WB>::NPT; i++) {
+    const int e = tid + HQ_THREADS * i;
+    const int c = e / IBP, p = e - c * IBP;
+    if (e < IBP * IBP) TTs[c * LDT + p] = R.t[i];
+  }
+  // the direct-to-LDS loads of this block complete on the vector memory
+  // counter: wait for them before the barrier that publishes the buffer
+  __builtin_amdgcn_s_waitcnt(0);
 }
 
-// SSRFB = CORE_dtsmqr(Left, Trans, nb, nb, nb, nb, nb, ib, A1, nb, A2, nb, V, nb, T, ib), one slab
-template <int NB, int WB>
-__device__ __forceinline__ void task_ssrfb(Smem<NB> &S, double *A1, double *A2, const double *V, const double *T,
-                           int ib, int slab) {
+// LARFB (UNIT, no A1) = CORE_dormqr(Left, Trans, nb, nb, nb, ib, V, nb, T, ib, X, nb)
+// SSRFB (full V, A1)  = CORE_dtsmqr(Left, Trans, nb, nb, nb, nb, nb, ib, A1, nb, X, nb, V, nb, T, ib)
+// on one 64-column slab.  The strip of X stays in registers for all blocks;
+// block b+1 of V/T is loaded while block b is applied (two LDS buffers, one
+// barrier per block).
+template <int NB, int WB, bool UNIT>
+__device__ __forceinline__ void task_update(Smem<NB> &S, const double *V, const double *T,
+                                            double *X, double *A1, int ib, int slab) {
   const int lane = threadIdx.x & 63, m = lane & 15, wave = threadIdx.x >> 6;
   const int col = slab * HQ_SLAB + 16 * wave + m;
   const bool ok = col < NB;
+  const int nblk = (NB + ib - 1) / ib;
+  double *a1c = UNIT ? nullptr : A1 + (size_t)col * NB;
   d4 acc[NB / 16];
-  load_strip<NB>(acc, A2, col, ok, 0);
-  for (int i0 = 0; i0 < NB; i0 += ib) {
-    const int kb = min(ib, NB - i0);
-    stage_block<NB, WB, false>(S, V, T, ib, i0, kb);
-    __syncthreads();
-    apply_block<NB, WB, true>(acc, 0, S, A1 + (size_t)col * NB + i0, kb, ok);
+  d4 A1v[WB], A1n[WB];
+  BlockRegs<NB, WB> R;
+  fetch_block<NB, WB>(R, S.V, V, T, ib, 0, min(ib, NB));
+  load_strip<NB>(acc, X, col, ok, 0);
+  if (!UNIT) load_a1<WB>(A1v, a1c, min(ib, NB), ok);
+  store_block<NB, WB>(R, S.V, S.TT);
+  __syncthreads();
+  for (int b = 0; b < nblk; b++) {
+    const int i0 = b * ib, kb = min(ib, NB - i0);
+    const bool more = b + 1 < nblk;
+    double *Vn = (b & 1) ? S.V : S.V1;
+    double *TTn = (b & 1) ? S.TT : S.TT1;
+    // everything for block b+1 is issued before the MFMAs of block b and
+    // waited for only at the end of the block
+    if (more) {
+      fetch_block<NB, WB>(R, Vn, V, T, ib, i0 + ib, min(ib, NB - i0 - ib));
+      if (!UNIT) load_a1<WB>(A1n, a1c + i0 + ib, min(ib, NB - i0 - ib), ok);
+    }
+    const double *Vs = (b & 1) ? S.V1 : S.V;
+    const double *TTs = (b & 1) ? S.TT1 : S.TT;
+    if (UNIT)
+      apply_block<NB, WB, false, true>(acc, i0 / 16, Vs, TTs, A1v, nullptr, kb, ok, i0);
+    else
+      apply_block<NB, WB, true>(acc, 0, Vs, TTs, A1v, a1c + i0, kb, ok);
+    if (more) {
+      store_block<NB, WB>(R, Vn, TTn);
+#pragma unroll
+      for (int wb = 0; wb < WB; wb++) A1v[wb] = A1n[wb];
+    }
     __syncthreads();
   }
-  store_strip<NB>(acc, A2, col, ok, 0);
+  store_strip<NB>(acc, X, col, ok, 0);
+}
+
+template <int NB, int WB>
+__device__ __forceinline__ void task_larfb(Smem<NB> &S, const double *V, const double *T, double *C,
+                                           int ib, int slab) {
+  task_update<NB, WB, true>(S, V, T, C, nullptr, ib, slab);
+}
+
+template <int NB, int WB>
+__device__ __forceinline__ void task_ssrfb(Smem<NB> &S, double *A1, double *A2, const double *V,
+                                           const double *T, int ib, int slab) {
+  task_update<NB, WB, false>(S, V, T, A2, A1, ib, slab);
 }
 
 __device__ __forceinline__ double *tile_ptr(const hq_geom &G, int i, int j) {
@@ -520,9 +620,10 @@ __device__ __forceinline__ void run_task(Smem<NB> &S, const hq_geom &G, const hq
 template <int NB, int WB>
 __global__ __launch_bounds__(HQ_THREADS) void sched_kernel(hq_geom G, const hq_task *__restrict__ tasks,
                                                           int ntasks, int *flags, int epoch,
-                                                          int *counter) {
+                                                          int *counter, long long *trace) {
   __shared__ Smem<NB> S;
   int done = -1;  // task finished by this workgroup in the previous iteration
+  long long t_claim = 0;
   for (;;) {
     // Thread 0 publishes the previous task and claims the next one in ONE
     // divergent region.  (Two separate thread-0 regions at the end and the
@@ -533,7 +634,9 @@ __global__ __launch_bounds__(HQ_THREADS) void sched_kernel(hq_geom G, const hq_t
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __hip_atomic_store(&flags[done], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
+      if (trace && done >= 0) trace[4 * done + 2] = wall_clock64();
       int t = atomicAdd(counter, 1);
+      if (trace) t_claim = wall_clock64();
       // counter[1] != 0: a wait timed out somewhere; drain without work
       if (__hip_atomic_load(&counter[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
         t = ntasks;
@@ -561,10 +664,16 @@ __global__ __launch_bounds__(HQ_THREADS) void sched_kernel(hq_geom G, const hq_t
     const int t = __builtin_amdgcn_readfirstlane(S.task);
     if (t >= ntasks) break;
     const hq_task tk = tasks[t];
+    if (trace && threadIdx.x == 0) {
+      trace[4 * t + 0] = t_claim;
+      trace[4 * t + 1] = wall_clock64();
+      trace[4 * t + 3] = blockIdx.x;
+    }
     run_task<NB, WB>(S, G, tk);
     __syncthreads();
     done = t;
   }
+  if (trace && threadIdx.x == 0 && done >= 0) trace[4 * done + 2] = wall_clock64();
 }
 
 // One task per launch (stream order provides the dependencies).
@@ -657,12 +766,12 @@ static hipError_t hq_dispatch(int nb, int ib, Fn &&fn) {
   constexpr int WB = decltype(wbc)::value;
 
 hipError_t hq_launch_sched(const hq_geom &G, const hq_task *tasks, int ntasks, int *flags, int epoch,
-                           int *counter, int grid, hipStream_t st) {
+                           int *counter, int grid, long long *trace, hipStream_t st) {
   hipLaunchKernelGGL(hq::reset_counter_kernel, dim3(1), dim3(64), 0, st, counter);
   return hq_dispatch(G.nb, G.ib, [&](auto nbc, auto wbc) {
     HQ_NBWB
     hipLaunchKernelGGL((hq::sched_kernel<NB, WB>), dim3(grid), dim3(HQ_THREADS), 0, st, G, tasks,
-                       ntasks, flags, epoch, counter);
+                       ntasks, flags, epoch, counter, trace);
   });
 }
 

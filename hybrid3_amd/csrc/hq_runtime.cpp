@@ -21,7 +21,7 @@
 
 // launch wrappers (hq_factor.hip)
 hipError_t hq_launch_sched(const hq_geom &G, const hq_task *tasks, int ntasks, int *flags, int epoch,
-                           int *counter, int grid, hipStream_t st);
+                           int *counter, int grid, long long *trace, hipStream_t st);
 hipError_t hq_launch_task(const hq_geom &G, const hq_task &tk, hipStream_t st);
 hipError_t hq_launch_geqrt(int nb, int ib, double *A, double *T, double *TAU, hipStream_t st);
 hipError_t hq_launch_tsqrt(int nb, int ib, double *A1, double *A2, double *T, double *TAU,
@@ -315,7 +315,7 @@ int ensure_T(Context *c, int MT, int NT, int nb, int ib, double **T) {
 }
 
 int tile_qr_dev_impl(int MT, int NT, int nb, int ib, double *A, double *TAU, double *T,
-                     void *stream, bool serial) {
+                     void *stream, bool serial, long long *trace = nullptr) {
   if (!geometry_ok(MT, NT, nb, ib)) return g_err;
   if (!A || !TAU) return set_error(HQ_ERR_ARG, "A and TAU must be device pointers");
   Context *c;
@@ -333,7 +333,7 @@ int tile_qr_dev_impl(int MT, int NT, int nb, int ib, double *A, double *TAU, dou
   if (get_schedule(c, MT, NT, nb, ib, &cs) != HQ_OK) return g_err;
   c->epoch++;
   HQ_HIP(hq_launch_sched(G, (const hq_task *)cs->tasks.p, cs->ntasks, (int *)c->flags.p, c->epoch,
-                         (int *)c->counter.p, cs->grid, st));
+                         (int *)c->counter.p, cs->grid, trace, st));
   return ok();
 }
 
@@ -373,6 +373,38 @@ int hq_tile_qr_dev(int MT, int NT, int nb, int ib, double *A, double *TAU, doubl
 int hq_tile_qr_dev_serial(int MT, int NT, int nb, int ib, double *A, double *TAU, double *T,
                           void *stream) {
   return tile_qr_dev_impl(MT, NT, nb, ib, A, TAU, T, stream, true);
+}
+
+int hq_tile_qr_dev_traced(int MT, int NT, int nb, int ib, double *A, double *TAU, double *T,
+                          long long *trace, void *stream) {
+  if (!trace) return set_error(HQ_ERR_ARG, "trace must be a device buffer of 4*ntasks int64");
+  return tile_qr_dev_impl(MT, NT, nb, ib, A, TAU, T, stream, false, trace);
+}
+
+int hq_schedule_tasks(int MT, int NT, int nb, int ib, int *out, int capacity) {
+  if (!geometry_ok(MT, NT, nb, ib)) return g_err;
+  Context *c;
+  if (get_ctx(&c) != HQ_OK) return g_err;
+  CachedSched *cs;
+  if (get_schedule(c, MT, NT, nb, ib, &cs) != HQ_OK) return g_err;
+  if (capacity < cs->ntasks * 5) return set_error(HQ_ERR_ARG, "need capacity >= 5*ntasks");
+  std::vector<hq_task> tl(cs->ntasks);
+  HQ_HIP(hipMemcpy(tl.data(), cs->tasks.p, sizeof(hq_task) * tl.size(), hipMemcpyDeviceToHost));
+  for (int i = 0; i < cs->ntasks; i++) {
+    out[5 * i + 0] = tl[i].kind;
+    out[5 * i + 1] = tl[i].k;
+    out[5 * i + 2] = tl[i].r;
+    out[5 * i + 3] = tl[i].j;
+    out[5 * i + 4] = tl[i].slab;
+  }
+  return ok();
+}
+
+int hq_wall_clock_khz(int *khz) {
+  int dev = 0;
+  HQ_HIP(hipGetDevice(&dev));
+  HQ_HIP(hipDeviceGetAttribute(khz, hipDeviceAttributeWallClockRate, dev));
+  return ok();
 }
 
 int hq_schedule_info(int MT, int NT, int nb, int ib, int *ntasks, int *grid) {

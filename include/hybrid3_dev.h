@@ -48,6 +48,21 @@ int hq_tile_qr_dev(int MT, int NT, int nb, int ib, double *A, double *TAU,
 int hq_tile_qr_dev_serial(int MT, int NT, int nb, int ib, double *A,
                           double *TAU, double *T, void *stream);
 
+/* hq_tile_qr_dev with a per-task timeline: trace (device, 4*ntasks int64)
+ * receives for task i (in claim order) the device wall-clock ticks at claim,
+ * start (dependencies satisfied) and end, and the workgroup index.  The
+ * counterpart of the reference's CRAYJ_TIMING / CRAYJ_TIMELINE builds. */
+int hq_tile_qr_dev_traced(int MT, int NT, int nb, int ib, double *A,
+                          double *TAU, double *T, long long *trace,
+                          void *stream);
+
+/* Task list in claim order: out[5*i..5*i+4] = kind (0 GEQRT, 1 TSQRT,
+ * 2 LARFB, 3 SSRFB), step k, row r, column j, slab.  capacity >= 5*ntasks. */
+int hq_schedule_tasks(int MT, int NT, int nb, int ib, int *out, int capacity);
+
+/* Frequency of the device wall clock used by the trace, in kHz. */
+int hq_wall_clock_khz(int *khz);
+
 /* Schedule of the DAG for a geometry: number of tasks and the number of
  * persistent workgroups a launch uses. */
 int hq_schedule_info(int MT, int NT, int nb, int ib, int *ntasks, int *grid);
