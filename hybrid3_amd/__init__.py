@@ -66,6 +66,10 @@ SIGNATURES = {
     "hq_tile_qr_dev_traced": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp, _vp]),
     "hq_schedule_tasks": (C.c_int, [C.c_int] * 4 + [C.POINTER(C.c_int), C.c_int]),
     "hq_wall_clock_khz": (C.c_int, [C.POINTER(C.c_int)]),
+    "hq_tile_build_t_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp]),
+    "hq_tile_ormqr_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp, C.c_int, _vp]),
+    "hq_tile_orgqr_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp, _vp]),
+    "hq_tile_make_q_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp]),
     "hq_sync": (C.c_int, [_vp]),
     "hq_finalize": (None, []),
 }
@@ -176,3 +180,23 @@ def dtsmqr_dev(nb, ib, A1, A2, V, T, stream=None):
 def sync(stream=None):
     """Wait for the stream and raise if the device scheduler reported a timeout."""
     check(lib().hq_sync(_stream(stream)))
+
+
+def tile_ormqr_dev(MT, NT, nb, ib, RY, TAU, X, XNT, T=None, stream=None):
+    """X <- Q^T X on the device (X has MT x XNT tiles)."""
+    check(lib().hq_tile_ormqr_dev(MT, NT, nb, ib, _ptr(RY), _ptr(TAU), _ptr(T), _ptr(X), XNT,
+                                  _stream(stream)))
+
+
+def tile_orgqr_dev(MT, NT, nb, ib, RY, TAU, Q, T=None, stream=None):
+    """Q <- Q^T (the reference's dorgqr output array) on the device."""
+    check(lib().hq_tile_orgqr_dev(MT, NT, nb, ib, _ptr(RY), _ptr(TAU), _ptr(T), _ptr(Q),
+                                  _stream(stream)))
+
+
+def tile_make_q_dev(MT, NT, nb, ib, RY, TAU, Q, stream=None):
+    check(lib().hq_tile_make_q_dev(MT, NT, nb, ib, _ptr(RY), _ptr(TAU), _ptr(Q), _stream(stream)))
+
+
+def tile_build_t_dev(MT, NT, nb, ib, RY, TAU, T, stream=None):
+    check(lib().hq_tile_build_t_dev(MT, NT, nb, ib, _ptr(RY), _ptr(TAU), _ptr(T), _stream(stream)))

@@ -350,7 +350,7 @@ __device__ __forceinline__ void build_t(Smem<NB> &S, int sb, double *Tg, int ib,
     if (p < ib)
       for (int c = 0; c < sb; c++) Tg[(size_t)c * ib + p] = S.TT[c * LDT + p];
   }
-  if (p < sb) taug[p] = S.tau[p];
+  if (taug && p < sb) taug[p] = S.tau[p];
 }
 
 // --------------------------------------------------------------------------
@@ -582,6 +582,9 @@ __device__ __forceinline__ void task_ssrfb(Smem<NB> &S, double *A1, double *A2, 
 __device__ __forceinline__ double *tile_ptr(const hq_geom &G, int i, int j) {
   return G.A + ((size_t)i + (size_t)j * G.MT) * (size_t)G.nb * G.nb;
 }
+__device__ __forceinline__ double *xtile_ptr(const hq_geom &G, int i, int j) {
+  return G.X + ((size_t)i + (size_t)j * G.MT) * (size_t)G.nb * G.nb;
+}
 __device__ __forceinline__ double *t_ptr(const hq_geom &G, int i, int j) {
   return G.T + ((size_t)i + (size_t)j * G.MT) * (size_t)G.ib * G.nb;
 }
@@ -600,11 +603,11 @@ __device__ __forceinline__ void run_task(Smem<NB> &S, const hq_geom &G, const hq
                          tau_ptr(G, t.r, t.k), G.ib);
       break;
     case HQ_LARFB:
-      task_larfb<NB, WB>(S, tile_ptr(G, t.k, t.k), t_ptr(G, t.k, t.k), tile_ptr(G, t.k, t.j), G.ib,
+      task_larfb<NB, WB>(S, tile_ptr(G, t.k, t.k), t_ptr(G, t.k, t.k), xtile_ptr(G, t.k, t.j), G.ib,
                          t.slab);
       break;
     case HQ_SSRFB:
-      task_ssrfb<NB, WB>(S, tile_ptr(G, t.k, t.j), tile_ptr(G, t.r, t.j), tile_ptr(G, t.r, t.k),
+      task_ssrfb<NB, WB>(S, xtile_ptr(G, t.k, t.j), xtile_ptr(G, t.r, t.j), tile_ptr(G, t.r, t.k),
                          t_ptr(G, t.r, t.k), G.ib, t.slab);
       break;
     default:
@@ -706,6 +709,74 @@ __global__ __launch_bounds__(HQ_THREADS) void ssrfb_kernel(double *A1, double *A
                                                           const double *T, int ib) {
   __shared__ Smem<NB> S;
   task_ssrfb<NB, WB>(S, A1, A2, V, T, ib, blockIdx.x);
+}
+
+// Rebuild the ib x nb T factor of every reflector tile (r, k), r >= k, from
+// V and TAU (LAPACK dlarft('F','C') per ib block), for applying Q when only
+// RY and TAU are given (the reference rebuilds T the same way in
+// Hybrid_dorgqr.c:84,118 / Hybrid_make_Q.c:124-130,159-163).  Diagonal tiles
+// use the unit lower part of the tile; their last reflector is dropped, as
+// the reference does by zeroing Y(nb-1,nb-1) (Hybrid_dorgqr.c:83).
+// grid.x = ib block, grid.y = flattened lower tile index.
+template <int NB, int WB>
+__global__ __launch_bounds__(HQ_THREADS) void tbuild_kernel(hq_geom G, const int2 *tiles) {
+  __shared__ Smem<NB> S;
+  constexpr int LDV = Smem<NB>::LDV;
+  constexpr int IBP = 16 * WB;
+  const int2 rk = tiles[blockIdx.y];
+  const int r = rk.x, k = rk.y;
+  const int ib = G.ib, i0 = blockIdx.x * ib;
+  if (i0 >= NB) return;
+  const int kb = min(ib, NB - i0);
+  const bool diag = (r == k);
+  const double *V = tile_ptr(G, r, k);
+  const double *tau = tau_ptr(G, r, k) + i0;
+  for (int e = threadIdx.x; e < IBP * NB; e += HQ_THREADS) {
+    const int c = e / NB, rr = e - c * NB;
+    double val = 0.0;
+    if (c < kb) {
+      const int d = i0 + c;
+      val = V[(size_t)d * NB + rr];
+      if (diag) val = rr > d ? val : (rr == d ? 1.0 : 0.0);
+    }
+    S.V[c * LDV + rr] = val;
+  }
+  if (threadIdx.x < HQ_IBMAX) {
+    const int c = threadIdx.x;
+    double tv = c < kb ? tau[c] : 0.0;
+    if (diag && i0 + c == NB - 1) tv = 0.0;
+    S.tau[c] = tv;
+  }
+  __syncthreads();
+  gram<NB, WB>(S, diag ? i0 / 16 : 0);
+  __syncthreads();
+  if (threadIdx.x < 64) build_t<NB>(S, kb, t_ptr(G, r, k) + (size_t)i0 * ib, ib, nullptr);
+}
+
+// X <- identity (full = true: every entry written) or only ones on the
+// diagonal of the diagonal tiles (Hybrid_make_Q.c:91-95 writes only those).
+__global__ void identity_kernel(double *X, int MT, int XNT, int nb, int full) {
+  const size_t ts = (size_t)nb * nb, total = ts * (size_t)MT * XNT;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    size_t t = e / ts, w = e - t * ts;
+    size_t ti = t % MT, tj = t / MT, c = w / nb, rr = w - c * nb;
+    bool on = (ti == tj) && (c == rr);
+    if (full || on) X[e] = on ? 1.0 : 0.0;
+  }
+}
+
+// Zero the reflectors of RY (below-diagonal part of the diagonal tiles and
+// every tile below them), leaving R (Hybrid_make_Q.c:148-150,179-182).
+__global__ void zero_reflectors_kernel(double *A, int MT, int NT, int nb) {
+  const size_t ts = (size_t)nb * nb, total = ts * (size_t)MT * NT;
+  const int KT = MT < NT ? MT : NT;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    size_t t = e / ts, w = e - t * ts;
+    size_t ti = t % MT, tj = t / MT, c = w / nb, rr = w - c * nb;
+    if (tj < (size_t)KT && (ti > tj || (ti == tj && rr > c))) A[e] = 0.0;
+  }
 }
 
 __global__ void reset_counter_kernel(int *counter) {
@@ -811,6 +882,31 @@ hipError_t hq_launch_ssrfb(int nb, int ib, double *A1, double *A2, const double 
     hipLaunchKernelGGL((hq::ssrfb_kernel<NB, WB>), dim3((NB + HQ_SLAB - 1) / HQ_SLAB),
                        dim3(HQ_THREADS), 0, st, A1, A2, V, T, ib);
   });
+}
+
+hipError_t hq_launch_tbuild(const hq_geom &G, const int2 *tiles, int ntiles, hipStream_t st) {
+  return hq_dispatch(G.nb, G.ib, [&](auto nbc, auto wbc) {
+    HQ_NBWB
+    hipLaunchKernelGGL((hq::tbuild_kernel<NB, WB>), dim3((NB + G.ib - 1) / G.ib, ntiles),
+                       dim3(HQ_THREADS), 0, st, G, tiles);
+  });
+}
+
+static unsigned grid_for(size_t total) {
+  size_t blocks = (total + 255) / 256;
+  return (unsigned)(blocks > 65536 ? 65536 : (blocks ? blocks : 1));
+}
+
+hipError_t hq_launch_identity(double *X, int MT, int XNT, int nb, int full, hipStream_t st) {
+  hipLaunchKernelGGL(hq::identity_kernel, dim3(grid_for((size_t)MT * XNT * nb * nb)), dim3(256), 0,
+                     st, X, MT, XNT, nb, full);
+  return hipGetLastError();
+}
+
+hipError_t hq_launch_zero_reflectors(double *A, int MT, int NT, int nb, hipStream_t st) {
+  hipLaunchKernelGGL(hq::zero_reflectors_kernel, dim3(grid_for((size_t)MT * NT * nb * nb)),
+                     dim3(256), 0, st, A, MT, NT, nb);
+  return hipGetLastError();
 }
 
 hipError_t hq_launch_fill_uniform(double *A, int MT, int NT, int nb, uint64_t seed, hipStream_t st) {

@@ -34,10 +34,13 @@ struct hq_task {
 //   TAU  (i,j):  TAU + (i + j*MT) * nb
 //   T    (i,j):  T   + (i + j*MT) * ib*nb      (ld = ib; block b at column b*ib)
 struct hq_geom {
-  double *A;
+  double *A;    // reflector tiles (the factorised matrix RY)
   double *TAU;
   double *T;
   int MT, NT, nb, ib;
+  double *X;    // matrix updated by LARFB / SSRFB: A itself while factorising,
+                // the Q / C array (MT x XNT tiles) while applying Q^T
+  int XNT;
 };
 
 #define HQ_THREADS 256

@@ -31,6 +31,9 @@ hipError_t hq_launch_larfb(int nb, int ib, const double *V, const double *T, dou
 hipError_t hq_launch_ssrfb(int nb, int ib, double *A1, double *A2, const double *V, const double *T,
                            hipStream_t st);
 hipError_t hq_launch_fill_uniform(double *A, int MT, int NT, int nb, uint64_t seed, hipStream_t st);
+hipError_t hq_launch_tbuild(const hq_geom &G, const int2 *tiles, int ntiles, hipStream_t st);
+hipError_t hq_launch_identity(double *X, int MT, int XNT, int nb, int full, hipStream_t st);
+hipError_t hq_launch_zero_reflectors(double *A, int MT, int NT, int nb, hipStream_t st);
 
 namespace {
 
@@ -80,22 +83,30 @@ struct Schedule {
   int grid = 0;
 };
 
+// Apply mode (Q^T X, the sweep of Hybrid_dorgqr.c:92-138 / Hybrid_dormqr.c:
+// 88-127 / Hybrid_make_Q.c:139-178): no panel tasks; every tile column j of X
+// (0 <= j < XNT) gets
+//   LARFB(k,j,s)   MEM: SSRFB(k-1,j,k,s)
+//   SSRFB(k,j,r,s) LIN: SSRFB(k,j,r-1,s) | LARFB(k,j,s); MEM: SSRFB(k-1,j,r,s)
 struct DagBuilder {
   int MT, NT, nb, ib, S, KT;
+  bool apply;
+  int XNT;
   std::vector<hq_task> t;
   std::vector<int> idg, idt, idl, ids;
-  DagBuilder(int MT_, int NT_, int nb_, int ib_) : MT(MT_), NT(NT_), nb(nb_), ib(ib_) {
+  DagBuilder(int MT_, int NT_, int nb_, int ib_, bool apply_ = false, int XNT_ = 0)
+      : MT(MT_), NT(NT_), nb(nb_), ib(ib_), apply(apply_), XNT(apply_ ? XNT_ : NT_) {
     S = (nb + HQ_SLAB - 1) / HQ_SLAB;
     KT = std::min(MT, NT);
     idg.assign(KT, -1);
     idt.assign((size_t)KT * MT, -1);
-    idl.assign((size_t)KT * NT * S, -1);
-    ids.assign((size_t)KT * NT * MT * S, -1);
+    idl.assign((size_t)KT * XNT * S, -1);
+    ids.assign((size_t)KT * XNT * MT * S, -1);
   }
   int &G(int k) { return idg[k]; }
   int &Tq(int k, int r) { return idt[(size_t)k * MT + r]; }
-  int &L(int k, int j, int s) { return idl[((size_t)k * NT + j) * S + s]; }
-  int &Sr(int k, int j, int r, int s) { return ids[(((size_t)k * NT + j) * MT + r) * S + s]; }
+  int &L(int k, int j, int s) { return idl[((size_t)k * XNT + j) * S + s]; }
+  int &Sr(int k, int j, int r, int s) { return ids[(((size_t)k * XNT + j) * MT + r) * S + s]; }
   int add(int kind, int k, int r, int j, int s) {
     hq_task x{};
     x.kind = kind;
@@ -116,7 +127,27 @@ struct DagBuilder {
     }
     x.dep[x.ndep++] = d;
   }
+  void build_apply() {
+    for (int k = 0; k < KT; k++) {
+      for (int j = 0; j < XNT; j++)
+        for (int s = 0; s < S; s++) {
+          int l = L(k, j, s) = add(HQ_LARFB, k, k, j, s);
+          if (k > 0) dep(l, Sr(k - 1, j, k, s));
+        }
+      for (int r = k + 1; r < MT; r++)
+        for (int j = 0; j < XNT; j++)
+          for (int s = 0; s < S; s++) {
+            int x = Sr(k, j, r, s) = add(HQ_SSRFB, k, r, j, s);
+            dep(x, r == k + 1 ? L(k, j, s) : Sr(k, j, r - 1, s));
+            if (k > 0) dep(x, Sr(k - 1, j, r, s));
+          }
+    }
+  }
   void build() {
+    if (apply) {
+      build_apply();
+      return;
+    }
     for (int k = 0; k < KT; k++) {
       int g = G(k) = add(HQ_GEQRT, k, k, k, 0);
       if (k > 0)
@@ -231,9 +262,10 @@ struct Context {
   int device = -1;
   int num_cu = 0;
   hipStream_t stream = nullptr;
-  DevBuf flags, counter, T, A, TAU;
+  DevBuf flags, counter, T, A, TAU, X;
   int epoch = 0;
-  std::map<std::tuple<int, int, int, int>, std::unique_ptr<CachedSched>> scheds;
+  std::map<std::tuple<int, int, int, int, int>, std::unique_ptr<CachedSched>> scheds;
+  std::map<std::pair<int, int>, std::unique_ptr<DevBuf>> tilelists;  // (MT, NT) -> int2 list
 };
 
 std::mutex g_mu;
@@ -260,14 +292,15 @@ int get_ctx(Context **out) {
 // entry points pass the library's own stream explicitly.
 hipStream_t pick_stream(Context *, void *s) { return (hipStream_t)s; }
 
-int get_schedule(Context *c, int MT, int NT, int nb, int ib, CachedSched **out) {
-  auto key = std::make_tuple(MT, NT, nb, ib);
+int get_schedule(Context *c, int MT, int NT, int nb, int ib, CachedSched **out, int XNT = -1) {
+  // XNT < 0: factorisation DAG; XNT >= 1: Q^T application to MT x XNT tiles
+  auto key = std::make_tuple(MT, NT, nb, ib, XNT);
   auto it = c->scheds.find(key);
   if (it != c->scheds.end()) {
     *out = it->second.get();
     return HQ_OK;
   }
-  DagBuilder b(MT, NT, nb, ib);
+  DagBuilder b(MT, NT, nb, ib, XNT >= 1, XNT);
   b.build();
   int grid = c->num_cu > 0 ? c->num_cu : 256;
   std::vector<int> ord = b.order(grid);
@@ -322,7 +355,7 @@ int tile_qr_dev_impl(int MT, int NT, int nb, int ib, double *A, double *TAU, dou
   if (get_ctx(&c) != HQ_OK) return g_err;
   hipStream_t st = pick_stream(c, stream);
   if (ensure_T(c, MT, NT, nb, ib, &T) != HQ_OK) return g_err;
-  hq_geom G{A, TAU, T, MT, NT, nb, ib};
+  hq_geom G{A, TAU, T, MT, NT, nb, ib, A, NT};
   if (serial) {
     std::vector<hq_task> tl;
     host_schedule(MT, NT, nb, ib, c->num_cu, tl);
@@ -334,6 +367,56 @@ int tile_qr_dev_impl(int MT, int NT, int nb, int ib, double *A, double *TAU, dou
   c->epoch++;
   HQ_HIP(hq_launch_sched(G, (const hq_task *)cs->tasks.p, cs->ntasks, (int *)c->flags.p, c->epoch,
                          (int *)c->counter.p, cs->grid, trace, st));
+  return ok();
+}
+
+int tile_list(Context *c, int MT, int NT, const int2 **out, int *n) {
+  auto key = std::make_pair(MT, NT);
+  auto &slot = c->tilelists[key];
+  std::vector<int2> v;
+  int KT = std::min(MT, NT);
+  for (int k = 0; k < KT; k++)
+    for (int r = k; r < MT; r++) v.push_back(make_int2(r, k));
+  if (!slot) {
+    slot.reset(new DevBuf());
+    if (slot->reserve(sizeof(int2) * v.size()) != HQ_OK) return g_err;
+    HQ_HIP(hipMemcpy(slot->p, v.data(), sizeof(int2) * v.size(), hipMemcpyHostToDevice));
+  }
+  *out = (const int2 *)slot->p;
+  *n = (int)v.size();
+  return HQ_OK;
+}
+
+int build_t_impl(Context *c, int MT, int NT, int nb, int ib, const double *RY, const double *TAU,
+                 double *T, hipStream_t st) {
+  const int2 *tiles;
+  int n;
+  if (tile_list(c, MT, NT, &tiles, &n) != HQ_OK) return g_err;
+  hq_geom G{(double *)RY, (double *)TAU, T, MT, NT, nb, ib, nullptr, 0};
+  HQ_HIP(hq_launch_tbuild(G, tiles, n, st));
+  return HQ_OK;
+}
+
+// X (MT x XNT tiles) <- Q^T X, Q from the reflectors of RY (MT x NT tiles).
+int apply_qt_impl(int MT, int NT, int nb, int ib, const double *RY, const double *TAU,
+                  const double *T, double *X, int XNT, void *stream) {
+  if (!geometry_ok(MT, NT, nb, ib)) return g_err;
+  if (XNT < 1) return set_error(HQ_ERR_ARG, "X needs at least one tile column");
+  if (!RY || !TAU || !X) return set_error(HQ_ERR_ARG, "RY, TAU and X must be device pointers");
+  Context *c;
+  if (get_ctx(&c) != HQ_OK) return g_err;
+  hipStream_t st = pick_stream(c, stream);
+  double *Tw = (double *)T;
+  if (!Tw) {
+    if (ensure_T(c, MT, NT, nb, ib, &Tw) != HQ_OK) return g_err;
+    if (build_t_impl(c, MT, NT, nb, ib, RY, TAU, Tw, st) != HQ_OK) return g_err;
+  }
+  CachedSched *cs;
+  if (get_schedule(c, MT, NT, nb, ib, &cs, XNT) != HQ_OK) return g_err;
+  hq_geom G{(double *)RY, (double *)TAU, Tw, MT, NT, nb, ib, X, XNT};
+  c->epoch++;
+  HQ_HIP(hq_launch_sched(G, (const hq_task *)cs->tasks.p, cs->ntasks, (int *)c->flags.p, c->epoch,
+                         (int *)c->counter.p, cs->grid, nullptr, st));
   return ok();
 }
 
@@ -459,6 +542,41 @@ int hq_dtsmqr_dev(int nb, int ib, double *A1, double *A2, const double *V, const
   return ok();
 }
 
+int hq_tile_build_t_dev(int MT, int NT, int nb, int ib, const double *RY, const double *TAU,
+                        double *T, void *stream) {
+  if (!geometry_ok(MT, NT, nb, ib)) return g_err;
+  Context *c;
+  if (get_ctx(&c) != HQ_OK) return g_err;
+  if (build_t_impl(c, MT, NT, nb, ib, RY, TAU, T, pick_stream(c, stream)) != HQ_OK) return g_err;
+  return ok();
+}
+
+int hq_tile_ormqr_dev(int MT, int NT, int nb, int ib, const double *RY, const double *TAU,
+                      const double *T, double *X, int XNT, void *stream) {
+  return apply_qt_impl(MT, NT, nb, ib, RY, TAU, T, X, XNT, stream);
+}
+
+int hq_tile_orgqr_dev(int MT, int NT, int nb, int ib, const double *RY, const double *TAU,
+                      const double *T, double *Q, void *stream) {
+  if (!geometry_ok(MT, NT, nb, ib)) return g_err;
+  Context *c;
+  if (get_ctx(&c) != HQ_OK) return g_err;
+  HQ_HIP(hq_launch_identity(Q, MT, NT, nb, 1, pick_stream(c, stream)));
+  return apply_qt_impl(MT, NT, nb, ib, RY, TAU, T, Q, NT, stream);
+}
+
+int hq_tile_make_q_dev(int MT, int NT, int nb, int ib, double *RY, const double *TAU, double *Q,
+                       void *stream) {
+  if (!geometry_ok(MT, NT, nb, ib)) return g_err;
+  Context *c;
+  if (get_ctx(&c) != HQ_OK) return g_err;
+  hipStream_t st = pick_stream(c, stream);
+  HQ_HIP(hq_launch_identity(Q, MT, NT, nb, 0, st));
+  if (apply_qt_impl(MT, NT, nb, ib, RY, TAU, nullptr, Q, NT, stream) != HQ_OK) return g_err;
+  HQ_HIP(hq_launch_zero_reflectors(RY, MT, NT, nb, st));
+  return ok();
+}
+
 int hq_fill_uniform_dev(int MT, int NT, int nb, uint64_t seed, double *A, void *stream) {
   if (MT < 1 || NT < 1 || nb < 1 || !A) return set_error(HQ_ERR_ARG, "bad fill arguments");
   Context *c;
@@ -542,6 +660,70 @@ void Hybrid_tile_QR(mat_stat_t *status, double *A, double *TAU, int my_rank, int
   };
   run();
   report("Hybrid_tile_QR");
+}
+
+}  // extern "C"
+
+// Shared body of the host-pointer Q entry points: copies RY, TAU and X to
+// the device, runs `fn`, copies X (and RY when `ry_out`) back.
+template <typename Fn>
+static int host_q_call(mat_stat_t *status, double *X, double *RY, double *TAU, int my_rank,
+                       int Proc_num, bool x_in, bool ry_out, Fn &&fn) {
+  if (check_rank(my_rank, Proc_num) != HQ_OK) return g_err;
+  if (!status || !X || !RY || !TAU || status->tile_M != status->tile_N)
+    return set_error(HQ_ERR_ARG, "need square tiles and non-null arrays");
+  const int MT = (int)status->glob_tile_M, NT = (int)status->glob_tile_N;
+  const int nb = (int)status->tile_M;
+  const int ib = std::min(nb, HQ_IBMAX);
+  if (!geometry_ok(MT, NT, nb, ib)) return g_err;
+  Context *c;
+  if (get_ctx(&c) != HQ_OK) return g_err;
+  const size_t nA = (size_t)MT * NT * nb * nb, nT = (size_t)MT * NT * nb;
+  if (c->A.reserve(nA * sizeof(double)) != HQ_OK) return g_err;
+  if (c->TAU.reserve(nT * sizeof(double)) != HQ_OK) return g_err;
+  if (c->X.reserve(nA * sizeof(double)) != HQ_OK) return g_err;
+  double *dRY = (double *)c->A.p, *dTAU = (double *)c->TAU.p, *dX = (double *)c->X.p;
+  HQ_HIP(hipMemcpyAsync(dRY, RY, nA * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HQ_HIP(hipMemcpyAsync(dTAU, TAU, nT * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (x_in) HQ_HIP(hipMemcpyAsync(dX, X, nA * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (fn(MT, NT, nb, ib, dRY, dTAU, dX, c->stream) != HQ_OK) return g_err;
+  HQ_HIP(hipMemcpyAsync(X, dX, nA * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (ry_out) HQ_HIP(hipMemcpyAsync(RY, dRY, nA * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HQ_HIP(hipStreamSynchronize(c->stream));
+  return ok();
+}
+
+extern "C" {
+
+void Hybrid_dorgqr(mat_stat_t *status, double *Q, double *RY, double *TAU, int my_rank,
+                   int Proc_num) {
+  host_q_call(status, Q, RY, TAU, my_rank, Proc_num, false, false,
+              [](int MT, int NT, int nb, int ib, double *dRY, double *dTAU, double *dX,
+                 hipStream_t st) {
+                return hq_tile_orgqr_dev(MT, NT, nb, ib, dRY, dTAU, nullptr, dX, st);
+              });
+  report("Hybrid_dorgqr");
+}
+
+void Hybrid_dormqr(mat_stat_t *status, double *RY, double *TAU, double *C, int my_rank,
+                   int Proc_num) {
+  host_q_call(status, C, RY, TAU, my_rank, Proc_num, true, false,
+              [](int MT, int NT, int nb, int ib, double *dRY, double *dTAU, double *dX,
+                 hipStream_t st) {
+                return hq_tile_ormqr_dev(MT, NT, nb, ib, dRY, dTAU, nullptr, dX, NT, st);
+              });
+  report("Hybrid_dormqr");
+}
+
+void Hybrid_make_Q(mat_stat_t *status, double *Q, double *RY, double *TAU, int my_rank,
+                   int Proc_num, int ib) {
+  (void)ib;  // the reference applies full-width T here (ib unused, Hybrid_make_Q.c:124-178)
+  host_q_call(status, Q, RY, TAU, my_rank, Proc_num, true, true,
+              [](int MT, int NT, int nb, int ib2, double *dRY, double *dTAU, double *dX,
+                 hipStream_t st) {
+                return hq_tile_make_q_dev(MT, NT, nb, ib2, dRY, dTAU, dX, st);
+              });
+  report("Hybrid_make_Q");
 }
 
 }  // extern "C"

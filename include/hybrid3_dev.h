@@ -81,6 +81,30 @@ int hq_dormqr_dev(int nb, int ib, const double *V, const double *T, double *C,
 int hq_dtsmqr_dev(int nb, int ib, double *A1, double *A2, const double *V,
                   const double *T, void *stream);
 
+/* T factors (ib x nb per tile, layout above) of every reflector tile (r,k),
+ * r >= k, rebuilt from RY and TAU (LAPACK dlarft per ib block); the last
+ * reflector of each diagonal tile is dropped, as Hybrid_dorgqr.c:83 does. */
+int hq_tile_build_t_dev(int MT, int NT, int nb, int ib, const double *RY,
+                        const double *TAU, double *T, void *stream);
+
+/* X (MT x XNT tiles) <- Q^T X with Q = H(0,0) H(1,0) ... from the tile QR
+ * in RY / TAU (MT x NT tiles); T may be NULL (rebuilt).  The sweep of
+ * Hybrid_dormqr.c:64-167 on the device DAG scheduler. */
+int hq_tile_ormqr_dev(int MT, int NT, int nb, int ib, const double *RY,
+                      const double *TAU, const double *T, double *X, int XNT,
+                      void *stream);
+
+/* Q (MT x NT tiles) <- Q^T (the array Hybrid_dorgqr returns, Hybrid_dorgqr.c:
+ * 56-138): identity, then hq_tile_ormqr_dev. */
+int hq_tile_orgqr_dev(int MT, int NT, int nb, int ib, const double *RY,
+                      const double *TAU, const double *T, double *Q,
+                      void *stream);
+
+/* Hybrid_make_Q semantics (Hybrid_make_Q.c:56-237): ones on the diagonal of
+ * Q (other entries kept), Q <- Q^T-apply, then RY's reflectors zeroed. */
+int hq_tile_make_q_dev(int MT, int NT, int nb, int ib, double *RY,
+                       const double *TAU, double *Q, void *stream);
+
 /* A (tile layout, MT x NT tiles of nb) <- uniform[-1,1) from SplitMix64 of
  * (seed, column-major global index). */
 int hq_fill_uniform_dev(int MT, int NT, int nb, uint64_t seed, double *A,

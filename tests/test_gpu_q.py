@@ -1,0 +1,155 @@
+"""GPU parity of Q generation / application (Hybrid_dorgqr.c, Hybrid_dormqr.c,
+Hybrid_make_Q.c semantics: the Q array holds Q^T in tile layout) against the
+CPU oracle, through the C-ABI, plus the north-star norms."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def hq():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import hybrid3_amd
+    hybrid3_amd.lib()
+    return hybrid3_amd
+
+
+def factor(hq, orc, MT, NT, nb, ib, seed):
+    import torch
+    A = orc.fill_uniform_cm(MT * nb, NT * nb, seed)
+    At = orc.cm_to_tile(A, nb)
+    dA = torch.from_numpy(At.copy()).to("cuda")
+    dTAU = torch.zeros(MT * NT * nb, dtype=torch.float64, device="cuda")
+    dT = torch.zeros(MT * NT * ib * nb, dtype=torch.float64, device="cuda")
+    hq.tile_qr_dev(MT, NT, nb, ib, dA, dTAU, dT)
+    hq.sync()
+    return A, dA, dTAU, dT
+
+
+GEOMS = [(3, 3, 64, 16), (4, 4, 128, 32), (5, 3, 64, 16), (2, 2, 256, 32)]
+
+
+@pytest.mark.parametrize("MT,NT,nb,ib", GEOMS)
+def test_build_t_matches_factorisation_t(hq, orc, MT, NT, nb, ib):
+    import torch
+    print(f"build_t {MT} {NT} {nb} {ib}", flush=True)
+    A, dA, dTAU, dT = factor(hq, orc, MT, NT, nb, ib, 3)
+    dT2 = torch.zeros_like(dT)
+    hq.tile_build_t_dev(MT, NT, nb, ib, dA, dTAU, dT2)
+    hq.sync()
+    T1, T2 = dT.cpu().numpy(), dT2.cpu().numpy()
+    assert np.abs(T1 - T2).max() / np.abs(T1).max() < 1e-12
+
+
+@pytest.mark.parametrize("MT,NT,nb,ib", GEOMS)
+def test_orgqr_ormqr_make_q_match_oracle(hq, orc, MT, NT, nb, ib):
+    import torch
+    print(f"orgqr {MT} {NT} {nb} {ib}", flush=True)
+    A, dA, dTAU, dT = factor(hq, orc, MT, NT, nb, ib, 4)
+    RY, TAU = dA.cpu().numpy(), dTAU.cpu().numpy()
+    # dorgqr: Q array <- Q^T
+    dQ = torch.empty_like(dA)
+    hq.tile_orgqr_dev(MT, NT, nb, ib, dA, dTAU, dQ)
+    hq.sync()
+    Qo = orc.dorgqr(RY, TAU, MT, NT, nb)
+    assert np.abs(dQ.cpu().numpy() - Qo).max() < 1e-12
+    # with the factorisation's own T: same result
+    dQ2 = torch.empty_like(dA)
+    hq.tile_orgqr_dev(MT, NT, nb, ib, dA, dTAU, dQ2, T=dT)
+    hq.sync()
+    assert np.abs(dQ2.cpu().numpy() - Qo).max() < 1e-12
+    # dormqr: C <- Q^T C
+    Cm = orc.fill_uniform_cm(MT * nb, NT * nb, 99)
+    Ct = orc.cm_to_tile(Cm, nb)
+    dC = torch.from_numpy(Ct.copy()).to("cuda")
+    hq.tile_ormqr_dev(MT, NT, nb, ib, dA, dTAU, dC, NT)
+    hq.sync()
+    Co = orc.dormqr(RY, TAU, Ct, MT, NT, nb)
+    assert np.abs(dC.cpu().numpy() - Co).max() / np.abs(Co).max() < 1e-12
+    # make_Q: Q zero + diagonal ones, reflectors of RY zeroed
+    dQ3 = torch.zeros_like(dA)
+    dRY = dA.clone()
+    hq.tile_make_q_dev(MT, NT, nb, ib, dRY, dTAU, dQ3)
+    hq.sync()
+    Qm, RYm = orc.make_Q(RY, TAU, MT, NT, nb)
+    assert np.abs(dQ3.cpu().numpy() - Qm).max() < 1e-12
+    assert np.array_equal(dRY.cpu().numpy(), RYm)
+
+
+def test_ormqr_on_narrow_block(hq, orc):
+    """C with fewer tile columns than RY (apply Q^T to a block of right-hand sides)."""
+    import torch
+    MT, NT, nb, ib = 4, 2, 64, 16
+    A, dA, dTAU, dT = factor(hq, orc, MT, NT, nb, ib, 5)
+    Cm = orc.fill_uniform_cm(MT * nb, nb, 7)
+    dC = torch.from_numpy(orc.cm_to_tile(Cm, nb)).to("cuda")
+    hq.tile_ormqr_dev(MT, NT, nb, ib, dA, dTAU, dC, 1)
+    hq.sync()
+    out = orc.tile_to_cm(dC.cpu().numpy(), MT, 1, nb)
+    # explicit Q^T from the oracle's Hybrid_make_Q on the GPU factors (square only
+    # forms all of Q^T when MT == NT), so check against numpy: Q^T C = R-part logic
+    Q, _ = np.linalg.qr(A, mode="complete")
+    R = np.triu(orc.tile_to_cm(dA.cpu().numpy(), MT, NT, nb))
+    # first NT*nb rows of Q^T C are determined up to the row signs of R
+    QtC_top = out[: NT * nb]
+    s = np.sign(np.diag(R[: NT * nb]))
+    Qn = Q[:, : NT * nb] * (s * np.sign(np.diag(np.linalg.qr(A, mode="r"))))[None, :]
+    assert np.abs(QtC_top - Qn.T @ Cm).max() < 1e-12
+    # and Q^T is orthogonal: column norms are kept
+    assert np.allclose(np.linalg.norm(out, axis=0), np.linalg.norm(Cm, axis=0), rtol=1e-13)
+
+
+def test_config1_north_star_norms(hq, orc):
+    """2048^2, nb=128, ib=32: ||A-QR||_F/||A||_F and ||I-Q^T Q||_F with Q from
+    the GPU dorgqr path."""
+    import torch
+    MT = NT = 16
+    nb, ib = 128, 32
+    A, dA, dTAU, dT = factor(hq, orc, MT, NT, nb, ib, 20151130)
+    dQ = torch.empty_like(dA)
+    hq.tile_orgqr_dev(MT, NT, nb, ib, dA, dTAU, dQ)
+    hq.sync()
+    Qt = orc.tile_to_cm(dQ.cpu().numpy(), MT, NT, nb)
+    Q = Qt.T
+    R = np.triu(orc.tile_to_cm(dA.cpu().numpy(), MT, NT, nb))
+    res = np.linalg.norm(A - Q @ R) / np.linalg.norm(A)
+    orth = np.linalg.norm(np.eye(2048) - Q.T @ Q)
+    print(f"||A-QR||/||A|| = {res:.3e}  ||I-QtQ|| = {orth:.3e}")
+    assert res < 1e-14 and orth < 1e-12
+
+
+def test_host_pointer_entry_points(hq, orc):
+    """Hybrid_tile_QR / Hybrid_dorgqr / Hybrid_dormqr / Hybrid_make_Q on host
+    arrays (the reference's calling convention, main.c:148,175)."""
+    L = hq.lib()
+    MT = NT = 4
+    nb, ib = 64, 16
+    n = MT * nb
+    st = hq.mat_init(n, n, nb)
+    A = orc.cm_to_tile(orc.fill_uniform_cm(n, n, 11), nb)
+    RY = A.copy()
+    TAU = np.zeros((n // 2) ** 2)  # the reference allocates (n/2)^2 (main.c:114)
+    dp = C.POINTER(C.c_double)
+    p = lambda a: a.ctypes.data_as(dp)
+    L.Hybrid_tile_QR(C.byref(st), p(RY), p(TAU), 0, 1, ib)
+    assert hq.last_error()[0] == 0
+    RYo, TAUo = orc.tile_qr(A, MT, NT, nb, ib)
+    assert np.abs(RY - RYo).max() < 1e-12 and np.abs(TAU[: MT * NT * nb] - TAUo).max() < 1e-12
+    Q = np.zeros_like(RY)
+    L.Hybrid_dorgqr(C.byref(st), p(Q), p(RY), p(TAU), 0, 1)
+    assert hq.last_error()[0] == 0
+    assert np.abs(Q - orc.dorgqr(RYo, TAUo, MT, NT, nb)).max() < 1e-12
+    Cm = orc.cm_to_tile(orc.fill_uniform_cm(n, n, 12), nb)
+    Cc = Cm.copy()
+    L.Hybrid_dormqr(C.byref(st), p(RY), p(TAU), p(Cc), 0, 1)
+    assert np.abs(Cc - orc.dormqr(RYo, TAUo, Cm, MT, NT, nb)).max() < 1e-12
+    Q2 = np.zeros_like(RY)
+    RY2 = RY.copy()
+    L.Hybrid_make_Q(C.byref(st), p(Q2), p(RY2), p(TAU), 0, 1, ib)
+    Qm, RYm = orc.make_Q(RYo, TAUo, MT, NT, nb)
+    assert np.abs(Q2 - Qm).max() < 1e-12 and np.abs(RY2 - RYm).max() < 1e-12
