@@ -70,6 +70,8 @@ SIGNATURES = {
     "hq_tile_ormqr_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp, C.c_int, _vp]),
     "hq_tile_orgqr_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp, _vp]),
     "hq_tile_make_q_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp]),
+    "hq_dag_host": (C.c_int, [C.c_int] * 6 + [C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int,
+                                             C.POINTER(C.c_int)]),
     "hq_sync": (C.c_int, [_vp]),
     "hq_finalize": (None, []),
 }
@@ -200,3 +202,16 @@ def tile_make_q_dev(MT, NT, nb, ib, RY, TAU, Q, stream=None):
 
 def tile_build_t_dev(MT, NT, nb, ib, RY, TAU, T, stream=None):
     check(lib().hq_tile_build_t_dev(MT, NT, nb, ib, _ptr(RY), _ptr(TAU), _ptr(T), _stream(stream)))
+
+
+def dag_host(MT, NT, nb, ib, XNT=0, P=256):
+    """Task DAG in claim order (no GPU): (tasks[n,5], deps[n,6] with -1 padding)."""
+    L = lib()
+    n = C.c_int(0)
+    check(L.hq_dag_host(MT, NT, nb, ib, XNT, P, None, None, 0, C.byref(n)))
+    tasks = np.zeros(5 * n.value, dtype=np.int32)
+    deps = np.zeros(6 * n.value, dtype=np.int32)
+    ip = C.POINTER(C.c_int)
+    check(L.hq_dag_host(MT, NT, nb, ib, XNT, P, tasks.ctypes.data_as(ip), deps.ctypes.data_as(ip),
+                        n.value, C.byref(n)))
+    return tasks.reshape(-1, 5), deps.reshape(-1, 6)

@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libhybrid3.so")
-SOURCES = ["hq_factor.hip", "hq_runtime.cpp"]
+SOURCES = ["hq_factor.hip", "hq_homegrown.hip", "hq_runtime.cpp"]
 ARCH = os.environ.get("HQ_OFFLOAD_ARCH", "gfx950")
 
 # Packed FP32 ops are switched off for every device compile (the hardware

@@ -35,6 +35,21 @@ hipError_t hq_launch_tbuild(const hq_geom &G, const int2 *tiles, int ntiles, hip
 hipError_t hq_launch_identity(double *X, int MT, int XNT, int nb, int full, hipStream_t st);
 hipError_t hq_launch_zero_reflectors(double *A, int MT, int NT, int nb, hipStream_t st);
 
+// homegrown tile_kernel.h API and verification helpers (hq_homegrown.hip)
+hipError_t hqh_launch_dgeqrt(int row, int col, int block, double *A, int lda, double *TAU,
+                             double *Y, double *T, int ldt, double *work, hipStream_t st);
+hipError_t hqh_launch_dtsqrt(int row, int col, int block, double *A1, int lda1, double *A2,
+                             int lda2, double *TAU, double *T, int ldt, double *work,
+                             hipStream_t st);
+hipError_t hqh_launch_dlarfb(int row, int col, double *A, int lda, const double *Y,
+                             const double *T, int ldt, double *work, hipStream_t st);
+hipError_t hqh_launch_dssrfb(int row, int col, double *A1, double *A2, int lda, const double *Y,
+                             const double *T, int ldt, double *work, hipStream_t st);
+hipError_t hqh_launch_dlarft(int row, int col, int block, const double *Y, int lda,
+                             const double *TAU, double *T, int ldt, double *work, hipStream_t st);
+hipError_t hqh_launch_tile_gemm_tn(const double *A, const double *B, double *Cm, int MT, int NT,
+                                   int nb, hipStream_t st);
+
 namespace {
 
 // ---------------------------------------------------------------- errors
@@ -262,7 +277,7 @@ struct Context {
   int device = -1;
   int num_cu = 0;
   hipStream_t stream = nullptr;
-  DevBuf flags, counter, T, A, TAU, X;
+  DevBuf flags, counter, T, A, TAU, X, hg;
   int epoch = 0;
   std::map<std::tuple<int, int, int, int, int>, std::unique_ptr<CachedSched>> scheds;
   std::map<std::pair<int, int>, std::unique_ptr<DevBuf>> tilelists;  // (MT, NT) -> int2 list
@@ -479,6 +494,30 @@ int hq_schedule_tasks(int MT, int NT, int nb, int ib, int *out, int capacity) {
     out[5 * i + 2] = tl[i].r;
     out[5 * i + 3] = tl[i].j;
     out[5 * i + 4] = tl[i].slab;
+  }
+  return ok();
+}
+
+int hq_dag_host(int MT, int NT, int nb, int ib, int XNT, int P, int *tasks, int *deps,
+                int capacity, int *ntasks) {
+  if (!geometry_ok(MT, NT, nb, ib)) return g_err;
+  if (P < 1) return set_error(HQ_ERR_ARG, "P must be >= 1");
+  DagBuilder b(MT, NT, nb, ib, XNT >= 1, XNT);
+  b.build();
+  const int n = (int)b.t.size();
+  if (ntasks) *ntasks = n;
+  if (!tasks || !deps) return ok();
+  if (capacity < n) return set_error(HQ_ERR_ARG, "capacity %d < %d tasks", capacity, n);
+  std::vector<int> ord = b.order(P), pos(n);
+  for (int i = 0; i < n; i++) pos[ord[i]] = i;
+  for (int i = 0; i < n; i++) {
+    const hq_task &x = b.t[ord[i]];
+    tasks[5 * i + 0] = x.kind;
+    tasks[5 * i + 1] = x.k;
+    tasks[5 * i + 2] = x.r;
+    tasks[5 * i + 3] = x.j;
+    tasks[5 * i + 4] = x.slab;
+    for (int d = 0; d < HQ_MAXDEP; d++) deps[HQ_MAXDEP * i + d] = d < x.ndep ? pos[x.dep[d]] : -1;
   }
   return ok();
 }
@@ -724,6 +763,191 @@ void Hybrid_make_Q(mat_stat_t *status, double *Q, double *RY, double *TAU, int m
                 return hq_tile_make_q_dev(MT, NT, nb, ib2, dRY, dTAU, dX, st);
               });
   report("Hybrid_make_Q");
+}
+
+}  // extern "C"
+
+// =====================================================================
+// Homegrown tile_kernel.h API on host pointers (tile_kernel.h:28-162)
+// =====================================================================
+namespace {
+
+// One host array of a homegrown call: `rows` x `cols` with leading dimension
+// `ld` (only (cols-1)*ld + rows entries are touched), copied in and/or out.
+struct HArr {
+  double *h;
+  int rows, cols, ld;
+  bool in, out;
+  double *d = nullptr;
+  size_t n() const { return cols <= 0 || rows <= 0 ? 0 : (size_t)(cols - 1) * ld + rows; }
+};
+
+template <typename Fn>
+int homegrown_call(const char *fn_name, std::vector<HArr> arrs, size_t work_elems, Fn &&fn) {
+  Context *c;
+  if (get_ctx(&c) != HQ_OK) return g_err;
+  size_t total = work_elems;
+  for (auto &a : arrs) total += (a.n() + 31) & ~(size_t)31;
+  if (c->hg.reserve(total * sizeof(double)) != HQ_OK) return g_err;
+  double *base = (double *)c->hg.p;
+  size_t off = 0;
+  for (auto &a : arrs) {
+    a.d = base + off;
+    off += (a.n() + 31) & ~(size_t)31;
+    if (a.in && a.n())
+      HQ_HIP(hipMemcpyAsync(a.d, a.h, a.n() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  }
+  double *work = base + off;
+  HQ_HIP(fn(arrs, work, c->stream));
+  for (auto &a : arrs)
+    if (a.out && a.n())
+      HQ_HIP(hipMemcpyAsync(a.h, a.d, a.n() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HQ_HIP(hipStreamSynchronize(c->stream));
+  (void)fn_name;
+  return ok();
+}
+
+bool hg_dims_ok(int row, int col, int ld) {
+  if (row < 1 || col < 1 || col > 4096 || ld < row)
+    return set_error(HQ_ERR_ARG, "tile kernel: need 1 <= col <= 4096, row >= 1, ld >= row"), false;
+  return true;
+}
+
+size_t hg_work(int row, int col) {
+  size_t m = (size_t)(row > col ? row : col);
+  return 3 * m * m + 64;
+}
+
+}  // namespace
+
+extern "C" {
+
+void *tile_init(char type, int tile_row, int tile_col) {
+  (void)type;  // the reference allocates doubles whatever the type (tile_kernel.c:19-25)
+  return calloc((size_t)tile_row * tile_col * 3, sizeof(double));
+}
+
+void tile_dgeqrt(const int row, const int col, const int block, double *A, const int lda,
+                 double *TAU, double *Y, double *T, const int ldt, double *work) {
+  (void)work;
+  if (!hg_dims_ok(row, col, lda) || block < 1 || ldt < col) {
+    if (g_err == HQ_OK) set_error(HQ_ERR_ARG, "tile_dgeqrt: bad block / ldt");
+    report("tile_dgeqrt");
+    return;
+  }
+  homegrown_call("tile_dgeqrt",
+                 {{A, row, col, lda, true, true}, {TAU, col, 1, col, true, true},
+                  {Y, row, col, lda, true, true}, {T, col, col, ldt, true, true}},
+                 hg_work(row, col), [&](std::vector<HArr> &a, double *w, hipStream_t st) {
+                   return hqh_launch_dgeqrt(row, col, block, a[0].d, lda, a[1].d, a[2].d, a[3].d,
+                                            ldt, w, st);
+                 });
+  report("tile_dgeqrt");
+}
+
+void tile_dtsqrt(const int row, const int col, const int block, double *A1, const int lda1,
+                 double *A2, const int lda2, double *TAU, double *T, const int ldt, double *work) {
+  (void)work;
+  if (!hg_dims_ok(row, col, lda2) || lda1 < col || block < 1 || ldt < col) {
+    if (g_err == HQ_OK) set_error(HQ_ERR_ARG, "tile_dtsqrt: bad block / lda1 / ldt");
+    report("tile_dtsqrt");
+    return;
+  }
+  homegrown_call("tile_dtsqrt",
+                 {{A1, col, col, lda1, true, true}, {A2, row, col, lda2, true, true},
+                  {TAU, col, 1, col, true, true}, {T, col, col, ldt, true, true}},
+                 hg_work(row, col), [&](std::vector<HArr> &a, double *w, hipStream_t st) {
+                   return hqh_launch_dtsqrt(row, col, block, a[0].d, lda1, a[1].d, lda2, a[2].d,
+                                            a[3].d, ldt, w, st);
+                 });
+  report("tile_dtsqrt");
+}
+
+void tile_dlarfb(const int row, const int col, double *A, const int lda, const double *Y,
+                 const double *T, const int ldt, double *work) {
+  (void)work;
+  if (!hg_dims_ok(row, col, lda) || ldt < col) {
+    if (g_err == HQ_OK) set_error(HQ_ERR_ARG, "tile_dlarfb: bad ldt");
+    report("tile_dlarfb");
+    return;
+  }
+  homegrown_call("tile_dlarfb",
+                 {{A, row, col, lda, true, true}, {(double *)Y, row, col, lda, true, false},
+                  {(double *)T, col, col, ldt, true, false}},
+                 hg_work(row, col), [&](std::vector<HArr> &a, double *w, hipStream_t st) {
+                   return hqh_launch_dlarfb(row, col, a[0].d, lda, a[1].d, a[2].d, ldt, w, st);
+                 });
+  report("tile_dlarfb");
+}
+
+void tile_dssrfb(const int row, const int col, double *A1, double *A2, const int lda,
+                 const double *Y, const double *T, const int ldt, double *work) {
+  (void)work;
+  if (!hg_dims_ok(row, col, lda) || ldt < col) {
+    if (g_err == HQ_OK) set_error(HQ_ERR_ARG, "tile_dssrfb: bad ldt");
+    report("tile_dssrfb");
+    return;
+  }
+  homegrown_call("tile_dssrfb",
+                 {{A1, row, col, lda, true, true}, {A2, row, col, lda, true, true},
+                  {(double *)Y, row, col, lda, true, false},
+                  {(double *)T, col, col, ldt, true, false}},
+                 hg_work(row, col), [&](std::vector<HArr> &a, double *w, hipStream_t st) {
+                   return hqh_launch_dssrfb(row, col, a[0].d, a[1].d, lda, a[2].d, a[3].d, ldt, w,
+                                            st);
+                 });
+  report("tile_dssrfb");
+}
+
+void tile_dlarft(const int row, const int col, const int block, const double *Y, const int lda,
+                 const double *TAU, double *T, const int ldt, double *work) {
+  (void)work;
+  if (!hg_dims_ok(row, col, lda) || ldt < col) {
+    if (g_err == HQ_OK) set_error(HQ_ERR_ARG, "tile_dlarft: bad ldt");
+    report("tile_dlarft");
+    return;
+  }
+  homegrown_call("tile_dlarft",
+                 {{(double *)Y, row, col, lda, true, false},
+                  {(double *)TAU, col, 1, col, true, false}, {T, col, col, ldt, true, true}},
+                 hg_work(row, col), [&](std::vector<HArr> &a, double *w, hipStream_t st) {
+                   return hqh_launch_dlarft(row, col, block, a[0].d, lda, a[1].d, a[2].d, ldt, w,
+                                            st);
+                 });
+  report("tile_dlarft");
+}
+
+// Hybrid_make_I.c:20-53 (one rank): Id <- identity in tile layout.
+void Hybrid_make_I(mat_stat_t *status, double *Id, int my_rank, int Proc_num) {
+  if (check_rank(my_rank, Proc_num) != HQ_OK || !status || !Id) {
+    if (g_err == HQ_OK) set_error(HQ_ERR_ARG, "Hybrid_make_I: null argument");
+    report("Hybrid_make_I");
+    return;
+  }
+  const size_t MT = status->my_tile_M, NT = status->my_tile_N, nb = status->tile_M;
+  memset(Id, 0, sizeof(double) * MT * NT * nb * status->tile_N);
+  for (size_t i = 0; i < NT && i < MT; i++)
+    for (size_t r = 0; r < nb; r++) Id[(i + i * MT) * nb * nb + r + r * nb] = 1.0;
+  ok();
+}
+
+// Hybrid_dgemm.c:16-70 (one rank, square tile grids): C <- A^T B, on the GPU.
+void Hybrid_dgemm(mat_stat_t *status, double *A, double *B, double *Cm, int my_rank,
+                  int Proc_num) {
+  if (check_rank(my_rank, Proc_num) != HQ_OK || !status || !A || !B || !Cm ||
+      status->tile_M != status->tile_N) {
+    if (g_err == HQ_OK) set_error(HQ_ERR_ARG, "Hybrid_dgemm: need square tiles, non-null arrays");
+    report("Hybrid_dgemm");
+    return;
+  }
+  const int MT = (int)status->glob_tile_M, NT = (int)status->glob_tile_N, nb = (int)status->tile_M;
+  const int n = (int)((size_t)MT * NT * nb * nb);
+  homegrown_call("Hybrid_dgemm",
+                 {{A, n, 1, n, true, false}, {B, n, 1, n, true, false}, {Cm, n, 1, n, false, true}},
+                 0, [&](std::vector<HArr> &a, double *, hipStream_t st) {
+                   return hqh_launch_tile_gemm_tn(a[0].d, a[1].d, a[2].d, MT, NT, nb, st);
+                 });
+  report("Hybrid_dgemm");
 }
 
 }  // extern "C"

@@ -60,6 +60,15 @@ int hq_tile_qr_dev_traced(int MT, int NT, int nb, int ib, double *A,
  * 2 LARFB, 3 SSRFB), step k, row r, column j, slab.  capacity >= 5*ntasks. */
 int hq_schedule_tasks(int MT, int NT, int nb, int ib, int *out, int capacity);
 
+/* Host-only: the DAG the device scheduler runs, in claim order (no GPU
+ * needed).  XNT < 1: factorisation; XNT >= 1: Q^T application to MT x XNT
+ * tiles.  P = number of workers of the list-scheduling simulation that
+ * orders the tasks.  tasks[5*i..] as hq_schedule_tasks; deps[6*i..6*i+5] =
+ * claim indices of the task's dependencies, -1 padded.  With tasks == NULL
+ * only *ntasks is set. */
+int hq_dag_host(int MT, int NT, int nb, int ib, int XNT, int P, int *tasks,
+                int *deps, int capacity, int *ntasks);
+
 /* Frequency of the device wall clock used by the trace, in kHz. */
 int hq_wall_clock_khz(int *khz);
 
