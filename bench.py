@@ -36,7 +36,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=20151130)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
-    ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic.json"))
+    ap.add_argument("--traffic-file",
+                    default=os.path.join(ROOT, "profiles", "sched_kernel_traffic.txt"))
     return ap.parse_args()
 
 
@@ -122,16 +123,15 @@ def main():
             print(json.dumps({"error": "factorisation check failed", "check": check}))
             sys.exit(3)
 
+    # HBM bytes per launch: calibrated FETCH_SIZE + WRITE_SIZE of two
+    # rocprofv3 --pmc runs of this same command (profiles/ summary)
     traffic = None
-    if os.path.exists(args.traffic_json):
-        try:
-            with open(args.traffic_json) as fh:
-                tj_ = json.load(fh)
-            key = f"{n}_{nb}_{ib}"
-            if key in tj_:
-                traffic = tj_[key]
-        except Exception:
-            traffic = None
+    if os.path.exists(args.traffic_file):
+        key = f"{n}_{nb}_{ib}"
+        for line in open(args.traffic_file):
+            parts = line.split()
+            if len(parts) == 2 and parts[0] == key:
+                traffic = float(parts[1])
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(nb, ib, args.seed)
     out = {
