@@ -15,7 +15,11 @@ ARCH = os.environ.get("HQ_OFFLOAD_ARCH", "gfx950")
 
 # Packed FP32 ops are switched off for every device compile (the hardware
 # forms of v_pk_* FP32 arithmetic are not used by this library).
-DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                # keep MFMA accumulators in ordinary VGPRs: the strip of the
+                # updated tile is both an MFMA accumulator and an MFMA operand,
+                # and the AGPR form copies it between register files each block
+                "-mllvm", "-amdgpu-mfma-vgpr-form=1"]
 
 
 def hipcc() -> str:

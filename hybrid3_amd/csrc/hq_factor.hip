@@ -19,6 +19,12 @@
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
+// Phase markers for timing the panel tasks (compile to nothing unless a
+// build defines HQ_PHASE as a function-like macro).
+#ifndef HQ_PHASE
+#define HQ_PHASE(i)
+#endif
+
 namespace hq {
 
 __device__ __forceinline__ d4 mfma(double a, double b, d4 c) {
@@ -34,12 +40,42 @@ __device__ __forceinline__ d4 mfma(double a, double b, d4 c) {
 // <-> row 4g + s.
 __device__ __forceinline__ int arow(int i) { return 4 * (i & 3) + (i >> 2); }
 
-// Butterfly sum: every lane ends with the bitwise-identical total (each level
-// adds the same two operands, only in swapped order).
-__device__ __forceinline__ double wave_sum(double v) {
+// Wave-wide sums on the VALU with DPP lane moves (no LDS round trips):
+// xor-1 and xor-2 quad permutes, half-row mirror and row mirror leave every
+// lane of a 16-lane row holding the row sum (each step adds the same two
+// operands in both partner lanes, so partners agree bit for bit); the four
+// row sums are then read back with readlane and added in the same order by
+// every lane.  N independent sums are interleaved to overlap latencies.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double x) {
+  int lo = __double2loint(x), hi = __double2hiint(x);
+  lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false);
+  hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double readlane_d(double x, int l) {
+  int lo = __builtin_amdgcn_readlane(__double2loint(x), l);
+  int hi = __builtin_amdgcn_readlane(__double2hiint(x), l);
+  return __hiloint2double(hi, lo);
+}
+template <int N>
+__device__ __forceinline__ void wave_sums(double (&v)[N]) {
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  for (int i = 0; i < N; i++) v[i] += dpp_move<0xB1>(v[i]);   // quad_perm [1,0,3,2]
+#pragma unroll
+  for (int i = 0; i < N; i++) v[i] += dpp_move<0x4E>(v[i]);   // quad_perm [2,3,0,1]
+#pragma unroll
+  for (int i = 0; i < N; i++) v[i] += dpp_move<0x141>(v[i]);  // row_half_mirror
+#pragma unroll
+  for (int i = 0; i < N; i++) v[i] += dpp_move<0x140>(v[i]);  // row_mirror
+#pragma unroll
+  for (int i = 0; i < N; i++)
+    v[i] = (readlane_d(v[i], 0) + readlane_d(v[i], 16)) + (readlane_d(v[i], 32) + readlane_d(v[i], 48));
+}
+__device__ __forceinline__ double wave_sum1(double x) {
+  double v[1] = {x};
+  wave_sums<1>(v);
+  return v[0];
 }
 
 __device__ __forceinline__ double dlapy2(double x, double y) {
@@ -135,40 +171,80 @@ __device__ __forceinline__ void apply_block(d4 (&acc)[NB / 16], int rb0, const d
                                             const double *TTs, const d4 (&A1v)[WB], double *a1col,
                                             int kb, bool ok, int i0 = 0) {
   constexpr int LDV = Smem<NB>::LDV, LDT = Smem<NB>::LDT;
+  constexpr int NBR = NB / 16;
   const int lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4;
   const int pm = arow(m);
+  // UNIT: Vs holds the raw tile; mask to the unit lower trapezoid whose
+  // diagonal is at row i0 + c.  (Every other caller stages V with zero
+  // padding, so no mask is needed there.)
   auto vmask = [&](double val, int r, int c) -> double {
     if (UNIT) {
       const int d = i0 + c;
       val = r > d ? val : (r == d ? 1.0 : 0.0);
+      val = c < kb ? val : 0.0;
     }
-    return c < kb ? val : 0.0;
+    return val;
   };
-  d4 W[WB];
+  // ---- W = A1 + V^T X.  MFMA (rb, s): k index at lane group g = row 16rb+4g+s.
+  // Operands of row block rb+1 are loaded while the MFMAs of rb issue; two
+  // accumulator sets (even / odd row blocks) keep 2*WB independent chains.
+  d4 W[WB], Wo[WB];
 #pragma unroll
-  for (int wb = 0; wb < WB; wb++) W[wb] = HAS_A1 ? A1v[wb] : d4{0.0, 0.0, 0.0, 0.0};
-  // W = A1 + V^T X: the k index of MFMA (rb, s) at lane group g is row 16rb+4g+s.
+  for (int wb = 0; wb < WB; wb++) {
+    W[wb] = HAS_A1 ? A1v[wb] : d4{0.0, 0.0, 0.0, 0.0};
+    Wo[wb] = d4{0.0, 0.0, 0.0, 0.0};
+  }
+  double op1[2][4 * WB];
+  auto ld1 = [&](double (&o)[4 * WB], int rb) {
 #pragma unroll
-  for (int rb = 0; rb < NB / 16; rb++) {
-    if (rb < rb0) continue;
+    for (int s = 0; s < 4; s++)
 #pragma unroll
-    for (int s = 0; s < 4; s++) {
+      for (int wb = 0; wb < WB; wb++)
+        o[s * WB + wb] = vmask(Vs[(16 * wb + pm) * LDV + 16 * rb + 4 * g + s],
+                               16 * rb + 4 * g + s, 16 * wb + pm);
+  };
+  ld1(op1[0], 0);
 #pragma unroll
-      for (int wb = 0; wb < WB; wb++) {
-        double a = vmask(Vs[(16 * wb + pm) * LDV + 16 * rb + 4 * g + s], 16 * rb + 4 * g + s,
-                         16 * wb + pm);
-        W[wb] = mfma(a, acc[rb][s], W[wb]);
-      }
+  for (int rb = 0; rb < NBR; rb++) {
+    if (rb + 1 < NBR) ld1(op1[(rb + 1) & 1], rb + 1);
+    if (rb >= rb0) {
+#pragma unroll
+      for (int s = 0; s < 4; s++)
+#pragma unroll
+        for (int wb = 0; wb < WB; wb++) {
+          const double a = op1[rb & 1][s * WB + wb];
+          if (rb & 1)
+            Wo[wb] = mfma(a, acc[rb][s], Wo[wb]);
+          else
+            W[wb] = mfma(a, acc[rb][s], W[wb]);
+        }
     }
-    // keep the operand loads of one row block next to its MFMAs (stops the
-    // scheduler from hoisting every LDS load of the loop and spilling)
     __builtin_amdgcn_sched_group_barrier(0x100, 4 * WB, 0);
     __builtin_amdgcn_sched_group_barrier(0x008, 4 * WB, 0);
   }
+#pragma unroll
+  for (int wb = 0; wb < WB; wb++) W[wb] += Wo[wb];
+
+  // ---- X -= V W operand chunks: chunk q = (wb, s, h) covers row blocks
+  // h*CH .. h*CH+CH-1; k index of MFMA (wb, s) at lane group g = W row 16wb+4g+s.
+  constexpr int CH = NBR < 8 ? NBR : 8;
+  constexpr int NH = NBR / CH;
+  constexpr int NCH = WB * 4 * NH;
+  double op3[2][CH];
+  auto ld3 = [&](double (&o)[CH], int q) {
+    const int wb = q / (4 * NH), s = (q / NH) % 4, h = q % NH;
+#pragma unroll
+    for (int i = 0; i < CH; i++) {
+      const int rb = h * CH + i;
+      o[i] = vmask(Vs[(16 * wb + 4 * g + s) * LDV + 16 * rb + pm], 16 * rb + pm, 16 * wb + 4 * g + s);
+    }
+  };
+  ld3(op3[0], 0);  // overlaps the T^T product below
+
+  // ---- W2 = T^T W (T^T lower triangular)
   d4 W2[WB];
 #pragma unroll
   for (int wb = 0; wb < WB; wb++) W2[wb] = d4{0.0, 0.0, 0.0, 0.0};
-  // W2 = T^T W (T^T lower triangular)
 #pragma unroll
   for (int wi = 0; wi < WB; wi++)
 #pragma unroll
@@ -189,37 +265,38 @@ __device__ __forceinline__ void apply_block(d4 (&acc)[NB / 16], int rb0, const d
   }
 #pragma unroll
   for (int wb = 0; wb < WB; wb++) W2[wb] = -W2[wb];
-  // X -= V W: k index of MFMA (wb, s) at lane group g is W row 16wb+4g+s.
+  // ---- X -= V W, row blocks innermost (consecutive MFMAs are independent)
 #pragma unroll
-  for (int rb = 0; rb < NB / 16; rb++) {
-    if (rb < rb0) continue;
+  for (int q = 0; q < NCH; q++) {
+    if (q + 1 < NCH) ld3(op3[(q + 1) & 1], q + 1);
+    const int wb = q / (4 * NH), s = (q / NH) % 4, h = q % NH;
 #pragma unroll
-    for (int wb = 0; wb < WB; wb++)
-#pragma unroll
-      for (int s = 0; s < 4; s++) {
-        double a = vmask(Vs[(16 * wb + 4 * g + s) * LDV + 16 * rb + pm], 16 * rb + pm,
-                         16 * wb + 4 * g + s);
-        acc[rb] = mfma(a, W2[wb][s], acc[rb]);
-      }
-    __builtin_amdgcn_sched_group_barrier(0x100, 4 * WB, 1);
-    __builtin_amdgcn_sched_group_barrier(0x008, 4 * WB, 1);
+    for (int i = 0; i < CH; i++) {
+      const int rb = h * CH + i;
+      if (rb >= rb0) acc[rb] = mfma(op3[q & 1][i], W2[wb][s], acc[rb]);
+    }
+    __builtin_amdgcn_sched_group_barrier(0x100, CH, 1);
+    __builtin_amdgcn_sched_group_barrier(0x008, CH, 1);
   }
 }
 
 // --------------------------------------------------------------------------
-// Panel factorisation in LDS.
+// Panel factorisation (one ib-column panel of GEQRT or TSQRT).
 //  GEQRT (TS=false): panel columns 0..sb-1 = tile columns ii.., rows ii..NB-1;
 //    column c's reflector has its unit element at row d = ii+c.
-//  TSQRT (TS=true): triangle block S.A1 (sb x sb) on top of S.V (NB rows);
-//    reflector c = [e_c ; V(:,c)].
-// Each lane always handles rows lane + 64q, so row data never crosses lanes
-// except through the butterfly sums.  One barrier per column: the wave that
-// owns column c+1 (c+1 = wave mod 4) updates it first and generates the next
-// reflector before the barrier.
+//  TSQRT (TS=true): triangle block (sb x sb, S.A1) on top of the NB-row
+//    panel; reflector c = [e_c ; v_c].
+// Wave w owns the panel columns j = 4t + w (t < 8) and keeps them in
+// registers for the whole panel (lane L holds rows L + 64q; for TSQRT lanes
+// 0..31 also hold rows 0..31 of the triangle block).  Only the reflector of
+// the current column travels through LDS.  One barrier per column: after
+// updating its columns with reflector c, the owner of column c+1 generates
+// the next reflector.
 // --------------------------------------------------------------------------
 template <int NB, bool TS>
-__device__ __forceinline__ void gen_reflector(Smem<NB> &S, int c, int d, double (&x)[NB / 64]) {
-  constexpr int LDV = Smem<NB>::LDV, LDT = Smem<NB>::LDT;
+__device__ __forceinline__ void gen_reflector(Smem<NB> &S, int c, int d, double (&x)[NB / 64],
+                                              double &a1) {
+  constexpr int LDV = Smem<NB>::LDV;
   const int lane = threadIdx.x & 63;
   double ss = 0.0, xd = 0.0;
 #pragma unroll
@@ -228,8 +305,17 @@ __device__ __forceinline__ void gen_reflector(Smem<NB> &S, int c, int d, double 
     if (r > d) ss += x[q] * x[q];
     if (!TS && r == d) xd = x[q];
   }
-  ss = wave_sum(ss);
-  double alpha = TS ? S.A1[c * LDT + c] : wave_sum(xd);
+  double alpha;
+  if (TS) {
+    ss = wave_sum1(ss);
+    alpha = readlane_d(a1, c);
+  } else {
+    double two[2] = {ss, xd};
+    wave_sums<2>(two);
+    ss = two[0];
+    alpha = two[1];
+  }
+  // LAPACK dlarfg: beta = -sign(alpha) * ||(alpha, x)||
   double xnorm = sqrt(ss);
   double tau, beta, scal;
   if (xnorm == 0.0) {
@@ -248,25 +334,30 @@ __device__ __forceinline__ void gen_reflector(Smem<NB> &S, int c, int d, double 
       x[q] *= scal;
       S.V[c * LDV + r] = x[q];
     }
-    if (!TS && r == d) S.V[c * LDV + r] = beta;
+    if (!TS && r == d) x[q] = beta;
   }
-  if (lane == 0) {
-    S.tau[c] = tau;
-    if (TS) S.A1[c * LDT + c] = beta;
-  }
+  if (TS && lane == c) a1 = beta;
+  if (lane == 0) S.tau[c] = tau;
 }
 
 template <int NB, bool TS>
 __device__ __forceinline__ void panel_factor(Smem<NB> &S, int ii, int sb) {
   constexpr int LDV = Smem<NB>::LDV, LDT = Smem<NB>::LDT;
-  constexpr int NQ = NB / 64;
+  constexpr int NQ = NB / 64, JW = HQ_IBMAX / 4;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (wave == 0) {
-    double x[NQ];
+  double x[JW][NQ], a1[JW];
 #pragma unroll
-    for (int q = 0; q < NQ; q++) x[q] = S.V[lane + 64 * q];
-    gen_reflector<NB, TS>(S, 0, TS ? -1 : ii, x);
+  for (int t = 0; t < JW; t++) {
+    const int j = 4 * t + wave;
+    const bool valid = j < sb;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      const int r = lane + 64 * q;
+      x[t][q] = (valid && (TS || r >= ii)) ? S.V[j * LDV + r] : 0.0;
+    }
+    a1[t] = (TS && valid && lane < HQ_IBMAX) ? S.A1[j * LDT + lane] : 0.0;
   }
+  if (wave == 0) gen_reflector<NB, TS>(S, 0, TS ? -1 : ii, x[0], a1[0]);
   __syncthreads();
   for (int c = 0; c < sb; c++) {
     const double tau = S.tau[c];
@@ -278,27 +369,43 @@ __device__ __forceinline__ void panel_factor(Smem<NB> &S, int ii, int sb) {
       double val = S.V[c * LDV + r];
       v[q] = TS ? val : (r > d ? val : (r == d ? 1.0 : 0.0));
     }
-    for (int j = c + 1 + ((wave - c - 1) & 3); j < sb; j += 4) {
-      double x[NQ];
-      double s = 0.0;
+    double dot[JW];
 #pragma unroll
-      for (int q = 0; q < NQ; q++) {
-        x[q] = S.V[j * LDV + lane + 64 * q];
-        s += v[q] * x[q];
-      }
-      s = wave_sum(s);
-      double top = TS ? S.A1[j * LDT + c] : 0.0;
-      double w = tau * (s + top);
+    for (int t = 0; t < JW; t++) {
+      dot[t] = 0.0;
 #pragma unroll
-      for (int q = 0; q < NQ; q++) {
-        x[q] -= v[q] * w;
-        S.V[j * LDV + lane + 64 * q] = x[q];
+      for (int q = 0; q < NQ; q++) dot[t] += v[q] * x[t][q];
+    }
+    if (c + 1 < sb) wave_sums<JW>(dot);
+#pragma unroll
+    for (int t = 0; t < JW; t++) {
+      const int j = 4 * t + wave;
+      if (j > c && j < sb) {
+        const double top = TS ? readlane_d(a1[t], c) : 0.0;
+        const double w = tau * (dot[t] + top);
+#pragma unroll
+        for (int q = 0; q < NQ; q++) x[t][q] -= v[q] * w;
+        if (TS && lane == c) a1[t] = top - w;
       }
-      if (TS && lane == 0) S.A1[j * LDT + c] = top - w;
-      if (j == c + 1) gen_reflector<NB, TS>(S, c + 1, TS ? -1 : ii + c + 1, x);
+    }
+    if (c + 1 < sb && wave == ((c + 1) & 3)) {
+#pragma unroll
+      for (int t = 0; t < JW; t++)
+        if (t == ((c + 1) >> 2)) gen_reflector<NB, TS>(S, c + 1, TS ? -1 : ii + c + 1, x[t], a1[t]);
     }
     __syncthreads();
   }
+  // the finished panel back to LDS (rows < ii of a GEQRT panel as zeros)
+#pragma unroll
+  for (int t = 0; t < JW; t++) {
+    const int j = 4 * t + wave;
+    if (j < sb) {
+#pragma unroll
+      for (int q = 0; q < NQ; q++) S.V[j * LDV + lane + 64 * q] = x[t][q];
+      if (TS && lane < HQ_IBMAX) S.A1[j * LDT + lane] = a1[t];
+    }
+  }
+  __syncthreads();
 }
 
 // G = V^T V (upper block triangle) with MFMA; rows < 16*rb0 are zero in V.
@@ -331,24 +438,38 @@ __device__ __forceinline__ void gram(Smem<NB> &S, int rb0) {
 
 // T (upper, sb x sb) from G and tau, LAPACK dlarft('F','C') recursion
 //   T(0:c, c) = -tau_c T(0:c, 0:c) G(0:c, c),  T(c,c) = tau_c.
-// Lane p of wave 0 owns row p of T, kept in S.TT (TT[c*LDT + p] = T(p,c)), so
-// every value a lane reads it wrote itself.  Writes T to global (ld = ib,
-// block at column ii), and tau to TAU.
+// Lane p of wave 0 holds row p of T in registers; column c needs only the
+// lane's own row and the broadcast column c of G.  Writes T^T to S.TT, T to
+// global (ld = ib, block at column ii) and, if taug, tau to TAU.
 template <int NB>
 __device__ __forceinline__ void build_t(Smem<NB> &S, int sb, double *Tg, int ib, double *taug) {
   constexpr int LDT = Smem<NB>::LDT;
   const int p = threadIdx.x;  // wave 0 only
-  if (p < HQ_IBMAX) {
-    for (int c = 0; c < sb; c++) {
-      double t = 0.0;
-#pragma unroll 8
-      for (int q = 0; q < c; q++) t += S.TT[q * LDT + p] * S.G[q * LDT + c];
+  double Tr[HQ_IBMAX];
+#pragma unroll
+  for (int c = 0; c < HQ_IBMAX; c++) {
+    double val = 0.0;
+    if (c < sb) {
+      double t0 = 0.0, t1 = 0.0;
+#pragma unroll
+      for (int q = 0; q < c; q++) {
+        if (q & 1)
+          t1 += Tr[q] * S.G[q * LDT + c];
+        else
+          t0 += Tr[q] * S.G[q * LDT + c];
+      }
       const double tc = S.tau[c];
-      S.TT[c * LDT + p] = (p < c) ? -tc * t : (p == c ? tc : 0.0);
+      val = (p < c) ? -tc * (t0 + t1) : (p == c ? tc : 0.0);
     }
-    for (int c = sb; c < HQ_IBMAX; c++) S.TT[c * LDT + p] = 0.0;
-    if (p < ib)
-      for (int c = 0; c < sb; c++) Tg[(size_t)c * ib + p] = S.TT[c * LDT + p];
+    Tr[c] = val;
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  if (p < HQ_IBMAX) {
+#pragma unroll
+    for (int c = 0; c < HQ_IBMAX; c++) {
+      S.TT[c * LDT + p] = Tr[c];
+      if (c < sb && p < ib) Tg[(size_t)c * ib + p] = Tr[c];
+    }
   }
   if (taug && p < sb) taug[p] = S.tau[p];
 }
@@ -356,6 +477,38 @@ __device__ __forceinline__ void build_t(Smem<NB> &S, int sb, double *Tg, int ib,
 // --------------------------------------------------------------------------
 // Tasks.
 // --------------------------------------------------------------------------
+
+typedef __attribute__((address_space(3))) char lds_char;
+
+// Tile columns col0 .. col0+ncol-1 (all NB rows) into the LDS panel buffer
+// Vs (column stride LDV); padding columns ncol .. IBP-1 set to zero.
+// NB >= 128 uses direct-to-LDS 16-byte loads; the caller waits (s_waitcnt)
+// and synchronises before reading.
+template <int NB, int WB>
+__device__ __forceinline__ void stage_columns(double *Vs, const double *tile, int col0, int ncol) {
+  constexpr int LDV = Smem<NB>::LDV;
+  constexpr int IBP = 16 * WB;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if constexpr (NB >= 128) {
+    constexpr int HALVES = NB / 128;
+    for (int q = wave; q < ncol * HALVES; q += HQ_THREADS / 64) {
+      const int c = q / HALVES, h = q - c * HALVES;
+      const double *src = tile + (size_t)(col0 + c) * NB + 128 * h + 2 * lane;
+      lds_char *dst = (lds_char *)(Vs + c * LDV + 128 * h);
+      __builtin_amdgcn_global_load_lds((const void *)src, (void __attribute__((address_space(3))) *)dst,
+                                       16, 0, 0);
+    }
+  } else {
+    for (int e = tid; e < ncol * NB; e += HQ_THREADS) {
+      const int c = e / NB, r = e - c * NB;
+      Vs[c * LDV + r] = tile[(size_t)(col0 + c) * NB + r];
+    }
+  }
+  for (int e = tid; e < (IBP - ncol) * NB; e += HQ_THREADS) {
+    const int c = ncol + e / NB, r = e - (c - ncol) * NB;
+    Vs[c * LDV + r] = 0.0;
+  }
+}
 
 // GEQRT = CORE_dgeqrt(nb, nb, ib, A, nb, T, ib, TAU)
 template <int NB, int WB>
@@ -365,10 +518,8 @@ __device__ __forceinline__ void task_geqrt(Smem<NB> &S, double *A, double *T, do
   const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, wave = tid >> 6;
   for (int ii = 0; ii < NB; ii += ib) {
     const int sb = min(ib, NB - ii);
-    for (int e = tid; e < IBP * NB; e += HQ_THREADS) {
-      int c = e / NB, r = e - c * NB;
-      S.V[c * LDV + r] = (c < sb && r >= ii) ? A[(size_t)(ii + c) * NB + r] : 0.0;
-    }
+    stage_columns<NB, WB>(S.V, A, ii, sb);
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     panel_factor<NB, false>(S, ii, sb);
     // write R / V back, then make the LDS copy unit lower trapezoidal
@@ -407,16 +558,16 @@ __device__ __forceinline__ void task_tsqrt(Smem<NB> &S, double *A1, double *A2, 
   const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, wave = tid >> 6;
   for (int ii = 0; ii < NB; ii += ib) {
     const int sb = min(ib, NB - ii);
-    for (int e = tid; e < IBP * NB; e += HQ_THREADS) {
-      int c = e / NB, r = e - c * NB;
-      S.V[c * LDV + r] = (c < sb) ? A2[(size_t)(ii + c) * NB + r] : 0.0;
-    }
+    stage_columns<NB, WB>(S.V, A2, ii, sb);
     for (int e = tid; e < IBP * IBP; e += HQ_THREADS) {
       int c = e / IBP, p = e - c * IBP;
       S.A1[c * LDT + p] = (c < sb && p <= c) ? A1[(size_t)(ii + c) * NB + ii + p] : 0.0;
     }
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
+    HQ_PHASE(0);
     panel_factor<NB, true>(S, ii, sb);
+    HQ_PHASE(1);
     for (int e = tid; e < sb * NB; e += HQ_THREADS) {
       int c = e / NB, r = e - c * NB;
       A2[(size_t)(ii + c) * NB + r] = S.V[c * LDV + r];
@@ -426,10 +577,13 @@ __device__ __forceinline__ void task_tsqrt(Smem<NB> &S, double *A1, double *A2, 
       if (p <= c) A1[(size_t)(ii + c) * NB + ii + p] = S.A1[c * LDT + p];
     }
     __syncthreads();
+    HQ_PHASE(2);
     gram<NB, WB>(S, 0);
     __syncthreads();
+    HQ_PHASE(3);
     if (wave == 0) build_t<NB>(S, sb, T + (size_t)ii * ib, ib, TAU + ii);
     __syncthreads();
+    HQ_PHASE(4);
     const int j0 = ii + sb;
     const int ngrp = (NB - j0 + 15) / 16;
     for (int grp = wave; grp < ngrp; grp += 4) {
@@ -443,6 +597,7 @@ __device__ __forceinline__ void task_tsqrt(Smem<NB> &S, double *A1, double *A2, 
       store_strip<NB>(acc, A2, col, ok, 0);
     }
     __syncthreads();
+    HQ_PHASE(5);
   }
 }
 
@@ -460,8 +615,6 @@ struct BlockRegs {
   double t[NPT];
   double v[NPV];
 };
-
-typedef __attribute__((address_space(3))) char lds_char;
 
 template <int NB, int WB>
 __device__ __forceinline__ void fetch_block(BlockRegs<NB, WB> &R, double *Vs_next, const double *V,
@@ -496,10 +649,18 @@ __device__ __forceinline__ void fetch_block(BlockRegs<NB, WB> &R, double *Vs_nex
 }
 
 template <int NB, int WB>
-__device__ __forceinline__ void store_block(const BlockRegs<NB, WB> &R, double *Vs, double *TTs) {
+__device__ __forceinline__ void store_block(const BlockRegs<NB, WB> &R, double *Vs, double *TTs,
+                                            int kb) {
   constexpr int IBP = 16 * WB;
   constexpr int LDV = Smem<NB>::LDV, LDT = Smem<NB>::LDT;
   const int tid = threadIdx.x;
+  if constexpr (NB >= 128) {
+    // a ragged last block: the columns the direct-to-LDS loads skipped are zero
+    for (int e = tid; e < (IBP - kb) * NB; e += HQ_THREADS) {
+      const int c = kb + e / NB, r = e - (c - kb) * NB;
+      Vs[c * LDV + r] = 0.0;
+    }
+  }
   if constexpr (NB < 128) {
 #pragma unroll
     for (int i = 0; i < BlockRegs<NB, WB>::NPV; i++) {
@@ -538,7 +699,7 @@ __device__ __forceinline__ void task_update(Smem<NB> &S, const double *V, const 
   fetch_block<NB, WB>(R, S.V, V, T, ib, 0, min(ib, NB));
   load_strip<NB>(acc, X, col, ok, 0);
   if (!UNIT) load_a1<WB>(A1v, a1c, min(ib, NB), ok);
-  store_block<NB, WB>(R, S.V, S.TT);
+  store_block<NB, WB>(R, S.V, S.TT, min(ib, NB));
   __syncthreads();
   for (int b = 0; b < nblk; b++) {
     const int i0 = b * ib, kb = min(ib, NB - i0);
@@ -558,7 +719,7 @@ __device__ __forceinline__ void task_update(Smem<NB> &S, const double *V, const 
     else
       apply_block<NB, WB, true>(acc, 0, Vs, TTs, A1v, a1c + i0, kb, ok);
     if (more) {
-      store_block<NB, WB>(R, Vn, TTn);
+      store_block<NB, WB>(R, Vn, TTn, min(ib, NB - i0 - ib));
 #pragma unroll
       for (int wb = 0; wb < WB; wb++) A1v[wb] = A1n[wb];
     }
