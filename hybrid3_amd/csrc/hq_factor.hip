@@ -48,9 +48,8 @@ __device__ __forceinline__ int arow(int i) { return 4 * (i & 3) + (i >> 2); }
 // every lane.  N independent sums are interleaved to overlap latencies.
 template <int CTRL>
 __device__ __forceinline__ double dpp_move(double x) {
-  int lo = __double2loint(x), hi = __double2hiint(x);
-  lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false);
-  hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false);
+  int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xF, 0xF, true);
+  int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, 0xF, 0xF, true);
   return __hiloint2double(hi, lo);
 }
 __device__ __forceinline__ double readlane_d(double x, int l) {
@@ -58,19 +57,68 @@ __device__ __forceinline__ double readlane_d(double x, int l) {
   int hi = __builtin_amdgcn_readlane(__double2hiint(x), l);
   return __hiloint2double(hi, lo);
 }
+// x + (x of the lane 16 apart: rows 0<->1, 2<->3), and + (lane 32 apart);
+// v_permlane{16,32}_swap exchange rows of two registers, so swapping a value
+// with itself and adding the pair gives the cross-row sum in every lane.
+__device__ __forceinline__ double xrow_sum16(double x) {
+  auto l = __builtin_amdgcn_permlane16_swap(__double2loint(x), __double2loint(x), false, false);
+  auto h = __builtin_amdgcn_permlane16_swap(__double2hiint(x), __double2hiint(x), false, false);
+  return __hiloint2double(h[0], l[0]) + __hiloint2double(h[1], l[1]);
+}
+__device__ __forceinline__ double xrow_sum32(double x) {
+  auto l = __builtin_amdgcn_permlane32_swap(__double2loint(x), __double2loint(x), false, false);
+  auto h = __builtin_amdgcn_permlane32_swap(__double2hiint(x), __double2hiint(x), false, false);
+  return __hiloint2double(h[0], l[0]) + __hiloint2double(h[1], l[1]);
+}
+// One butterfly stage that halves M per-lane values: the two lanes of a pair
+// (DPP pattern CTRL) keep opposite halves, so each sends the half it drops.
+template <int M, int CTRL>
+__device__ __forceinline__ void fold(double *u, bool hi_half) {
+#pragma unroll
+  for (int k = 0; k < M / 2; k++) {
+    const double keep = hi_half ? u[k + M / 2] : u[k];
+    const double send = hi_half ? u[k] : u[k + M / 2];
+    u[k] = keep + dpp_move<CTRL>(send);
+  }
+}
+// Wave-wide sums of N per-lane values (N = 1, 2, 4, 8), result v[t] uniform.
+// Stages pair lanes i^1 (quad_perm [1,0,3,2]), i^2 (quad_perm [2,3,0,1]),
+// i^7 (row_half_mirror), i^15 (row_mirror), then rows.  While more than one
+// value is left, stage s keeps the half picked by f1 = b0^b2, f2 = b1^b2,
+// f3 = b2^b3 of the lane bits: each is flipped by its own pairing and left
+// unchanged by the later ones, so paired lanes always hold the same columns.
+// After the stages lane L holds the total of value 4*f1+2*f2+f3 (N = 8),
+// 2*f1+f2 (N = 4), f1 (N = 2); pairs of lanes add the same two numbers, so
+// every copy of a total is bitwise identical.
 template <int N>
 __device__ __forceinline__ void wave_sums(double (&v)[N]) {
+  static_assert(N == 1 || N == 2 || N == 4 || N == 8, "wave_sums: N in {1,2,4,8}");
+  const int lane = threadIdx.x & 63;
+  const bool f1 = (lane ^ (lane >> 2)) & 1;
+  const bool f2 = ((lane >> 1) ^ (lane >> 2)) & 1;
+  const bool f3 = ((lane >> 2) ^ (lane >> 3)) & 1;
+  double u[N];
 #pragma unroll
-  for (int i = 0; i < N; i++) v[i] += dpp_move<0xB1>(v[i]);   // quad_perm [1,0,3,2]
+  for (int i = 0; i < N; i++) u[i] = v[i];
+  if constexpr (N >= 2) fold<N, 0xB1>(u, f1); else u[0] += dpp_move<0xB1>(u[0]);
+  if constexpr (N >= 4) fold<N / 2, 0x4E>(u, f2); else u[0] += dpp_move<0x4E>(u[0]);
+  if constexpr (N >= 8) fold<N / 4, 0x141>(u, f3); else u[0] += dpp_move<0x141>(u[0]);
+  double y = u[0] + dpp_move<0x140>(u[0]);
+  y = xrow_sum32(xrow_sum16(y));
+  if constexpr (N == 1) {
+    v[0] = y;
+  } else {
 #pragma unroll
-  for (int i = 0; i < N; i++) v[i] += dpp_move<0x4E>(v[i]);   // quad_perm [2,3,0,1]
-#pragma unroll
-  for (int i = 0; i < N; i++) v[i] += dpp_move<0x141>(v[i]);  // row_half_mirror
-#pragma unroll
-  for (int i = 0; i < N; i++) v[i] += dpp_move<0x140>(v[i]);  // row_mirror
-#pragma unroll
-  for (int i = 0; i < N; i++)
-    v[i] = (readlane_d(v[i], 0) + readlane_d(v[i], 16)) + (readlane_d(v[i], 32) + readlane_d(v[i], 48));
+    for (int t = 0; t < N; t++) {
+      // a lane (row 0, b3 = 0) holding value t
+      int g1, g2, g3;
+      if constexpr (N == 8) { g1 = t >> 2; g2 = (t >> 1) & 1; g3 = t & 1; }
+      if constexpr (N == 4) { g1 = t >> 1; g2 = t & 1; g3 = 0; }
+      if constexpr (N == 2) { g1 = t; g2 = 0; g3 = 0; }
+      const int b2 = g3, b0 = g1 ^ b2, b1 = g2 ^ b2;
+      v[t] = readlane_d(y, b0 + 2 * b1 + 4 * b2);
+    }
+  }
 }
 __device__ __forceinline__ double wave_sum1(double x) {
   double v[1] = {x};
@@ -111,6 +159,7 @@ struct Smem {
   };
   double tau[HQ_IBMAX];
   int task;
+  int nref;  // reflectors published in the current panel factorisation
 };
 
 // --------------------------------------------------------------------------
@@ -288,10 +337,8 @@ __device__ __forceinline__ void apply_block(d4 (&acc)[NB / 16], int rb0, const d
 //    panel; reflector c = [e_c ; v_c].
 // Wave w owns the panel columns j = 4t + w (t < 8) and keeps them in
 // registers for the whole panel (lane L holds rows L + 64q; for TSQRT lanes
-// 0..31 also hold rows 0..31 of the triangle block).  Only the reflector of
-// the current column travels through LDS.  One barrier per column: after
-// updating its columns with reflector c, the owner of column c+1 generates
-// the next reflector.
+// 0..31 also hold rows 0..31 of the triangle block).  Only the reflectors
+// travel through LDS, each announced by an LDS counter instead of a barrier.
 // --------------------------------------------------------------------------
 template <int NB, bool TS>
 __device__ __forceinline__ void gen_reflector(Smem<NB> &S, int c, int d, double (&x)[NB / 64],
@@ -340,11 +387,20 @@ __device__ __forceinline__ void gen_reflector(Smem<NB> &S, int c, int d, double 
   if (lane == 0) S.tau[c] = tau;
 }
 
+// Wait (bounded) until the LDS counter reaches val; acquire at workgroup scope.
+__device__ __forceinline__ void lds_wait_geq(int *flag, int val) {
+  for (unsigned spins = 0; spins < (1u << 22); spins++) {
+    int f = __hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (__builtin_amdgcn_readfirstlane(f) >= val) break;
+  }
+}
+
 template <int NB, bool TS>
 __device__ __forceinline__ void panel_factor(Smem<NB> &S, int ii, int sb) {
   constexpr int LDV = Smem<NB>::LDV, LDT = Smem<NB>::LDT;
   constexpr int NQ = NB / 64, JW = HQ_IBMAX / 4;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   double x[JW][NQ], a1[JW];
 #pragma unroll
   for (int t = 0; t < JW; t++) {
@@ -357,9 +413,16 @@ __device__ __forceinline__ void panel_factor(Smem<NB> &S, int ii, int sb) {
     }
     a1[t] = (TS && valid && lane < HQ_IBMAX) ? S.A1[j * LDT + lane] : 0.0;
   }
-  if (wave == 0) gen_reflector<NB, TS>(S, 0, TS ? -1 : ii, x[0], a1[0]);
-  __syncthreads();
+  // S.nref counts the published reflectors (0 on entry).  Reflector c+1 is
+  // produced by the owner of column c+1 right after it applies reflector c
+  // to that column; its other columns follow, so the next reflector's chain
+  // overlaps the bulk of the column updates.
+  if (wave == 0) {
+    gen_reflector<NB, TS>(S, 0, TS ? -1 : ii, x[0], a1[0]);
+    __hip_atomic_store(&S.nref, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
   for (int c = 0; c < sb; c++) {
+    lds_wait_geq(&S.nref, c + 1);
     const double tau = S.tau[c];
     const int d = TS ? -1 : ii + c;
     double v[NQ];
@@ -369,32 +432,47 @@ __device__ __forceinline__ void panel_factor(Smem<NB> &S, int ii, int sb) {
       double val = S.V[c * LDV + r];
       v[q] = TS ? val : (r > d ? val : (r == d ? 1.0 : 0.0));
     }
-    double dot[JW];
-#pragma unroll
-    for (int t = 0; t < JW; t++) {
-      dot[t] = 0.0;
-#pragma unroll
-      for (int q = 0; q < NQ; q++) dot[t] += v[q] * x[t][q];
-    }
-    if (c + 1 < sb) wave_sums<JW>(dot);
-#pragma unroll
-    for (int t = 0; t < JW; t++) {
-      const int j = 4 * t + wave;
-      if (j > c && j < sb) {
-        const double top = TS ? readlane_d(a1[t], c) : 0.0;
-        const double w = tau * (dot[t] + top);
-#pragma unroll
-        for (int q = 0; q < NQ; q++) x[t][q] -= v[q] * w;
-        if (TS && lane == c) a1[t] = top - w;
-      }
-    }
     if (c + 1 < sb && wave == ((c + 1) & 3)) {
 #pragma unroll
-      for (int t = 0; t < JW; t++)
-        if (t == ((c + 1) >> 2)) gen_reflector<NB, TS>(S, c + 1, TS ? -1 : ii + c + 1, x[t], a1[t]);
+      for (int t = 0; t < JW; t++) {
+        if (t == ((c + 1) >> 2)) {
+          double dt = 0.0;
+#pragma unroll
+          for (int q = 0; q < NQ; q++) dt += v[q] * x[t][q];
+          dt = wave_sum1(dt);
+          const double top = TS ? readlane_d(a1[t], c) : 0.0;
+          const double w = tau * (dt + top);
+#pragma unroll
+          for (int q = 0; q < NQ; q++) x[t][q] -= v[q] * w;
+          if (TS && lane == c) a1[t] = top - w;
+          gen_reflector<NB, TS>(S, c + 1, TS ? -1 : ii + c + 1, x[t], a1[t]);
+        }
+      }
+      __hip_atomic_store(&S.nref, c + 2, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
-    __syncthreads();
+    if (c + 2 < sb) {
+      double dot[JW];
+#pragma unroll
+      for (int t = 0; t < JW; t++) {
+        dot[t] = 0.0;
+#pragma unroll
+        for (int q = 0; q < NQ; q++) dot[t] += v[q] * x[t][q];
+      }
+      wave_sums<JW>(dot);
+#pragma unroll
+      for (int t = 0; t < JW; t++) {
+        const int j = 4 * t + wave;
+        if (j > c + 1 && j < sb) {
+          const double top = TS ? readlane_d(a1[t], c) : 0.0;
+          const double w = tau * (dot[t] + top);
+#pragma unroll
+          for (int q = 0; q < NQ; q++) x[t][q] -= v[q] * w;
+          if (TS && lane == c) a1[t] = top - w;
+        }
+      }
+    }
   }
+  __syncthreads();
   // the finished panel back to LDS (rows < ii of a GEQRT panel as zeros)
 #pragma unroll
   for (int t = 0; t < JW; t++) {
@@ -519,6 +597,7 @@ __device__ __forceinline__ void task_geqrt(Smem<NB> &S, double *A, double *T, do
   for (int ii = 0; ii < NB; ii += ib) {
     const int sb = min(ib, NB - ii);
     stage_columns<NB, WB>(S.V, A, ii, sb);
+    if (tid == 0) S.nref = 0;
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     panel_factor<NB, false>(S, ii, sb);
@@ -559,6 +638,7 @@ __device__ __forceinline__ void task_tsqrt(Smem<NB> &S, double *A1, double *A2, 
   for (int ii = 0; ii < NB; ii += ib) {
     const int sb = min(ib, NB - ii);
     stage_columns<NB, WB>(S.V, A2, ii, sb);
+    if (tid == 0) S.nref = 0;
     for (int e = tid; e < IBP * IBP; e += HQ_THREADS) {
       int c = e / IBP, p = e - c * IBP;
       S.A1[c * LDT + p] = (c < sb && p <= c) ? A1[(size_t)(ii + c) * NB + ii + p] : 0.0;
