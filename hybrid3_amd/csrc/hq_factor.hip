@@ -808,16 +808,322 @@ __device__ __forceinline__ void task_update(Smem<NB> &S, const double *V, const 
   store_strip<NB>(acc, X, col, ok, 0);
 }
 
+// --------------------------------------------------------------------------
+// Fused update pipeline for LARFB / SSRFB on nb >= 128.
+// Each ib block of reflectors is applied as sub-blocks of at most 16
+// (Q_b = Q_b0 Q_b1 with the diagonal 16x16 blocks of the block's T: the
+// compact WY form of a product of two blocks is exactly that).  Pass s of
+// the pipeline interleaves, four row blocks at a time,
+//   X -= V_s W2_s            (update with sub-block s)
+//   W_{s+1} += V_{s+1}^T X   (the rows just updated)
+// so the MFMA stream has no phase boundary inside a pass; after the pass
+// W2_{s+1} = -T_{s+1}^T W_{s+1} (four MFMAs) and, for SSRFB, the A1 rows of
+// sub-block s+1 are written.  V sub-blocks live in a ring of four LDS
+// buffers of 16 columns: the direct-to-LDS loads of sub-block s+3 are issued
+// one per phase during pass s, and the end of pass s waits (counted) only
+// for sub-block s+2, issued a whole pass earlier.  One barrier per
+// sub-block.
+// --------------------------------------------------------------------------
+struct SubBlk {
+  int i0, kb, p0;  // first reflector (tile column), width, row offset inside its T block
+};
+template <int NB>
+__device__ __forceinline__ SubBlk sub_blk(int s, int ib) {
+  const int nsb = (ib + 15) >> 4;
+  const int b = s / nsb, h = s - b * nsb;
+  SubBlk r;
+  r.p0 = 16 * h;
+  r.i0 = b * ib + r.p0;
+  r.kb = min(16, min(ib, NB - b * ib) - r.p0);
+  return r;
+}
+template <int NB>
+__device__ __forceinline__ int sub_count(int ib) {
+  const int nsb = (ib + 15) >> 4, nblk = (NB + ib - 1) / ib;
+  return (nblk - 1) * nsb + (NB - (nblk - 1) * ib + 15) / 16;
+}
+template <int NB>
+__device__ __forceinline__ double *fvbuf(Smem<NB> &S, int i) {
+  constexpr int LDV = Smem<NB>::LDV;
+  return (i < 2 ? S.V : S.V1) + (i & 1) * 16 * LDV;
+}
+constexpr int LDT16 = 17;
+template <int NB>
+__device__ __forceinline__ double *ftbuf(Smem<NB> &S, int i) {
+  return (i < 2 ? S.TT : S.TT1) + (i & 1) * 16 * LDT16;
+}
+// direct-to-LDS loads per wave for one 16-column sub-block
+template <int NB>
+struct FDma {
+  static constexpr int HALVES = NB / 128;
+  static constexpr int NOPS = 16 * HALVES / (HQ_THREADS / 64);
+  // s_waitcnt immediate: vmcnt(NOPS), expcnt untouched, lgkmcnt(0)
+  static constexpr int WAIT = (NOPS & 15) | ((NOPS >> 4) << 14) | (7 << 4);
+};
+// Load op q (< NOPS) of this wave for sub-block sb into ring slot Vb: 128
+// rows of one column per op.  Ops past the block's width repeat its last
+// column, so every sub-block costs exactly NOPS ops per wave.
+template <int NB>
+__device__ __forceinline__ void dma_op(int q, double *Vb, const double *V, const SubBlk &sb) {
+  constexpr int LDV = Smem<NB>::LDV, HALVES = FDma<NB>::HALVES;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int Q = wave + (HQ_THREADS / 64) * q;
+  const int c = min(Q / HALVES, sb.kb - 1), h = Q - (Q / HALVES) * HALVES;
+  const double *src = V + (size_t)(sb.i0 + c) * NB + 128 * h + 2 * lane;
+  lds_char *dst = (lds_char *)(Vb + c * LDV + 128 * h);
+  __builtin_amdgcn_global_load_lds((const void *)src, (void __attribute__((address_space(3))) *)dst, 16, 0, 0);
+}
+
+// operands of update phase (group G = row blocks 2G, 2G+1): A(i, k) of MFMA
+// (sp, rb) = V(16rb + arow(i), 4k + sp)
+template <int NB, bool UNIT>
+__device__ __forceinline__ void ld_u(double (&o)[8], const double *Vb, int G, int i0, int kb) {
+  constexpr int LDV = Smem<NB>::LDV;
+  const int lane = threadIdx.x & 63, g = lane >> 4, pm = arow(lane & 15);
+#pragma unroll
+  for (int sp = 0; sp < 4; sp++)
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      const int r = 16 * (2 * G + i) + pm, c = 4 * g + sp;
+      o[2 * sp + i] = Vb[c * LDV + r];
+    }
+}
+// operands of accumulate phase: A(i, k) of MFMA (rb, sp) = V(16rb + 4k + sp, arow(i))
+template <int NB, bool UNIT>
+__device__ __forceinline__ void ld_a(double (&o)[8], const double *Vb, int G, int i0, int kb) {
+  constexpr int LDV = Smem<NB>::LDV;
+  const int lane = threadIdx.x & 63, g = lane >> 4, pm = arow(lane & 15);
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int sp = 0; sp < 4; sp++) {
+      const int r = 16 * (2 * G + i) + 4 * g + sp, c = pm;
+      o[4 * i + sp] = Vb[c * LDV + r];
+    }
+}
+
+template <int NB, bool UNIT>
+struct PassArgs {
+  const double *Vu, *Vw, *TTw;
+  int i0u, kbu, i0w, kbw;
+  double *Vd;         // ring slot receiving sub-block sd
+  const double *Vg;   // V tile in global memory
+  SubBlk sd;
+};
+
+// One pass: phases U0 U1 A0 U2 A1 ... U(NG-1) A(NG-2) A(NG-1) (DO_U && DO_A),
+// or only the U or only the A phases; a phase is 8 MFMAs on two row blocks.
+// o[0] holds the operands of the first phase on entry; each phase loads the
+// operands of the next one, the last the T^T operands (tto) when DO_A.
+// Branch-free: LARFB runs the MFMAs of rows above its diagonal block too,
+// on the zeros its staging put there.
+template <int NB, bool UNIT, bool DO_U, bool DO_A, bool DO_DMA>
+__device__ __forceinline__ void fpass(d4 (&acc)[NB / 16], const d4 &W2, d4 (&Wn)[2],
+                                      double (&o)[2][8], double (&tto)[4], const PassArgs<NB, UNIT> &P) {
+  constexpr int NG = NB / 32;
+  constexpr int NP = (DO_U ? NG : 0) + (DO_A ? NG : 0);
+  static_assert(!DO_DMA || NP >= FDma<NB>::NOPS, "one direct-to-LDS op per phase");
+  auto kind = [](int p) -> int {  // 1 = update, 0 = accumulate
+    if (DO_U && !DO_A) return 1;
+    if (!DO_U) return 0;
+    if (p == 0) return 1;
+    if (p == 2 * NG - 1) return 0;
+    return (p & 1) ? 1 : 0;
+  };
+  auto grp = [](int p) -> int {
+    if (!(DO_U && DO_A)) return p;
+    if (p == 0) return 0;
+    if (p == 2 * NG - 1) return NG - 1;
+    return (p & 1) ? (p + 1) / 2 : p / 2 - 1;
+  };
+  const int lane = threadIdx.x & 63, g = lane >> 4, pm = arow(lane & 15);
+#pragma unroll
+  for (int p = 0; p < NP; p++) {
+    if (DO_DMA && p < FDma<NB>::NOPS) dma_op<NB>(p, P.Vd, P.Vg, P.sd);
+    if (p + 1 < NP) {
+      if (kind(p + 1))
+        ld_u<NB, UNIT>(o[(p + 1) & 1], P.Vu, grp(p + 1), P.i0u, P.kbu);
+      else
+        ld_a<NB, UNIT>(o[(p + 1) & 1], P.Vw, grp(p + 1), P.i0w, P.kbw);
+    } else if (DO_A) {
+#pragma unroll
+      for (int sp = 0; sp < 4; sp++) tto[sp] = P.TTw[pm * LDT16 + 4 * g + sp];
+    }
+    const int G = grp(p);
+    if (kind(p)) {
+#pragma unroll
+      for (int sp = 0; sp < 4; sp++)
+#pragma unroll
+        for (int i = 0; i < 2; i++) acc[2 * G + i] = mfma(o[p & 1][2 * sp + i], W2[sp], acc[2 * G + i]);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int sp = 0; sp < 4; sp++) Wn[sp & 1] = mfma(o[p & 1][4 * i + sp], acc[2 * G + i][sp], Wn[sp & 1]);
+    }
+    // next phase's LDS reads early in the phase, between its first MFMAs;
+    // no instruction crosses a phase boundary, so the operands are always
+    // one phase ahead and an accumulate phase never follows its update
+    // directly
+    // (the first MFMA goes before them: the wait for this phase's operands
+    // then does not cover the new reads)
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+    __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+    __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+    __builtin_amdgcn_sched_group_barrier(0x008, 6, 0);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// Columns kb..15 of ring slot Vb are zero (a ragged sub-block).
+template <int NB>
+__device__ __forceinline__ void zero_pad(double *Vb, const SubBlk &sb) {
+  constexpr int LDV = Smem<NB>::LDV;
+  for (int e = threadIdx.x; e < (16 - sb.kb) * NB; e += HQ_THREADS) {
+    const int c = sb.kb + e / NB, r = e - (c - sb.kb) * NB;
+    Vb[c * LDV + r] = 0.0;
+  }
+}
+// This thread's entry of T^T for sub-block sb (TT slot layout c*17 + p).
+__device__ __forceinline__ double tt_entry(const double *T, int ib, const SubBlk &sb) {
+  const int c = threadIdx.x >> 4, p = threadIdx.x & 15;
+  return (c < sb.kb && p <= c) ? T[(size_t)(sb.i0 + c) * ib + sb.p0 + p] : 0.0;
+}
+
+// LARFB: the staged columns are raw GEQRT tile columns (R above the
+// diagonal).  Make sub-block sb unit lower trapezoidal in LDS: rows
+// r <= i0 + c of column c become (r == i0 + c).  Runs between two barriers.
+template <int NB>
+__device__ __forceinline__ void fix_unit(double *Vb, const SubBlk &sb) {
+  constexpr int LDV = Smem<NB>::LDV;
+  const int R = min(sb.i0 + 16, NB);
+  for (int e = threadIdx.x; e < sb.kb * R; e += HQ_THREADS) {
+    const int c = e / R, r = e - c * R;
+    if (r <= sb.i0 + c) Vb[c * LDV + r] = (r == sb.i0 + c) ? 1.0 : 0.0;
+  }
+}
+
+template <int NB, bool UNIT>
+__device__ __forceinline__ void task_update_f(Smem<NB> &S, const double *V, const double *T, double *X,
+                                              double *A1, int ib, int slab) {
+  const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, g = lane >> 4, wave = tid >> 6;
+  const int col = slab * HQ_SLAB + 16 * wave + m;
+  const bool ok = col < NB;
+  double *a1col = UNIT ? nullptr : A1 + (size_t)col * NB;
+  const int nsub = sub_count<NB>(ib);
+  auto sub = [&](int s) { return sub_blk<NB>(min(s, nsub - 1), ib); };
+  d4 acc[NB / 16];
+  d4 W2 = d4{0.0, 0.0, 0.0, 0.0}, Wn[2], a1v = W2, a1o = W2;
+  double o[2][8], tto[4];
+  double treg;
+  // prologue: sub-blocks 0 and 1, T^T of both, the strip
+  {
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const SubBlk sj = sub(j);
+#pragma unroll
+      for (int q = 0; q < FDma<NB>::NOPS; q++) dma_op<NB>(q, fvbuf(S, j), V, sj);
+      if (sj.kb < 16) zero_pad<NB>(fvbuf(S, j), sj);
+      ftbuf(S, j)[(tid >> 4) * LDT16 + (tid & 15)] = tt_entry(T, ib, sj);
+    }
+    load_strip<NB>(acc, X, col, ok, 0);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (UNIT) {
+      fix_unit<NB>(fvbuf(S, 0), sub(0));
+      __syncthreads();
+    }
+    ld_a<NB, UNIT>(o[0], fvbuf(S, 0), 0, 0, 0);
+  }
+  // pass s: (U) X -= V_s W2_s, (A) W_{s+1} += V_{s+1}^T X, loads of s+3
+  auto pre = [&](int s, PassArgs<NB, UNIT> &P, SubBlk &sw) {
+    const SubBlk su = sub(s < 0 ? 0 : s);
+    sw = sub(s + 1);
+    treg = tt_entry(T, ib, sub(s + 2));
+    if (!UNIT) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int row = 4 * g + v;
+        a1v[v] = (ok && row < sw.kb) ? a1col[sw.i0 + row] : 0.0;
+      }
+    }
+    P.sd = sub(s + 3);
+    P.Vd = fvbuf(S, (s + 3) & 3);
+    P.Vg = V;
+    if (s + 3 < nsub && P.sd.kb < 16) zero_pad<NB>(P.Vd, P.sd);
+    Wn[0] = d4{0.0, 0.0, 0.0, 0.0};
+    Wn[1] = Wn[0];
+    P.Vu = fvbuf(S, (s + 4) & 3);
+    P.Vw = fvbuf(S, (s + 1) & 3);
+    P.TTw = ftbuf(S, (s + 1) & 3);
+    P.i0u = su.i0; P.kbu = su.kb; P.i0w = sw.i0; P.kbw = sw.kb;
+  };
+  // after pass s: W2_{s+1}, its A1 rows (stored once the barrier is passed,
+  // so the counted wait never waits on stores), T^T of s+2 into the ring
+  auto post = [&](int s, const PassArgs<NB, UNIT> &P, const SubBlk &sw) {
+    const d4 W = UNIT ? Wn[0] + Wn[1] : (Wn[0] + Wn[1]) + a1v;
+    d4 W2n = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int sp = 0; sp < 4; sp++) W2n = mfma(tto[sp], W[sp], W2n);
+    a1o = a1v - W2n;
+    W2 = -W2n;
+    ld_u<NB, UNIT>(o[0], P.Vw, 0, sw.i0, sw.kb);
+    ftbuf(S, (s + 2) & 3)[(tid >> 4) * LDT16 + (tid & 15)] = treg;
+    __builtin_amdgcn_s_waitcnt(FDma<NB>::WAIT);  // sub-block s+2 has landed
+    __syncthreads();
+    if (UNIT && s + 2 < nsub) {
+      fix_unit<NB>(fvbuf(S, (s + 2) & 3), sub(s + 2));
+      __syncthreads();
+    }
+    if (!UNIT && ok) {
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        if (4 * g + v < sw.kb) a1col[sw.i0 + 4 * g + v] = a1o[v];
+    }
+  };
+  {
+    PassArgs<NB, UNIT> P;
+    SubBlk sw;
+    pre(-1, P, sw);
+    fpass<NB, UNIT, false, true, true>(acc, W2, Wn, o, tto, P);
+    post(-1, P, sw);
+  }
+  for (int s = 0; s + 1 < nsub; s++) {
+    PassArgs<NB, UNIT> P;
+    SubBlk sw;
+    pre(s, P, sw);
+    fpass<NB, UNIT, true, true, true>(acc, W2, Wn, o, tto, P);
+    post(s, P, sw);
+  }
+  {
+    PassArgs<NB, UNIT> P;
+    SubBlk sw;
+    pre(nsub - 1, P, sw);
+    fpass<NB, UNIT, true, false, false>(acc, W2, Wn, o, tto, P);
+  }
+  // no direct-to-LDS load may still be in flight when the next task reuses LDS
+  __builtin_amdgcn_s_waitcnt(0);
+  store_strip<NB>(acc, X, col, ok, 0);
+}
+
 template <int NB, int WB>
 __device__ __forceinline__ void task_larfb(Smem<NB> &S, const double *V, const double *T, double *C,
                                            int ib, int slab) {
-  task_update<NB, WB, true>(S, V, T, C, nullptr, ib, slab);
+  if constexpr (NB >= 128)
+    task_update_f<NB, true>(S, V, T, C, nullptr, ib, slab);
+  else
+    task_update<NB, WB, true>(S, V, T, C, nullptr, ib, slab);
 }
 
 template <int NB, int WB>
 __device__ __forceinline__ void task_ssrfb(Smem<NB> &S, double *A1, double *A2, const double *V,
                                            const double *T, int ib, int slab) {
-  task_update<NB, WB, false>(S, V, T, A2, A1, ib, slab);
+  if constexpr (NB >= 128)
+    task_update_f<NB, false>(S, V, T, A2, A1, ib, slab);
+  else
+    task_update<NB, WB, false>(S, V, T, A2, A1, ib, slab);
 }
 
 __device__ __forceinline__ double *tile_ptr(const hq_geom &G, int i, int j) {

@@ -9,7 +9,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-12
-CASES = [(64, 16), (64, 12), (128, 32), (128, 8), (256, 32), (256, 16), (256, 24)]
+CASES = [(64, 16), (64, 12), (128, 32), (128, 8), (128, 5), (256, 32), (256, 16), (256, 24), (256, 20)]
 
 
 def dev(a):

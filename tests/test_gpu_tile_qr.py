@@ -30,7 +30,7 @@ def run(hq, At, MT, NT, nb, ib, serial=False):
 
 
 GEOMS = [(2, 2, 64, 16), (4, 4, 64, 16), (3, 3, 128, 32), (5, 2, 64, 8), (2, 5, 64, 16),
-         (4, 4, 256, 32), (6, 6, 64, 12)]
+         (4, 4, 256, 32), (6, 6, 64, 12), (3, 3, 256, 20), (3, 2, 128, 24), (2, 3, 128, 32)]
 
 
 @pytest.mark.parametrize("serial", [True, False])
