@@ -820,9 +820,8 @@ __device__ __forceinline__ void task_update(Smem<NB> &S, const double *V, const 
 // W2_{s+1} = -T_{s+1}^T W_{s+1} (four MFMAs) and, for SSRFB, the A1 rows of
 // sub-block s+1 are written.  V sub-blocks live in a ring of four LDS
 // buffers of 16 columns: the direct-to-LDS loads of sub-block s+3 are issued
-// one per phase during pass s, and the end of pass s waits (counted) only
-// for sub-block s+2, issued a whole pass earlier.  One barrier per
-// sub-block.
+// one per phase in the first half of pass s and waited for at its end.  One
+// barrier per sub-block.
 // --------------------------------------------------------------------------
 struct SubBlk {
   int i0, kb, p0;  // first reflector (tile column), width, row offset inside its T block
@@ -835,6 +834,23 @@ __device__ __forceinline__ SubBlk sub_blk(int s, int ib) {
   r.p0 = 16 * h;
   r.i0 = b * ib + r.p0;
   r.kb = min(16, min(ib, NB - b * ib) - r.p0);
+  return r;
+}
+// The sub-block after x (x itself when x is the last one).
+template <int NB>
+__device__ __forceinline__ SubBlk next_sub(const SubBlk &x, int ib) {
+  if (x.i0 + x.kb >= NB) return x;
+  const int i0b = x.i0 - x.p0, kbb = min(ib, NB - i0b);
+  SubBlk r;
+  if (x.p0 + 16 < kbb) {
+    r.i0 = x.i0 + 16;
+    r.p0 = x.p0 + 16;
+    r.kb = min(16, kbb - r.p0);
+  } else {
+    r.i0 = i0b + ib;
+    r.p0 = 0;
+    r.kb = min(16, min(ib, NB - r.i0));
+  }
   return r;
 }
 template <int NB>
@@ -857,8 +873,6 @@ template <int NB>
 struct FDma {
   static constexpr int HALVES = NB / 128;
   static constexpr int NOPS = 16 * HALVES / (HQ_THREADS / 64);
-  // s_waitcnt immediate: vmcnt(NOPS), expcnt untouched, lgkmcnt(0)
-  static constexpr int WAIT = (NOPS & 15) | ((NOPS >> 4) << 14) | (7 << 4);
 };
 // Load op q (< NOPS) of this wave for sub-block sb into ring slot Vb: 128
 // rows of one column per op.  Ops past the block's width repeat its last
@@ -986,10 +1000,20 @@ __device__ __forceinline__ void zero_pad(double *Vb, const SubBlk &sb) {
     Vb[c * LDV + r] = 0.0;
   }
 }
-// This thread's entry of T^T for sub-block sb (TT slot layout c*17 + p).
-__device__ __forceinline__ double tt_entry(const double *T, int ib, const SubBlk &sb) {
+// This thread's entry of T^T for sub-block sb (TT slot layout c*17 + p),
+// as a raw load clamped to an entry of the block, and the 0/1 factor that
+// keeps only the upper triangle of the block.  The factor is applied where
+// the value is stored, after the pass: a selected load would be turned
+// into a load under an exec mask followed at once by a wait for every
+// load in flight.
+__device__ __forceinline__ double tt_raw(const double *T, int ib, const SubBlk &sb) {
   const int c = threadIdx.x >> 4, p = threadIdx.x & 15;
-  return (c < sb.kb && p <= c) ? T[(size_t)(sb.i0 + c) * ib + sb.p0 + p] : 0.0;
+  const int cc = min(c, sb.kb - 1);
+  return T[(size_t)(sb.i0 + cc) * ib + sb.p0 + min(p, cc)];
+}
+__device__ __forceinline__ double tt_mask(const SubBlk &sb) {
+  const int c = threadIdx.x >> 4, p = threadIdx.x & 15;
+  return (c < sb.kb && p <= c) ? 1.0 : 0.0;
 }
 
 // LARFB: the staged columns are raw GEQRT tile columns (R above the
@@ -1008,12 +1032,14 @@ __device__ __forceinline__ void fix_unit(double *Vb, const SubBlk &sb) {
 template <int NB, bool UNIT>
 __device__ __forceinline__ void task_update_f(Smem<NB> &S, const double *V, const double *T, double *X,
                                               double *A1, int ib, int slab) {
+  static_assert(NB % HQ_SLAB == 0, "slabs cover the tile exactly");
   const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, g = lane >> 4, wave = tid >> 6;
   const int col = slab * HQ_SLAB + 16 * wave + m;
-  const bool ok = col < NB;
+  const bool ok = true;
   double *a1col = UNIT ? nullptr : A1 + (size_t)col * NB;
   const int nsub = sub_count<NB>(ib);
-  auto sub = [&](int s) { return sub_blk<NB>(min(s, nsub - 1), ib); };
+  // sub-blocks s, s+1, s+2, s+3 of the current pass (clamped to the last)
+  SubBlk w0 = sub_blk<NB>(0, ib), w1 = w0, w2 = next_sub<NB>(w0, ib), w3 = next_sub<NB>(w2, ib);
   d4 acc[NB / 16];
   d4 W2 = d4{0.0, 0.0, 0.0, 0.0}, Wn[2], a1v = W2, a1o = W2;
   double o[2][8], tto[4];
@@ -1022,34 +1048,33 @@ __device__ __forceinline__ void task_update_f(Smem<NB> &S, const double *V, cons
   {
 #pragma unroll
     for (int j = 0; j < 2; j++) {
-      const SubBlk sj = sub(j);
+      const SubBlk sj = j ? w2 : w0;
 #pragma unroll
       for (int q = 0; q < FDma<NB>::NOPS; q++) dma_op<NB>(q, fvbuf(S, j), V, sj);
       if (sj.kb < 16) zero_pad<NB>(fvbuf(S, j), sj);
-      ftbuf(S, j)[(tid >> 4) * LDT16 + (tid & 15)] = tt_entry(T, ib, sj);
+      ftbuf(S, j)[(tid >> 4) * LDT16 + (tid & 15)] = tt_raw(T, ib, sj) * tt_mask(sj);
     }
     load_strip<NB>(acc, X, col, ok, 0);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (UNIT) {
-      fix_unit<NB>(fvbuf(S, 0), sub(0));
+      fix_unit<NB>(fvbuf(S, 0), w0);
       __syncthreads();
     }
     ld_a<NB, UNIT>(o[0], fvbuf(S, 0), 0, 0, 0);
   }
   // pass s: (U) X -= V_s W2_s, (A) W_{s+1} += V_{s+1}^T X, loads of s+3
   auto pre = [&](int s, PassArgs<NB, UNIT> &P, SubBlk &sw) {
-    const SubBlk su = sub(s < 0 ? 0 : s);
-    sw = sub(s + 1);
-    treg = tt_entry(T, ib, sub(s + 2));
+    const SubBlk su = w0;
+    sw = w1;
+    treg = tt_raw(T, ib, w2);
     if (!UNIT) {
+      // rows past a ragged sub-block load a clamped neighbour: their W rows
+      // meet only zero rows and columns of T and are never stored
 #pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int row = 4 * g + v;
-        a1v[v] = (ok && row < sw.kb) ? a1col[sw.i0 + row] : 0.0;
-      }
+      for (int v = 0; v < 4; v++) a1v[v] = a1col[min(sw.i0 + 4 * g + v, NB - 1)];
     }
-    P.sd = sub(s + 3);
+    P.sd = w3;
     P.Vd = fvbuf(S, (s + 3) & 3);
     P.Vg = V;
     if (s + 3 < nsub && P.sd.kb < 16) zero_pad<NB>(P.Vd, P.sd);
@@ -1060,8 +1085,9 @@ __device__ __forceinline__ void task_update_f(Smem<NB> &S, const double *V, cons
     P.TTw = ftbuf(S, (s + 1) & 3);
     P.i0u = su.i0; P.kbu = su.kb; P.i0w = sw.i0; P.kbw = sw.kb;
   };
-  // after pass s: W2_{s+1}, its A1 rows (stored once the barrier is passed,
-  // so the counted wait never waits on stores), T^T of s+2 into the ring
+  // after pass s: W2_{s+1}, its A1 rows (stored once the barrier is passed:
+  // the wait before the barrier then does not cover them), T^T of s+2 into
+  // the ring
   auto post = [&](int s, const PassArgs<NB, UNIT> &P, const SubBlk &sw) {
     const d4 W = UNIT ? Wn[0] + Wn[1] : (Wn[0] + Wn[1]) + a1v;
     d4 W2n = d4{0.0, 0.0, 0.0, 0.0};
@@ -1070,18 +1096,31 @@ __device__ __forceinline__ void task_update_f(Smem<NB> &S, const double *V, cons
     a1o = a1v - W2n;
     W2 = -W2n;
     ld_u<NB, UNIT>(o[0], P.Vw, 0, sw.i0, sw.kb);
-    ftbuf(S, (s + 2) & 3)[(tid >> 4) * LDT16 + (tid & 15)] = treg;
-    __builtin_amdgcn_s_waitcnt(FDma<NB>::WAIT);  // sub-block s+2 has landed
+    ftbuf(S, (s + 2) & 3)[(tid >> 4) * LDT16 + (tid & 15)] = treg * tt_mask(w2);
+    // Every load of the pass has landed, sub-block s+3 included (issued in
+    // the first half of the pass).  A counted wait (vmcnt(NOPS), relying on
+    // in-order completion) let results at n = 16384 go wrong: direct-to-LDS
+    // loads are not taken to complete in order here.
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (UNIT && s + 2 < nsub) {
-      fix_unit<NB>(fvbuf(S, (s + 2) & 3), sub(s + 2));
+      fix_unit<NB>(fvbuf(S, (s + 2) & 3), w2);
       __syncthreads();
     }
-    if (!UNIT && ok) {
+    if (!UNIT) {
+      if (sw.kb == 16 && (sw.i0 & 3) == 0) {
+        *reinterpret_cast<d4 *>(a1col + sw.i0 + 4 * g) = a1o;
+      } else {
 #pragma unroll
-      for (int v = 0; v < 4; v++)
-        if (4 * g + v < sw.kb) a1col[sw.i0 + 4 * g + v] = a1o[v];
+        for (int v = 0; v < 4; v++)
+          if (4 * g + v < sw.kb) a1col[sw.i0 + 4 * g + v] = a1o[v];
+      }
     }
+    // advance the sub-block window
+    w0 = w1;
+    w1 = w2;
+    w2 = w3;
+    w3 = next_sub<NB>(w3, ib);
   };
   {
     PassArgs<NB, UNIT> P;

@@ -77,3 +77,31 @@ def test_repeated_runs_are_identical(hq, orc):
     r3 = run(hq, At, MT, NT, nb, ib, serial=True)
     for a, b, c in zip(r1, r2, r3):
         assert np.array_equal(a, b) and np.array_equal(a, c)
+
+
+@pytest.mark.parametrize("n,nb,ib", [(8192, 256, 32), (6144, 256, 20), (4096, 128, 32)])
+def test_large_colnorm_invariant(hq, n, nb, ib):
+    """Whole-chip runs (hundreds of workgroups racing through the DAG): Q is
+    orthogonal, so every column of R has the norm of the same column of A.
+    Size-independent, so it is checked at sizes the oracle cannot finish
+    quickly; races in the pipelined kernels show up here first."""
+    import torch
+    MT = NT = n // nb
+    print(f"tile_qr colnorm n={n} nb={nb} ib={ib}", flush=True)
+    A = torch.empty(MT * NT * nb * nb, dtype=torch.float64, device="cuda")
+    hq.fill_uniform_dev(MT, NT, nb, 4242 + n, A)
+    A0 = A.clone()
+    TAU = torch.zeros(MT * NT * nb, dtype=torch.float64, device="cuda")
+    hq.tile_qr_dev(MT, NT, nb, ib, A, TAU, None)
+    hq.sync()
+    a = A0.view(NT, MT, nb, nb)   # [tile col][tile row][col][row]
+    f = A.view(NT, MT, nb, nb)
+    tj = torch.arange(NT, device="cuda").view(NT, 1, 1, 1)
+    ti = torch.arange(MT, device="cuda").view(1, MT, 1, 1)
+    col = torch.arange(nb, device="cuda").view(1, 1, nb, 1)
+    row = torch.arange(nb, device="cuda").view(1, 1, 1, nb)
+    upper = (ti * nb + row) <= (tj * nb + col)
+    an = (a * a).sum(dim=(1, 3)).sqrt()
+    rn = (f * f * upper).sum(dim=(1, 3)).sqrt()
+    assert bool(torch.isfinite(A).all())
+    assert ((an - rn).abs() / an).max().item() < 1e-12
