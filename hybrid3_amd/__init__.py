@@ -72,6 +72,13 @@ SIGNATURES = {
     "hq_tile_make_q_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp]),
     "hq_dag_host": (C.c_int, [C.c_int] * 6 + [C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int,
                                              C.POINTER(C.c_int)]),
+    "hq_batched_tile_qr_dev": (C.c_int, [C.c_int] * 3 + [C.POINTER(C.c_int)] * 2 + [C.POINTER(_vp)] * 3 + [_vp]),
+    "hq_batched_fill_uniform_dev": (C.c_int, [C.c_int] * 2 + [C.POINTER(C.c_int)] * 2
+                                    + [C.POINTER(C.c_uint64), C.POINTER(_vp), _vp]),
+    "hq_cm_to_tile_dev": (C.c_int, [C.c_int] * 2 + [C.POINTER(C.c_int)] * 2
+                          + [C.POINTER(_vp), C.POINTER(C.c_int), C.POINTER(_vp), _vp]),
+    "hq_tile_to_cm_dev": (C.c_int, [C.c_int] * 2 + [C.POINTER(C.c_int)] * 2
+                          + [C.POINTER(_vp), C.POINTER(_vp), C.POINTER(C.c_int), _vp]),
     "hq_sync": (C.c_int, [_vp]),
     "hq_finalize": (None, []),
 }
@@ -155,6 +162,39 @@ def tile_qr_dev(MT, NT, nb, ib, A, TAU, T=None, stream=None, serial=False):
 
 def fill_uniform_dev(MT, NT, nb, seed, A, stream=None):
     check(lib().hq_fill_uniform_dev(MT, NT, nb, seed, _ptr(A), _stream(stream)))
+
+
+def _iarr(xs):
+    return (C.c_int * len(xs))(*[int(x) for x in xs])
+
+
+def _parr(ptrs):
+    return (_vp * len(ptrs))(*[p if isinstance(p, int) else _ptr(p) for p in ptrs])
+
+
+def batched_tile_qr_dev(nb, ib, MT, NT, A, TAU, T=None, stream=None):
+    """Factor a batch in one launch.  MT, NT: per-matrix tile grids; A, TAU
+    (and T): per-matrix device tensors or raw device addresses (int)."""
+    n = len(MT)
+    Tp = None if T is None else _parr(T)
+    check(lib().hq_batched_tile_qr_dev(n, nb, ib, _iarr(MT), _iarr(NT), _parr(A), _parr(TAU), Tp,
+                                       _stream(stream)))
+
+
+def batched_fill_uniform_dev(nb, m, n, seeds, A, stream=None):
+    cnt = len(m)
+    sd = (C.c_uint64 * cnt)(*[int(x) for x in seeds])
+    check(lib().hq_batched_fill_uniform_dev(cnt, nb, _iarr(m), _iarr(n), sd, _parr(A), _stream(stream)))
+
+
+def cm_to_tile_dev(nb, m, n, Cs, ldc, A, stream=None):
+    check(lib().hq_cm_to_tile_dev(len(m), nb, _iarr(m), _iarr(n), _parr(Cs), _iarr(ldc), _parr(A),
+                                  _stream(stream)))
+
+
+def tile_to_cm_dev(nb, m, n, A, Cs, ldc, stream=None):
+    check(lib().hq_tile_to_cm_dev(len(m), nb, _iarr(m), _iarr(n), _parr(A), _parr(Cs), _iarr(ldc),
+                                  _stream(stream)))
 
 
 def schedule_info(MT, NT, nb, ib):

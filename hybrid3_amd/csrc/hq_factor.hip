@@ -1206,8 +1206,11 @@ __device__ __forceinline__ void run_task(Smem<NB> &S, const hq_geom &G, const hq
 // every dependency's flag equals this run's epoch, runs, then publishes its own
 // flag with release semantics.  Dependencies have smaller indices and were
 // claimed by running workgroups, so every wait ends.
+// A batch passes one geometry per matrix (geoms[task.mat]); a single
+// factorisation passes geoms = nullptr and its geometry by value.
 template <int NB, int WB>
-__global__ __launch_bounds__(HQ_THREADS) void sched_kernel(hq_geom G, const hq_task *__restrict__ tasks,
+__global__ __launch_bounds__(HQ_THREADS) void sched_kernel(hq_geom G, const hq_geom *__restrict__ geoms,
+                                                          const hq_task *__restrict__ tasks,
                                                           int ntasks, int *flags, int epoch,
                                                           int *counter, long long *trace) {
   __shared__ Smem<NB> S;
@@ -1258,7 +1261,10 @@ __global__ __launch_bounds__(HQ_THREADS) void sched_kernel(hq_geom G, const hq_t
       trace[4 * t + 1] = wall_clock64();
       trace[4 * t + 3] = blockIdx.x;
     }
-    run_task<NB, WB>(S, G, tk);
+    if (geoms)
+      run_task<NB, WB>(S, geoms[tk.mat], tk);
+    else
+      run_task<NB, WB>(S, G, tk);
     __syncthreads();
     done = t;
   }
@@ -1394,6 +1400,33 @@ __global__ void fill_uniform_tile_kernel(double *A, int MT, int NT, int nb, uint
   }
 }
 
+// Layout operations on a batch, one matrix per blockIdx.y, grid-stride over
+// its tiles: OP 0 fills uniform[-1,1) (value of (row, col) from
+// (seed, col*m + row), as a column-major m x n fill), OP 1 converts
+// column-major -> tile layout, OP 2 tile layout -> column-major.  Tile
+// entries past row m or column n are written as zero by OP 0 and OP 1.
+template <int OP>
+__global__ __launch_bounds__(256) void batched_layout_kernel(const hq_bmat *__restrict__ mats, int nb) {
+  const hq_bmat b = mats[blockIdx.y];
+  const size_t ts = (size_t)nb * nb;
+  const size_t total = ts * (size_t)b.MT * b.NT;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const size_t t = e / ts, w = e - t * ts;
+    const size_t ti = t % b.MT, tj = t / b.MT;
+    const size_t c = w / nb, r = w - c * nb;
+    const size_t row = ti * nb + r, col = tj * nb + c;
+    const bool in = row < (size_t)b.m && col < (size_t)b.n;
+    if (OP == 0) {
+      b.A[e] = in ? uniform_pm1(b.seed, col * b.m + row) : 0.0;
+    } else if (OP == 1) {
+      b.A[e] = in ? b.C[row + col * (size_t)b.ld] : 0.0;
+    } else {
+      if (in) b.C[row + col * (size_t)b.ld] = b.A[e];
+    }
+  }
+}
+
 }  // namespace hq
 
 // ---------------------------------------------------------------------------
@@ -1422,14 +1455,29 @@ static hipError_t hq_dispatch(int nb, int ib, Fn &&fn) {
   constexpr int NB = decltype(nbc)::value;      \
   constexpr int WB = decltype(wbc)::value;
 
-hipError_t hq_launch_sched(const hq_geom &G, const hq_task *tasks, int ntasks, int *flags, int epoch,
-                           int *counter, int grid, long long *trace, hipStream_t st) {
+hipError_t hq_launch_sched(const hq_geom &G, const hq_geom *geoms, const hq_task *tasks, int ntasks,
+                           int *flags, int epoch, int *counter, int grid, long long *trace, hipStream_t st) {
   hipLaunchKernelGGL(hq::reset_counter_kernel, dim3(1), dim3(64), 0, st, counter);
   return hq_dispatch(G.nb, G.ib, [&](auto nbc, auto wbc) {
     HQ_NBWB
-    hipLaunchKernelGGL((hq::sched_kernel<NB, WB>), dim3(grid), dim3(HQ_THREADS), 0, st, G, tasks,
+    hipLaunchKernelGGL((hq::sched_kernel<NB, WB>), dim3(grid), dim3(HQ_THREADS), 0, st, G, geoms, tasks,
                        ntasks, flags, epoch, counter, trace);
   });
+}
+
+hipError_t hq_launch_batched_layout(int op, const hq_bmat *mats, int count, int nb, int max_tiles,
+                                    hipStream_t st) {
+  // enough blocks in x for the largest matrix at ~8 elements per thread
+  const size_t elems = (size_t)max_tiles * nb * nb;
+  const unsigned bx = (unsigned)std::min<size_t>(1024, (elems + 2047) / 2048);
+  const dim3 grid(bx, (unsigned)count);
+  if (op == 0)
+    hipLaunchKernelGGL(hq::batched_layout_kernel<0>, grid, dim3(256), 0, st, mats, nb);
+  else if (op == 1)
+    hipLaunchKernelGGL(hq::batched_layout_kernel<1>, grid, dim3(256), 0, st, mats, nb);
+  else
+    hipLaunchKernelGGL(hq::batched_layout_kernel<2>, grid, dim3(256), 0, st, mats, nb);
+  return hipGetLastError();
 }
 
 hipError_t hq_launch_task(const hq_geom &G, const hq_task &tk, hipStream_t st) {

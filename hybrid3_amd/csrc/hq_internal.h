@@ -25,6 +25,7 @@ struct hq_task {
   int32_t r;      // tile row (TSQRT / SSRFB)
   int32_t j;      // tile column (LARFB / SSRFB)
   int32_t slab;   // 64-column slab of tile column j (LARFB / SSRFB)
+  int32_t mat;    // matrix of a batch (index into the geometry table; 0 otherwise)
   int32_t ndep;
   int32_t dep[HQ_MAXDEP];
 };
@@ -41,6 +42,15 @@ struct hq_geom {
   double *X;    // matrix updated by LARFB / SSRFB: A itself while factorising,
                 // the Q / C array (MT x XNT tiles) while applying Q^T
   int XNT;
+};
+
+// One matrix of a batch for the layout kernels: m x n entries in an MT x NT
+// grid of nb x nb tiles (tile layout as hq_geom; entries past m or n are 0).
+struct hq_bmat {
+  double *A;      // tile layout
+  double *C;      // column-major m x n, leading dimension ld (conversions)
+  int m, n, MT, NT, ld;
+  uint64_t seed;  // uniform fill: value(row, col) from (seed, col*m + row)
 };
 
 #define HQ_THREADS 256

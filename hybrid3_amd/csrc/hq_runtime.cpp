@@ -20,8 +20,8 @@
 #include "hq_internal.h"
 
 // launch wrappers (hq_factor.hip)
-hipError_t hq_launch_sched(const hq_geom &G, const hq_task *tasks, int ntasks, int *flags, int epoch,
-                           int *counter, int grid, long long *trace, hipStream_t st);
+hipError_t hq_launch_sched(const hq_geom &G, const hq_geom *geoms, const hq_task *tasks, int ntasks,
+                           int *flags, int epoch, int *counter, int grid, long long *trace, hipStream_t st);
 hipError_t hq_launch_task(const hq_geom &G, const hq_task &tk, hipStream_t st);
 hipError_t hq_launch_geqrt(int nb, int ib, double *A, double *T, double *TAU, hipStream_t st);
 hipError_t hq_launch_tsqrt(int nb, int ib, double *A1, double *A2, double *T, double *TAU,
@@ -31,6 +31,8 @@ hipError_t hq_launch_larfb(int nb, int ib, const double *V, const double *T, dou
 hipError_t hq_launch_ssrfb(int nb, int ib, double *A1, double *A2, const double *V, const double *T,
                            hipStream_t st);
 hipError_t hq_launch_fill_uniform(double *A, int MT, int NT, int nb, uint64_t seed, hipStream_t st);
+hipError_t hq_launch_batched_layout(int op, const hq_bmat *mats, int count, int nb, int max_tiles,
+                                    hipStream_t st);
 hipError_t hq_launch_tbuild(const hq_geom &G, const int2 *tiles, int ntiles, hipStream_t st);
 hipError_t hq_launch_identity(double *X, int MT, int XNT, int nb, int full, hipStream_t st);
 hipError_t hq_launch_zero_reflectors(double *A, int MT, int NT, int nb, hipStream_t st);
@@ -281,6 +283,9 @@ struct Context {
   int epoch = 0;
   std::map<std::tuple<int, int, int, int, int>, std::unique_ptr<CachedSched>> scheds;
   std::map<std::pair<int, int>, std::unique_ptr<DevBuf>> tilelists;  // (MT, NT) -> int2 list
+  // batches: key = (nb, ib, MT_0, NT_0, MT_1, NT_1, ...)
+  std::map<std::vector<int>, std::unique_ptr<CachedSched>> batch_scheds;
+  DevBuf geoms, bmats, Tbatch;
 };
 
 std::mutex g_mu;
@@ -306,6 +311,35 @@ int get_ctx(Context **out) {
 // NULL is HIP's default (null) stream, as for every HIP API; the host-pointer
 // entry points pass the library's own stream explicitly.
 hipStream_t pick_stream(Context *, void *s) { return (hipStream_t)s; }
+
+// flags: one int per task, zero-initialised; epochs start at 1
+int reserve_flags(Context *c, int n) {
+  size_t fbytes = sizeof(int) * (size_t)n;
+  if (c->flags.bytes < fbytes) {
+    if (c->flags.reserve(fbytes) != HQ_OK) return g_err;
+    HQ_HIP(hipMemset(c->flags.p, 0, c->flags.bytes));
+    c->epoch = 0;
+  }
+  return HQ_OK;
+}
+
+// Claim order of tasks tl (canonical order, dependencies as indices into
+// tl) from the list-scheduling simulation; dependencies remapped.
+std::vector<hq_task> claim_order(std::vector<hq_task> &&tl, int nb, int ib, int P) {
+  DagBuilder b(1, 1, nb, ib);
+  b.t = std::move(tl);
+  std::vector<int> ord = b.order(P);
+  const int n = (int)b.t.size();
+  std::vector<int> pos(n);
+  for (int i = 0; i < n; i++) pos[ord[i]] = i;
+  std::vector<hq_task> out(n);
+  for (int i = 0; i < n; i++) {
+    hq_task x = b.t[ord[i]];
+    for (int d = 0; d < x.ndep; d++) x.dep[d] = pos[x.dep[d]];
+    out[i] = x;
+  }
+  return out;
+}
 
 int get_schedule(Context *c, int MT, int NT, int nb, int ib, CachedSched **out, int XNT = -1) {
   // XNT < 0: factorisation DAG; XNT >= 1: Q^T application to MT x XNT tiles
@@ -333,13 +367,7 @@ int get_schedule(Context *c, int MT, int NT, int nb, int ib, CachedSched **out, 
   HQ_HIP(hipMemcpy(cs->tasks.p, tl.data(), sizeof(hq_task) * (size_t)n, hipMemcpyHostToDevice));
   cs->ntasks = n;
   cs->grid = std::min(grid, n);
-  // flags: one int per task, zero-initialised; epochs start at 1
-  size_t fbytes = sizeof(int) * (size_t)n;
-  if (c->flags.bytes < fbytes) {
-    if (c->flags.reserve(fbytes) != HQ_OK) return g_err;
-    HQ_HIP(hipMemset(c->flags.p, 0, c->flags.bytes));
-    c->epoch = 0;
-  }
+  if (reserve_flags(c, n) != HQ_OK) return g_err;
   *out = cs.get();
   c->scheds[key] = std::move(cs);
   return HQ_OK;
@@ -380,8 +408,8 @@ int tile_qr_dev_impl(int MT, int NT, int nb, int ib, double *A, double *TAU, dou
   CachedSched *cs;
   if (get_schedule(c, MT, NT, nb, ib, &cs) != HQ_OK) return g_err;
   c->epoch++;
-  HQ_HIP(hq_launch_sched(G, (const hq_task *)cs->tasks.p, cs->ntasks, (int *)c->flags.p, c->epoch,
-                         (int *)c->counter.p, cs->grid, trace, st));
+  HQ_HIP(hq_launch_sched(G, nullptr, (const hq_task *)cs->tasks.p, cs->ntasks, (int *)c->flags.p,
+                         c->epoch, (int *)c->counter.p, cs->grid, trace, st));
   return ok();
 }
 
@@ -430,8 +458,132 @@ int apply_qt_impl(int MT, int NT, int nb, int ib, const double *RY, const double
   if (get_schedule(c, MT, NT, nb, ib, &cs, XNT) != HQ_OK) return g_err;
   hq_geom G{(double *)RY, (double *)TAU, Tw, MT, NT, nb, ib, X, XNT};
   c->epoch++;
-  HQ_HIP(hq_launch_sched(G, (const hq_task *)cs->tasks.p, cs->ntasks, (int *)c->flags.p, c->epoch,
-                         (int *)c->counter.p, cs->grid, nullptr, st));
+  HQ_HIP(hq_launch_sched(G, nullptr, (const hq_task *)cs->tasks.p, cs->ntasks, (int *)c->flags.p,
+                         c->epoch, (int *)c->counter.p, cs->grid, nullptr, st));
+  return ok();
+}
+
+// Batch of independent factorisations, one persistent launch: every
+// matrix's DAG (the single-matrix DAG, canonical order) is concatenated with
+// its matrix index and the whole batch gets one claim order, so large
+// matrices' critical paths start first and small ones fill the gaps.
+int get_batch_schedule(Context *c, int count, int nb, int ib, const int *MT, const int *NT,
+                       CachedSched **out) {
+  std::vector<int> key;
+  key.reserve(2 + 2 * (size_t)count);
+  key.push_back(nb);
+  key.push_back(ib);
+  for (int i = 0; i < count; i++) {
+    key.push_back(MT[i]);
+    key.push_back(NT[i]);
+  }
+  auto it = c->batch_scheds.find(key);
+  if (it != c->batch_scheds.end()) {
+    *out = it->second.get();
+    return HQ_OK;
+  }
+  std::map<std::pair<int, int>, std::vector<hq_task>> per_geom;  // canonical DAG per (MT, NT)
+  std::vector<hq_task> all;
+  for (int i = 0; i < count; i++) {
+    auto &dag = per_geom[std::make_pair(MT[i], NT[i])];
+    if (dag.empty()) {
+      DagBuilder b(MT[i], NT[i], nb, ib);
+      b.build();
+      dag = b.t;
+    }
+    const int base = (int)all.size();
+    for (hq_task x : dag) {
+      x.mat = i;
+      for (int d = 0; d < x.ndep; d++) x.dep[d] += base;
+      all.push_back(x);
+    }
+  }
+  const int grid = c->num_cu > 0 ? c->num_cu : 256;
+  std::vector<hq_task> tl = claim_order(std::move(all), nb, ib, grid);
+  const int n = (int)tl.size();
+  std::unique_ptr<CachedSched> cs(new CachedSched());
+  if (cs->tasks.reserve(sizeof(hq_task) * (size_t)n) != HQ_OK) return g_err;
+  HQ_HIP(hipMemcpy(cs->tasks.p, tl.data(), sizeof(hq_task) * (size_t)n, hipMemcpyHostToDevice));
+  cs->ntasks = n;
+  cs->grid = std::min(grid, n);
+  if (reserve_flags(c, n) != HQ_OK) return g_err;
+  *out = cs.get();
+  c->batch_scheds[key] = std::move(cs);
+  return HQ_OK;
+}
+
+int batched_qr_impl(int count, int nb, int ib, const int *MT, const int *NT, double *const *A,
+                    double *const *TAU, double *const *T, void *stream) {
+  if (count < 1 || count > 65535) return set_error(HQ_ERR_ARG, "batch of %d matrices (1..65535)", count);
+  if (!MT || !NT || !A || !TAU) return set_error(HQ_ERR_ARG, "MT, NT, A and TAU arrays are required");
+  for (int i = 0; i < count; i++) {
+    if (!geometry_ok(MT[i], NT[i], nb, ib)) return g_err;
+    if (!A[i] || !TAU[i]) return set_error(HQ_ERR_ARG, "matrix %d: A and TAU must be device pointers", i);
+  }
+  Context *c;
+  if (get_ctx(&c) != HQ_OK) return g_err;
+  hipStream_t st = pick_stream(c, stream);
+  // T scratch for the matrices that bring none
+  size_t tdoubles = 0;
+  for (int i = 0; i < count; i++)
+    if (!T || !T[i]) tdoubles += (size_t)MT[i] * NT[i] * ib * nb;
+  if (tdoubles && c->Tbatch.reserve(sizeof(double) * tdoubles) != HQ_OK) return g_err;
+  std::vector<hq_geom> g(count);
+  size_t toff = 0;
+  for (int i = 0; i < count; i++) {
+    double *Ti = T ? T[i] : nullptr;
+    if (!Ti) {
+      Ti = (double *)c->Tbatch.p + toff;
+      toff += (size_t)MT[i] * NT[i] * ib * nb;
+    }
+    g[i] = hq_geom{A[i], TAU[i], Ti, MT[i], NT[i], nb, ib, A[i], NT[i]};
+  }
+  CachedSched *cs;
+  if (get_batch_schedule(c, count, nb, ib, MT, NT, &cs) != HQ_OK) return g_err;
+  if (c->geoms.reserve(sizeof(hq_geom) * (size_t)count) != HQ_OK) return g_err;
+  HQ_HIP(hipMemcpyAsync(c->geoms.p, g.data(), sizeof(hq_geom) * (size_t)count, hipMemcpyHostToDevice, st));
+  c->epoch++;
+  HQ_HIP(hq_launch_sched(g[0], (const hq_geom *)c->geoms.p, (const hq_task *)cs->tasks.p, cs->ntasks,
+                         (int *)c->flags.p, c->epoch, (int *)c->counter.p, cs->grid, nullptr, st));
+  return ok();
+}
+
+// Layout operations on a batch (op 0: fill uniform, 1: column-major -> tile,
+// 2: tile -> column-major); m x n entries inside ceil(m/nb) x ceil(n/nb)
+// tiles, padding zero.
+int batched_layout_impl(int op, int count, int nb, const int *m, const int *n, const uint64_t *seeds,
+                        const double *const *C, const int *ldc, double *const *A, void *stream) {
+  if (count < 1 || count > 65535) return set_error(HQ_ERR_ARG, "batch of %d matrices (1..65535)", count);
+  if (nb < 1) return set_error(HQ_ERR_ARG, "nb=%d", nb);
+  if (!m || !n || !A) return set_error(HQ_ERR_ARG, "m, n and A arrays are required");
+  std::vector<hq_bmat> v(count);
+  int max_tiles = 0;
+  for (int i = 0; i < count; i++) {
+    if (m[i] < 1 || n[i] < 1) return set_error(HQ_ERR_ARG, "matrix %d is %d x %d", i, m[i], n[i]);
+    if (!A[i]) return set_error(HQ_ERR_ARG, "matrix %d: A must be a device pointer", i);
+    hq_bmat b{};
+    b.A = A[i];
+    b.m = m[i];
+    b.n = n[i];
+    b.MT = (m[i] + nb - 1) / nb;
+    b.NT = (n[i] + nb - 1) / nb;
+    if (op == 0) {
+      b.seed = seeds ? seeds[i] : 0;
+    } else {
+      if (!C || !C[i] || !ldc || ldc[i] < m[i])
+        return set_error(HQ_ERR_ARG, "matrix %d: column-major pointer and ld >= m required", i);
+      b.C = (double *)C[i];
+      b.ld = ldc[i];
+    }
+    max_tiles = std::max(max_tiles, b.MT * b.NT);
+    v[i] = b;
+  }
+  Context *c;
+  if (get_ctx(&c) != HQ_OK) return g_err;
+  hipStream_t st = pick_stream(c, stream);
+  if (c->bmats.reserve(sizeof(hq_bmat) * (size_t)count) != HQ_OK) return g_err;
+  HQ_HIP(hipMemcpyAsync(c->bmats.p, v.data(), sizeof(hq_bmat) * (size_t)count, hipMemcpyHostToDevice, st));
+  HQ_HIP(hq_launch_batched_layout(op, (const hq_bmat *)c->bmats.p, count, nb, max_tiles, st));
   return ok();
 }
 
@@ -614,6 +766,27 @@ int hq_tile_make_q_dev(int MT, int NT, int nb, int ib, double *RY, const double 
   if (apply_qt_impl(MT, NT, nb, ib, RY, TAU, nullptr, Q, NT, stream) != HQ_OK) return g_err;
   HQ_HIP(hq_launch_zero_reflectors(RY, MT, NT, nb, st));
   return ok();
+}
+
+int hq_batched_tile_qr_dev(int count, int nb, int ib, const int *MT, const int *NT, double *const *A,
+                           double *const *TAU, double *const *T, void *stream) {
+  return batched_qr_impl(count, nb, ib, MT, NT, A, TAU, T, stream);
+}
+
+int hq_batched_fill_uniform_dev(int count, int nb, const int *m, const int *n, const uint64_t *seeds,
+                                double *const *A, void *stream) {
+  return batched_layout_impl(0, count, nb, m, n, seeds, nullptr, nullptr, A, stream);
+}
+
+int hq_cm_to_tile_dev(int count, int nb, const int *m, const int *n, const double *const *C,
+                      const int *ldc, double *const *A, void *stream) {
+  return batched_layout_impl(1, count, nb, m, n, nullptr, C, ldc, A, stream);
+}
+
+int hq_tile_to_cm_dev(int count, int nb, const int *m, const int *n, const double *const *A, double *const *C,
+                      const int *ldc, void *stream) {
+  return batched_layout_impl(2, count, nb, m, n, nullptr, (const double *const *)C, ldc,
+                             (double *const *)A, stream);
 }
 
 int hq_fill_uniform_dev(int MT, int NT, int nb, uint64_t seed, double *A, void *stream) {

@@ -119,6 +119,38 @@ int hq_tile_make_q_dev(int MT, int NT, int nb, int ib, double *RY,
 int hq_fill_uniform_dev(int MT, int NT, int nb, uint64_t seed, double *A,
                         void *stream);
 
+/* ---- Batches of independent matrices (no counterpart in the reference,
+ * which factors one matrix per call; for callers with thousands of small
+ * problems).  Arrays of length `count` are host arrays; the pointers in them
+ * are device pointers.  A matrix of m x n entries lives in tile layout in
+ * ceil(m/nb) x ceil(n/nb) tiles with the entries past row m / column n
+ * zero: Householder reflectors of a zero-padded matrix are zero in the
+ * padded rows, so R's leading n x n block is that of the unpadded matrix.
+ * count <= 65535. */
+
+/* Flat-tree tile QR of every matrix of the batch in ONE persistent launch:
+ * matrix i is an MT[i] x NT[i] grid of nb x nb tiles at A[i], TAU[i]
+ * (MT[i]*NT[i]*nb doubles), T[i] (MT[i]*NT[i]*ib*nb doubles, or T == NULL /
+ * T[i] == NULL for library scratch).  Same results as hq_tile_qr_dev on
+ * each matrix. */
+int hq_batched_tile_qr_dev(int count, int nb, int ib, const int *MT, const int *NT,
+                           double *const *A, double *const *TAU, double *const *T,
+                           void *stream);
+
+/* A[i] (tile layout, padding zero) <- m[i] x n[i] uniform[-1,1) values:
+ * entry (row, col) = SplitMix64 of (seeds[i], col*m[i] + row), the values of
+ * a column-major fill of the unpadded matrix. */
+int hq_batched_fill_uniform_dev(int count, int nb, const int *m, const int *n,
+                                const uint64_t *seeds, double *const *A, void *stream);
+
+/* Column-major (leading dimension ldc[i]) <-> tile layout conversions
+ * (mat_prod.h:57-67 layout, the reference's mat_prod.c:260-281 copies),
+ * zero padding written by the conversion into tile layout. */
+int hq_cm_to_tile_dev(int count, int nb, const int *m, const int *n, const double *const *C,
+                      const int *ldc, double *const *A, void *stream);
+int hq_tile_to_cm_dev(int count, int nb, const int *m, const int *n, const double *const *A,
+                      double *const *C, const int *ldc, void *stream);
+
 /* Synchronise `stream`; reports a scheduler timeout raised on the device. */
 int hq_sync(void *stream);
 

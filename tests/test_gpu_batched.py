@@ -1,0 +1,139 @@
+"""Batched uneven QR (no reference counterpart): many independent matrices
+factored by ONE persistent launch of the DAG scheduler, through the C-ABI.
+
+Pinned by:
+* the oracle's flat-tree tile QR of each zero-padded matrix (1e-12 of the
+  largest entry, same as the single-matrix parity tests);
+* bitwise equality with one hq_tile_qr_dev launch per matrix (same kernels,
+  same per-tile update order);
+* LAPACK's QR of the UNPADDED m x n matrix (numpy.linalg.qr): R unique up to
+  the signs of its rows, so the padding is exact;
+* Q orthogonal: every column of R keeps the norm of the same column of A
+  (whole-chip batch, sizes the oracle does not finish quickly)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def hq():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import hybrid3_amd
+    hybrid3_amd.lib()
+    return hybrid3_amd
+
+
+def make_batch(hq, sizes, nb, seed0):
+    import torch
+    tiles = [((m + nb - 1) // nb, (n + nb - 1) // nb) for m, n in sizes]
+    A = [torch.empty(MT * NT * nb * nb, dtype=torch.float64, device="cuda") for MT, NT in tiles]
+    hq.batched_fill_uniform_dev(nb, [m for m, _ in sizes], [n for _, n in sizes],
+                                [seed0 + i for i in range(len(sizes))], A)
+    return tiles, A
+
+
+SIZES = [(100, 70), (64, 64), (200, 33), (130, 129), (64, 32), (300, 150), (257, 200), (65, 65)]
+
+
+@pytest.mark.parametrize("nb,ib", [(64, 16), (64, 12), (128, 32)])
+def test_batched_matches_oracle_and_single(hq, orc, nb, ib):
+    import torch
+    print(f"batched nb={nb} ib={ib}", flush=True)
+    tiles, A = make_batch(hq, SIZES, nb, 900)
+    A0 = [a.clone() for a in A]
+    TAU = [torch.zeros(MT * NT * nb, dtype=torch.float64, device="cuda") for MT, NT in tiles]
+    hq.batched_tile_qr_dev(nb, ib, [t[0] for t in tiles], [t[1] for t in tiles], A, TAU)
+    hq.sync()
+    for i, ((m, n), (MT, NT)) in enumerate(zip(SIZES, tiles)):
+        # the oracle on the zero-padded matrix built by the same generator
+        cm = np.zeros((MT * nb, NT * nb), order="F")
+        cm[:m, :n] = orc.fill_uniform_cm(m, n, 900 + i)
+        At = orc.cm_to_tile(cm, nb)
+        assert np.array_equal(A0[i].cpu().numpy(), At), "device fill differs from the column-major fill"
+        RYo, TAUo = orc.tile_qr(At, MT, NT, nb, ib)
+        RY = A[i].cpu().numpy()
+        assert np.abs(RY - RYo).max() / np.abs(RYo).max() < 1e-12, (m, n)
+        assert np.abs(TAU[i].cpu().numpy() - TAUo).max() < 1e-12, (m, n)
+        # one launch per matrix gives the same bits
+        a1 = A0[i].clone()
+        t1 = torch.zeros_like(TAU[i])
+        hq.tile_qr_dev(MT, NT, nb, ib, a1, t1, None)
+        hq.sync()
+        assert torch.equal(a1, A[i]) and torch.equal(t1, TAU[i]), (m, n)
+
+
+def test_padding_is_exact_vs_lapack(hq, orc):
+    import torch
+    nb, ib = 64, 16
+    sizes = [(150, 90), (97, 97), (300, 41)]
+    print("batched padding vs numpy.linalg.qr", flush=True)
+    tiles, A = make_batch(hq, sizes, nb, 4100)
+    TAU = [torch.zeros(MT * NT * nb, dtype=torch.float64, device="cuda") for MT, NT in tiles]
+    hq.batched_tile_qr_dev(nb, ib, [t[0] for t in tiles], [t[1] for t in tiles], A, TAU)
+    hq.sync()
+    for i, ((m, n), (MT, NT)) in enumerate(zip(sizes, tiles)):
+        a = orc.fill_uniform_cm(m, n, 4100 + i)
+        R_lapack = np.linalg.qr(a, mode="r")[:n, :n]
+        R = np.triu(orc.tile_to_cm(A[i].cpu().numpy(), MT, NT, nb))[:n, :n]
+        d = np.sign(np.diag(R)) * np.sign(np.diag(R_lapack))
+        assert np.abs(R - d[:, None] * R_lapack).max() / np.abs(R_lapack).max() < 1e-12, (m, n)
+
+
+def test_layout_round_trip(hq):
+    import torch
+    nb = 64
+    sizes = [(70, 33), (128, 64), (1, 1), (200, 130)]
+    print("batched layout round trip", flush=True)
+    rng = np.random.default_rng(5)
+    cms = [torch.from_numpy(np.asfortranarray(rng.uniform(-1, 1, (m + 3, n))).ravel(order="F").copy()).cuda()
+           for m, n in sizes]
+    ld = [m + 3 for m, _ in sizes]
+    tiles = [((m + nb - 1) // nb, (n + nb - 1) // nb) for m, n in sizes]
+    A = [torch.full((MT * NT * nb * nb,), 7.0, dtype=torch.float64, device="cuda") for MT, NT in tiles]
+    hq.cm_to_tile_dev(nb, [m for m, _ in sizes], [n for _, n in sizes], cms, ld, A)
+    back = [torch.full_like(c, -3.0) for c in cms]
+    hq.tile_to_cm_dev(nb, [m for m, _ in sizes], [n for _, n in sizes], A, back, ld)
+    hq.sync()
+    for (m, n), (MT, NT), c, b, a in zip(sizes, tiles, cms, back, A):
+        cm = c.cpu().numpy().reshape(n, m + 3).T
+        bm = b.cpu().numpy().reshape(n, m + 3).T
+        assert np.array_equal(bm[:m, :], cm[:m, :])      # entries copied back exactly
+        assert np.all(bm[m:, :] == -3.0)                  # rows past m untouched
+        full = a.cpu().numpy().reshape(NT, MT, nb, nb)    # [tile col][tile row][col][row]
+        dense = full.transpose(1, 3, 0, 2).reshape(MT * nb, NT * nb)
+        assert np.array_equal(dense[:m, :n], cm[:m, :])
+        assert np.all(dense[m:, :] == 0.0) and np.all(dense[:, n:] == 0.0)
+
+
+def test_large_batch_colnorm_invariant(hq):
+    import torch
+    nb, ib = 64, 16
+    rng = np.random.default_rng(77)
+    count = 600
+    ms = rng.integers(64, 513, count)
+    ns = np.array([rng.integers(32, m + 1) for m in ms])
+    sizes = list(zip(ms.tolist(), ns.tolist()))
+    print(f"batched colnorm count={count}", flush=True)
+    tiles, A = make_batch(hq, sizes, nb, 7000)
+    A0 = [a.clone() for a in A]
+    TAU = [torch.zeros(MT * NT * nb, dtype=torch.float64, device="cuda") for MT, NT in tiles]
+    hq.batched_tile_qr_dev(nb, ib, [t[0] for t in tiles], [t[1] for t in tiles], A, TAU)
+    hq.sync()
+    worst = 0.0
+    for (MT, NT), a, f in zip(tiles, A0, A):
+        a = a.view(NT, MT, nb, nb)
+        f = f.view(NT, MT, nb, nb)
+        tj = torch.arange(NT, device="cuda").view(NT, 1, 1, 1)
+        ti = torch.arange(MT, device="cuda").view(1, MT, 1, 1)
+        col = torch.arange(nb, device="cuda").view(1, 1, nb, 1)
+        row = torch.arange(nb, device="cuda").view(1, 1, 1, nb)
+        upper = (ti * nb + row) <= (tj * nb + col)
+        an = (a * a).sum(dim=(1, 3)).sqrt()
+        rn = (f * f * upper).sum(dim=(1, 3)).sqrt()
+        live = an > 0          # zero padding columns have zero norm on both sides
+        assert torch.all(rn[~live] == 0)
+        worst = max(worst, ((an[live] - rn[live]).abs() / an[live]).max().item())
+    assert worst < 1e-12
