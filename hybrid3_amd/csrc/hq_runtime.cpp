@@ -190,15 +190,17 @@ struct DagBuilder {
       }
     }
   }
-  // Estimated durations (us) of each task kind for the priority computation.
+  // Estimated durations (us) of each task kind for the priority computation,
+  // from the measured per-task times at nb = 256 (GEQRT 590, TSQRT 630,
+  // LARFB 99, SSRFB 83 us per 64-column slab) scaled by the work of the kind.
   double cost(const hq_task &x) const {
-    double n3 = (double)nb * nb * nb;
-    double slabw = std::min(HQ_SLAB, nb);
+    const double f = nb / 256.0;
+    const double w = (double)std::min(HQ_SLAB, nb) / 64.0;  // slab width / 64
     switch (x.kind) {
-      case HQ_GEQRT: return (4.0 / 3.0) * n3 / 2.0e4;  // ~20 GFLOP/s per CU
-      case HQ_TSQRT: return 2.0 * n3 / 3.0e4;
-      case HQ_LARFB: return 2.0 * nb * nb * slabw / 1.5e5;
-      default: return 4.0 * nb * nb * slabw / 1.5e5;
+      case HQ_GEQRT: return 590.0 * f * f * f;
+      case HQ_TSQRT: return 630.0 * f * f * f;
+      case HQ_LARFB: return 99.0 * f * f * w;
+      default: return 83.0 * f * f * w;
     }
   }
   // List-scheduling simulation on P workers with bottom-level priorities;

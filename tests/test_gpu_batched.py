@@ -137,3 +137,18 @@ def test_large_batch_colnorm_invariant(hq):
         assert torch.all(rn[~live] == 0)
         worst = max(worst, ((an[live] - rn[live]).abs() / an[live]).max().item())
     assert worst < 1e-12
+
+
+def test_autotuner_plan_and_batch(hq):
+    """The tuner returns one measured (nb, ib) per matrix from the candidate
+    set, and the batch it plans factors correctly (launch per group)."""
+    from hybrid3_amd import batched as B
+    sizes = B.uneven_sizes(90, seed=11, m_range=(64, 700))
+    print("autotune 90 matrices", flush=True)
+    plan, report = B.autotune(sizes, sample=24, reps=1)
+    assert len(plan) == len(sizes)
+    assert all(tuple(p) in B.CANDIDATES for p in plan)
+    assert report["chosen"] in report["plans"]
+    bt = B.Batch(sizes, plan, seed0=3)
+    B.time_factor(bt, reps=1)
+    assert bt.colnorm_error() < 1e-12
