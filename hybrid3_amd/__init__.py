@@ -79,6 +79,7 @@ SIGNATURES = {
                           + [C.POINTER(_vp), C.POINTER(C.c_int), C.POINTER(_vp), _vp]),
     "hq_tile_to_cm_dev": (C.c_int, [C.c_int] * 2 + [C.POINTER(C.c_int)] * 2
                           + [C.POINTER(_vp), C.POINTER(_vp), C.POINTER(C.c_int), _vp]),
+    "hq_tile_qr_mixed_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp]),
     "hq_sync": (C.c_int, [_vp]),
     "hq_finalize": (None, []),
 }
@@ -158,6 +159,11 @@ def tile_qr_dev(MT, NT, nb, ib, A, TAU, T=None, stream=None, serial=False):
     """Factor the device tile-layout tensor A in place (torch float64 tensors)."""
     fn = lib().hq_tile_qr_dev_serial if serial else lib().hq_tile_qr_dev
     check(fn(MT, NT, nb, ib, _ptr(A), _ptr(TAU), _ptr(T), _stream(stream)))
+
+
+def tile_qr_mixed_dev(MT, NT, nb, ib, A, TAU, T=None, stream=None):
+    """FP64 panels, FP32 LARFB / SSRFB updates (mixed precision)."""
+    check(lib().hq_tile_qr_mixed_dev(MT, NT, nb, ib, _ptr(A), _ptr(TAU), _ptr(T), _stream(stream)))
 
 
 def fill_uniform_dev(MT, NT, nb, seed, A, stream=None):

@@ -1147,22 +1147,243 @@ __device__ __forceinline__ void task_update_f(Smem<NB> &S, const double *V, cons
   store_strip<NB>(acc, X, col, ok, 0);
 }
 
+// --------------------------------------------------------------------------
+// FP32 update path (mixed precision): LARFB / SSRFB of the same slab tasks
+// with FP32 operands and FP32 accumulation on v_mfma_f32_16x16x4f32; tiles
+// stay FP64 in memory (converted on load and on store), the panels that
+// produce V and T stay FP64.  Same fused pass structure as task_update_f.
+// v_mfma_f32_16x16x4f32 lane layout (gfx950, measured): A(i,k) at lane
+// i + 16k, B(k,j) at lane j + 16k, D(i,j) at lane j + 16*(i/4), register
+// i%4 -- lane group g holds rows 4g..4g+3 of its column without any row
+// permutation.  V and T^T are staged through registers (FP64 load, convert,
+// FP32 LDS store at the end of the pass); LARFB's unit lower trapezoid is
+// applied in that conversion.
+// --------------------------------------------------------------------------
+typedef float f4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ f4 mfma32(float a, float b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+constexpr int LDV32 = 260;  // FP32 ring slot column stride (16-byte aligned columns)
+template <int NB>
+__device__ __forceinline__ float *f32vbuf(Smem<NB> &S, int i) {
+  return reinterpret_cast<float *>(i < 2 ? S.V : S.V1) + (i & 1) * 16 * LDV32;
+}
+template <int NB>
+__device__ __forceinline__ float *f32tbuf(Smem<NB> &S, int i) {
+  return reinterpret_cast<float *>(S.TT) + i * 16 * LDT16;
+}
+
+template <int NB>
+__device__ __forceinline__ void ld_u32(float (&o)[8], const float *Vb, int G) {
+  const int lane = threadIdx.x & 63, g = lane >> 4, m = lane & 15;
+#pragma unroll
+  for (int sp = 0; sp < 4; sp++)
+#pragma unroll
+    for (int i = 0; i < 2; i++) o[2 * sp + i] = Vb[(4 * g + sp) * LDV32 + 16 * (2 * G + i) + m];
+}
+template <int NB>
+__device__ __forceinline__ void ld_a32(float (&o)[8], const float *Vb, int G) {
+  const int lane = threadIdx.x & 63, g = lane >> 4, m = lane & 15;
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int sp = 0; sp < 4; sp++) o[4 * i + sp] = Vb[m * LDV32 + 16 * (2 * G + i) + 4 * g + sp];
+}
+
+// V columns of a sub-block held in registers between the global loads (issued
+// one per phase, early in the pass) and the FP32 LDS store at its end.
+template <int NB>
+struct Stage32 {
+  static constexpr int NLD = 16 * NB / 2 / HQ_THREADS;  // 16-byte loads per thread
+  double2 v[NLD];
+  double t;
+};
+// load q of this thread: column c = tid/16 of the sub-block, rows
+// 2*(tid%16) + 32*q .. +1 (one 16-byte load; columns past kb load column kb-1)
+template <int NB>
+__device__ __forceinline__ void stage32_load(Stage32<NB> &R, int q, const double *V, const SubBlk &sb) {
+  const int c = min(threadIdx.x >> 4, sb.kb - 1);
+  const int r = 2 * (threadIdx.x & 15) + 32 * q;
+  R.v[q] = *reinterpret_cast<const double2 *>(V + (size_t)(sb.i0 + c) * NB + r);
+}
+template <int NB, bool UNIT>
+__device__ __forceinline__ void stage32_store(const Stage32<NB> &R, float *Vb, float *TTb, const SubBlk &sb) {
+  const int c = threadIdx.x >> 4;
+  const int d = sb.i0 + c;
+#pragma unroll
+  for (int q = 0; q < Stage32<NB>::NLD; q++) {
+    const int r = 2 * (threadIdx.x & 15) + 32 * q;
+    double a = R.v[q].x, b = R.v[q].y;
+    if (UNIT) {
+      a = r > d ? a : (r == d ? 1.0 : 0.0);
+      b = r + 1 > d ? b : (r + 1 == d ? 1.0 : 0.0);
+    }
+    const bool live = c < sb.kb;
+    float2 f = make_float2(live ? (float)a : 0.f, live ? (float)b : 0.f);
+    *reinterpret_cast<float2 *>(Vb + c * LDV32 + r) = f;
+  }
+  TTb[(threadIdx.x >> 4) * LDT16 + (threadIdx.x & 15)] = (float)(R.t * tt_mask(sb));
+}
+
+template <int NB, bool DO_U, bool DO_A, bool DO_LD>
+__device__ __forceinline__ void fpass32(f4 (&acc)[NB / 16], const f4 &W2, f4 (&Wn)[2], float (&o)[2][8],
+                                        float (&tto)[4], const float *Vu, const float *Vw, const float *TTw,
+                                        Stage32<NB> &R, const double *Vg, const SubBlk &sd) {
+  constexpr int NG = NB / 32;
+  constexpr int NP = (DO_U ? NG : 0) + (DO_A ? NG : 0);
+  static_assert(!DO_LD || NP >= Stage32<NB>::NLD, "one staging load per phase");
+  auto kind = [](int p) -> int {
+    if (DO_U && !DO_A) return 1;
+    if (!DO_U) return 0;
+    if (p == 0) return 1;
+    if (p == 2 * NG - 1) return 0;
+    return (p & 1) ? 1 : 0;
+  };
+  auto grp = [](int p) -> int {
+    if (!(DO_U && DO_A)) return p;
+    if (p == 0) return 0;
+    if (p == 2 * NG - 1) return NG - 1;
+    return (p & 1) ? (p + 1) / 2 : p / 2 - 1;
+  };
+  const int lane = threadIdx.x & 63, g = lane >> 4, m = lane & 15;
+#pragma unroll
+  for (int p = 0; p < NP; p++) {
+    if (DO_LD && p < Stage32<NB>::NLD) stage32_load<NB>(R, p, Vg, sd);
+    if (p + 1 < NP) {
+      if (kind(p + 1))
+        ld_u32<NB>(o[(p + 1) & 1], Vu, grp(p + 1));
+      else
+        ld_a32<NB>(o[(p + 1) & 1], Vw, grp(p + 1));
+    } else if (DO_A) {
+#pragma unroll
+      for (int sp = 0; sp < 4; sp++) tto[sp] = TTw[m * LDT16 + 4 * g + sp];
+    }
+    const int G = grp(p);
+    if (kind(p)) {
+#pragma unroll
+      for (int sp = 0; sp < 4; sp++)
+#pragma unroll
+        for (int i = 0; i < 2; i++) acc[2 * G + i] = mfma32(o[p & 1][2 * sp + i], W2[sp], acc[2 * G + i]);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int sp = 0; sp < 4; sp++) Wn[sp & 1] = mfma32(o[p & 1][4 * i + sp], acc[2 * G + i][sp], Wn[sp & 1]);
+    }
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+    __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+    __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+    __builtin_amdgcn_sched_group_barrier(0x008, 6, 0);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+template <int NB, bool UNIT>
+__device__ __forceinline__ void task_update_f32(Smem<NB> &S, const double *V, const double *T, double *X,
+                                                double *A1, int ib, int slab) {
+  static_assert(NB % HQ_SLAB == 0, "slabs cover the tile exactly");
+  const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, g = lane >> 4, wave = tid >> 6;
+  const int col = slab * HQ_SLAB + 16 * wave + m;
+  double *xcol = X + (size_t)col * NB;
+  double *a1col = UNIT ? nullptr : A1 + (size_t)col * NB;
+  const int nsub = sub_count<NB>(ib);
+  SubBlk w0 = sub_blk<NB>(0, ib), w1 = w0, w2 = next_sub<NB>(w0, ib), w3 = next_sub<NB>(w2, ib);
+  f4 acc[NB / 16];
+  f4 W2 = f4{0.f, 0.f, 0.f, 0.f}, Wn[2], a1v = W2, a1o = W2;
+  float o[2][8], tto[4];
+  Stage32<NB> R;
+  // prologue: sub-blocks 0 and 1 into ring slots 0, 1; the strip in FP32
+  {
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const SubBlk sj = j ? w2 : w0;
+#pragma unroll
+      for (int q = 0; q < Stage32<NB>::NLD; q++) stage32_load<NB>(R, q, V, sj);
+      R.t = tt_raw(T, ib, sj);
+      stage32_store<NB, UNIT>(R, f32vbuf(S, j), f32tbuf(S, j), sj);
+    }
+#pragma unroll
+    for (int rb = 0; rb < NB / 16; rb++) {
+      const d4 x = *reinterpret_cast<const d4 *>(xcol + 16 * rb + 4 * g);
+      acc[rb] = f4{(float)x[0], (float)x[1], (float)x[2], (float)x[3]};
+    }
+    __syncthreads();
+    ld_a32<NB>(o[0], f32vbuf(S, 0), 0);
+  }
+  auto pre = [&](int s) {
+    R.t = tt_raw(T, ib, w2);
+    if (!UNIT) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) a1v[v] = (float)a1col[min(w1.i0 + 4 * g + v, NB - 1)];
+    }
+    Wn[0] = f4{0.f, 0.f, 0.f, 0.f};
+    Wn[1] = Wn[0];
+  };
+  // after pass s: W2_{s+1}; sub-block s+2 into its ring slot; A1 rows of
+  // s+1 stored after the barrier
+  auto post = [&](int s) {
+    const f4 W = UNIT ? Wn[0] + Wn[1] : (Wn[0] + Wn[1]) + a1v;
+    f4 W2n = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int sp = 0; sp < 4; sp++) W2n = mfma32(tto[sp], W[sp], W2n);
+    a1o = a1v - W2n;
+    W2 = -W2n;
+    ld_u32<NB>(o[0], f32vbuf(S, (s + 1) & 3), 0);
+    stage32_store<NB, UNIT>(R, f32vbuf(S, (s + 2) & 3), f32tbuf(S, (s + 2) & 3), w2);
+    __syncthreads();
+    if (!UNIT) {
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        if (4 * g + v < w1.kb) a1col[w1.i0 + 4 * g + v] = (double)a1o[v];
+    }
+    w0 = w1;
+    w1 = w2;
+    w2 = w3;
+    w3 = next_sub<NB>(w3, ib);
+  };
+  pre(-1);
+  fpass32<NB, false, true, true>(acc, W2, Wn, o, tto, f32vbuf(S, 3), f32vbuf(S, 0), f32tbuf(S, 0), R, V, w2);
+  post(-1);
+  for (int s = 0; s + 1 < nsub; s++) {
+    pre(s);
+    fpass32<NB, true, true, true>(acc, W2, Wn, o, tto, f32vbuf(S, s & 3), f32vbuf(S, (s + 1) & 3),
+                                  f32tbuf(S, (s + 1) & 3), R, V, w2);
+    post(s);
+  }
+  fpass32<NB, true, false, false>(acc, W2, Wn, o, tto, f32vbuf(S, (nsub - 1) & 3), f32vbuf(S, 0),
+                                  f32tbuf(S, 0), R, V, w2);
+#pragma unroll
+  for (int rb = 0; rb < NB / 16; rb++) {
+    const f4 x = acc[rb];
+    *reinterpret_cast<d4 *>(xcol + 16 * rb + 4 * g) = d4{(double)x[0], (double)x[1], (double)x[2], (double)x[3]};
+  }
+}
+
 template <int NB, int WB>
 __device__ __forceinline__ void task_larfb(Smem<NB> &S, const double *V, const double *T, double *C,
-                                           int ib, int slab) {
-  if constexpr (NB >= 128)
-    task_update_f<NB, true>(S, V, T, C, nullptr, ib, slab);
-  else
+                                           int ib, int slab, int prec = 0) {
+  if constexpr (NB >= 128) {
+    if (prec)
+      task_update_f32<NB, true>(S, V, T, C, nullptr, ib, slab);
+    else
+      task_update_f<NB, true>(S, V, T, C, nullptr, ib, slab);
+  } else {
     task_update<NB, WB, true>(S, V, T, C, nullptr, ib, slab);
+  }
 }
 
 template <int NB, int WB>
 __device__ __forceinline__ void task_ssrfb(Smem<NB> &S, double *A1, double *A2, const double *V,
-                                           const double *T, int ib, int slab) {
-  if constexpr (NB >= 128)
-    task_update_f<NB, false>(S, V, T, A2, A1, ib, slab);
-  else
+                                           const double *T, int ib, int slab, int prec = 0) {
+  if constexpr (NB >= 128) {
+    if (prec)
+      task_update_f32<NB, false>(S, V, T, A2, A1, ib, slab);
+    else
+      task_update_f<NB, false>(S, V, T, A2, A1, ib, slab);
+  } else {
     task_update<NB, WB, false>(S, V, T, A2, A1, ib, slab);
+  }
 }
 
 __device__ __forceinline__ double *tile_ptr(const hq_geom &G, int i, int j) {
@@ -1190,11 +1411,11 @@ __device__ __forceinline__ void run_task(Smem<NB> &S, const hq_geom &G, const hq
       break;
     case HQ_LARFB:
       task_larfb<NB, WB>(S, tile_ptr(G, t.k, t.k), t_ptr(G, t.k, t.k), xtile_ptr(G, t.k, t.j), G.ib,
-                         t.slab);
+                         t.slab, G.prec);
       break;
     case HQ_SSRFB:
       task_ssrfb<NB, WB>(S, xtile_ptr(G, t.k, t.j), xtile_ptr(G, t.r, t.j), tile_ptr(G, t.r, t.k),
-                         t_ptr(G, t.r, t.k), G.ib, t.slab);
+                         t_ptr(G, t.r, t.k), G.ib, t.slab, G.prec);
       break;
     default:
       break;

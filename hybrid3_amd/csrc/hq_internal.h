@@ -42,6 +42,7 @@ struct hq_geom {
   double *X;    // matrix updated by LARFB / SSRFB: A itself while factorising,
                 // the Q / C array (MT x XNT tiles) while applying Q^T
   int XNT;
+  int prec;     // 0: FP64 everywhere; 1: LARFB / SSRFB in FP32 (panels FP64)
 };
 
 // One matrix of a batch for the layout kernels: m x n entries in an MT x NT

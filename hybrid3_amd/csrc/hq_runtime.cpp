@@ -393,14 +393,15 @@ int ensure_T(Context *c, int MT, int NT, int nb, int ib, double **T) {
 }
 
 int tile_qr_dev_impl(int MT, int NT, int nb, int ib, double *A, double *TAU, double *T,
-                     void *stream, bool serial, long long *trace = nullptr) {
+                     void *stream, bool serial, long long *trace = nullptr, int prec = 0) {
   if (!geometry_ok(MT, NT, nb, ib)) return g_err;
+  if (prec && nb < 128) return set_error(HQ_ERR_ARG, "FP32 updates need nb >= 128 (nb=%d)", nb);
   if (!A || !TAU) return set_error(HQ_ERR_ARG, "A and TAU must be device pointers");
   Context *c;
   if (get_ctx(&c) != HQ_OK) return g_err;
   hipStream_t st = pick_stream(c, stream);
   if (ensure_T(c, MT, NT, nb, ib, &T) != HQ_OK) return g_err;
-  hq_geom G{A, TAU, T, MT, NT, nb, ib, A, NT};
+  hq_geom G{A, TAU, T, MT, NT, nb, ib, A, NT, prec};
   if (serial) {
     std::vector<hq_task> tl;
     host_schedule(MT, NT, nb, ib, c->num_cu, tl);
@@ -620,6 +621,11 @@ void mat_init(mat_stat_t *status, long int mat_m, long int mat_n, int tile_m, in
 int hq_tile_qr_dev(int MT, int NT, int nb, int ib, double *A, double *TAU, double *T,
                    void *stream) {
   return tile_qr_dev_impl(MT, NT, nb, ib, A, TAU, T, stream, false);
+}
+
+int hq_tile_qr_mixed_dev(int MT, int NT, int nb, int ib, double *A, double *TAU, double *T,
+                         void *stream) {
+  return tile_qr_dev_impl(MT, NT, nb, ib, A, TAU, T, stream, false, nullptr, 1);
 }
 
 int hq_tile_qr_dev_serial(int MT, int NT, int nb, int ib, double *A, double *TAU, double *T,

@@ -43,6 +43,15 @@ const char *hq_version(void);
 int hq_tile_qr_dev(int MT, int NT, int nb, int ib, double *A, double *TAU,
                    double *T, void *stream);
 
+/* Mixed precision (no reference counterpart): the same flat-tree tile QR
+ * with GEQRT / TSQRT (reflectors, T factors, R) in FP64 and the trailing
+ * updates LARFB / SSRFB computed in FP32 on MFMA with FP32 accumulation
+ * (tiles stay FP64 in memory).  R carries FP32-level errors; it is meant as
+ * the factorisation inside an iteratively refined least-squares solve.
+ * nb in {128, 256}. */
+int hq_tile_qr_mixed_dev(int MT, int NT, int nb, int ib, double *A, double *TAU,
+                         double *T, void *stream);
+
 /* Same factorisation, one kernel launch per task in the scheduler's order
  * (stream order replaces the device dependency flags). */
 int hq_tile_qr_dev_serial(int MT, int NT, int nb, int ib, double *A,

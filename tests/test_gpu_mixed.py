@@ -1,0 +1,69 @@
+"""Mixed-precision tile QR (no reference counterpart): FP64 panels (GEQRT /
+TSQRT), FP32 trailing updates (LARFB / SSRFB on FP32 MFMA with FP32
+accumulation), through the C-ABI.
+
+Pinned by the FP64 oracle: R / V / TAU agree to FP32 level (a few 1e-6 of the
+largest entry) and differ from the FP64 result by more than FP64 rounding,
+which shows the FP32 path is the one that ran.  At whole-chip sizes, Q's
+orthogonality fixes the column norms of R to FP32 level."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def hq():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import hybrid3_amd
+    hybrid3_amd.lib()
+    return hybrid3_amd
+
+
+@pytest.mark.parametrize("MT,NT,nb,ib", [(3, 3, 128, 32), (4, 4, 256, 32), (3, 2, 256, 20),
+                                         (4, 3, 128, 24), (2, 3, 128, 16)])
+def test_mixed_matches_fp64_oracle_to_fp32_level(hq, orc, MT, NT, nb, ib):
+    import torch
+    print(f"mixed MT={MT} NT={NT} nb={nb} ib={ib}", flush=True)
+    A = orc.fill_uniform_cm(MT * nb, NT * nb, 31 + MT + NT)
+    At = orc.cm_to_tile(A, nb)
+    dA = torch.from_numpy(At.copy()).cuda()
+    dTAU = torch.zeros(MT * NT * nb, dtype=torch.float64, device="cuda")
+    hq.tile_qr_mixed_dev(MT, NT, nb, ib, dA, dTAU, None)
+    hq.sync()
+    RY, TAU = dA.cpu().numpy(), dTAU.cpu().numpy()
+    RYo, TAUo = orc.tile_qr(At, MT, NT, nb, ib)
+    err = np.abs(RY - RYo).max() / np.abs(RYo).max()
+    assert err < 2e-5, err
+    assert err > 1e-12, "no FP32 rounding visible: the FP32 update path did not run"
+    assert np.abs(TAU - TAUo).max() < 2e-4
+    # the first panel (tile column 0) is computed before any update: FP64 exact
+    R0 = orc.tile_to_cm(RY, MT, NT, nb)[:, :nb]
+    R0o = orc.tile_to_cm(RYo, MT, NT, nb)[:, :nb]
+    assert np.abs(R0 - R0o).max() / np.abs(R0o).max() < 1e-12
+
+
+def test_mixed_large_colnorm(hq):
+    import torch
+    n, nb, ib = 4096, 256, 32
+    MT = NT = n // nb
+    print("mixed colnorm 4096", flush=True)
+    A = torch.empty(MT * NT * nb * nb, dtype=torch.float64, device="cuda")
+    hq.fill_uniform_dev(MT, NT, nb, 99, A)
+    A0 = A.clone()
+    TAU = torch.zeros(MT * NT * nb, dtype=torch.float64, device="cuda")
+    hq.tile_qr_mixed_dev(MT, NT, nb, ib, A, TAU, None)
+    hq.sync()
+    a = A0.view(NT, MT, nb, nb)
+    f = A.view(NT, MT, nb, nb)
+    tj = torch.arange(NT, device="cuda").view(NT, 1, 1, 1)
+    ti = torch.arange(MT, device="cuda").view(1, MT, 1, 1)
+    col = torch.arange(nb, device="cuda").view(1, 1, nb, 1)
+    row = torch.arange(nb, device="cuda").view(1, 1, 1, nb)
+    upper = (ti * nb + row) <= (tj * nb + col)
+    an = (a * a).sum(dim=(1, 3)).sqrt()
+    rn = (f * f * upper).sum(dim=(1, 3)).sqrt()
+    rel = ((an - rn).abs() / an).max().item()
+    assert rel < 1e-4, rel
