@@ -40,6 +40,7 @@ def parse():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven-QR measurement")
     ap.add_argument("--batch-count", type=int, default=4000)
+    ap.add_argument("--no-lsq", action="store_true", help="skip the mixed-precision least-squares part")
     ap.add_argument("--traffic-file",
                     default=os.path.join(ROOT, "profiles", "sched_kernel_traffic.txt"))
     return ap.parse_args()
@@ -82,6 +83,41 @@ def batched_bench(count):
         "tuner": report,
         "max_col_norm_rel_diff": err,
     }
+
+
+def lsq_bench(m=32768, n=8192, kappas=(1e2, 1e6, 1e10), nb=256, ib=32):
+    """Least squares on A = U diag(sigma) V^T (U, V from seeded Gaussian QR,
+    sigma log-spaced to 1/kappa), b = A x_true + 1e-8 N(0,1): plain FP64
+    tile QR + refinement against mixed tile QR (FP32 updates) + refinement,
+    both refined with FP64 residuals until the step stagnates."""
+    import torch
+    from hybrid3_amd import lsq
+    out = []
+    for i, kappa in enumerate(kappas):
+        A = lsq.svd_matrix(m, n, kappa, seed=300 + i, nb=nb, ib=ib)
+        g = torch.Generator(device="cuda").manual_seed(400 + i)
+        xt = torch.randn(n, 1, generator=g, dtype=torch.float64, device="cuda")
+        b = A @ xt + 1e-8 * torch.randn(m, 1, generator=g, dtype=torch.float64, device="cuda")
+        res = {"kappa": kappa}
+        sols = {}
+        for name, mixed in (("fp64", False), ("mixed", True)):
+            lsq.lstsq(A[:2048, :512].contiguous().t().contiguous().t(), b[:2048], nb=nb, ib=ib, mixed=mixed,
+                      max_iter=2)  # warm the schedule caches of this path
+            x, info = lsq.lstsq(A, b, nb=nb, ib=ib, mixed=mixed, max_iter=30)
+            sols[name] = x
+            res[name] = {"factor_ms": round(info["factor_seconds"] * 1e3, 2),
+                         "solve_ms": round(info["total_seconds"] * 1e3, 2), "steps": info["steps"],
+                         "backward": lsq.backward_indicator(A, b, x),
+                         "err_vs_x_true": (torch.linalg.norm(x - xt) / torch.linalg.norm(xt)).item()}
+        res["mixed_vs_fp64_rel_diff"] = (torch.linalg.norm(sols["mixed"] - sols["fp64"])
+                                         / torch.linalg.norm(sols["fp64"])).item()
+        res["speedup_mixed_vs_fp64"] = round(res["fp64"]["solve_ms"] / res["mixed"]["solve_ms"], 3)
+        out.append(res)
+        del A, b, sols
+        torch.cuda.empty_cache()
+    return {"workload": f"least squares {m}x{n}, A = U diag(sigma) V^T, b = A x_true + 1e-8 N(0,1); "
+                        f"tile QR nb={nb} ib={ib} + FP64-residual refinement",
+            "qr_flops": 2.0 * m * n * n - 2.0 * n ** 3 / 3.0, "cases": out}
 
 
 def main():
@@ -171,6 +207,13 @@ def main():
         print(json.dumps({"error": "batched factorisation check failed", "batched": batched}))
         sys.exit(3)
 
+    lsq_res = None
+    if not args.no_lsq:
+        try:
+            lsq_res = lsq_bench()
+        except Exception as e:  # the headline line must still print
+            lsq_res = {"error": f"{type(e).__name__}: {e}"}
+
     cpu = None if args.no_cpu_baseline else cpu_baseline(nb, ib, args.seed)
     out = {
         "metric": "QR TFLOP/s FP64 vs mixed+refined at FP64 backward error; batched QRs/sec",
@@ -196,6 +239,7 @@ def main():
         "cpu_baseline": cpu,
         "check": check,
         "batched": batched,
+        "mixed_lsq": lsq_res,
         "step_ms": [round(x, 3) for x in fact_ms],
     }
     print(json.dumps(out))
