@@ -1455,16 +1455,24 @@ __global__ __launch_bounds__(HQ_THREADS) void sched_kernel(hq_geom G, const hq_g
         t = ntasks;
       if (t < ntasks) {
         const hq_task &tk = tasks[t];
-        for (int i = 0; i < tk.ndep && t < ntasks; i++) {
-          const int d = tk.dep[i];
-          unsigned spins = 0;
-          while (t < ntasks &&
-                 __hip_atomic_load(&flags[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-            __builtin_amdgcn_s_sleep(2);
-            if (++spins > (1u << 22)) {  // seconds: a dependency never completed
-              __hip_atomic_store(&counter[1], 1 + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              t = ntasks;
-            }
+        const int nd = tk.ndep;
+        unsigned spins = 0;
+        for (;;) {
+          // every dependency flag in flight at once (one round trip per check)
+          int f[HQ_MAXDEP];
+#pragma unroll
+          for (int i = 0; i < HQ_MAXDEP; i++)
+            f[i] = i < nd ? __hip_atomic_load(&flags[tk.dep[i]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                          : epoch;
+          bool ready = true;
+#pragma unroll
+          for (int i = 0; i < HQ_MAXDEP; i++) ready = ready && f[i] == epoch;
+          if (ready) break;
+          __builtin_amdgcn_s_sleep(2);
+          if (++spins > (1u << 22)) {  // seconds: a dependency never completed
+            __hip_atomic_store(&counter[1], 1 + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            t = ntasks;
+            break;
           }
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
