@@ -46,6 +46,11 @@ def parse():
     return ap.parse_args()
 
 
+def note(msg):
+    """One progress line on stderr per finished section (stdout holds only the JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_baseline(nb, ib, seed):
     """Oracle tile QR (TileSch-style DAG, OpenMP) on a 4096 x 4096 sample."""
     import oracle
@@ -113,6 +118,7 @@ def lsq_bench(m=32768, n=8192, kappas=(1e2, 1e6, 1e10), nb=256, ib=32):
                                          / torch.linalg.norm(sols["fp64"])).item()
         res["speedup_mixed_vs_fp64"] = round(res["fp64"]["solve_ms"] / res["mixed"]["solve_ms"], 3)
         out.append(res)
+        note(f"least squares kappa={kappa:g}: fp64 {res['fp64']['solve_ms']} ms, mixed {res['mixed']['solve_ms']} ms")
         del A, b, sols
         torch.cuda.empty_cache()
     return {"workload": f"least squares {m}x{n}, A = U diag(sigma) V^T, b = A x_true + 1e-8 N(0,1); "
@@ -166,6 +172,7 @@ def main():
     mean_ms = sum(fact_ms) / len(fact_ms)
     flops = 4.0 / 3.0 * n ** 3
     tflops = flops / (mean_ms * 1e-3) / 1e12
+    note(f"tile QR {n}x{n}: {mean_ms:.2f} ms, {tflops:.2f} TFLOP/s")
 
     # numerical check of the last factorisation (cheap, size-independent):
     # ||A e - Q R e|| via the reflectors is costly at n=16384, so check R's
@@ -203,6 +210,8 @@ def main():
     del pristine, A, TAU, T
     torch.cuda.empty_cache()
     batched = None if args.no_batched else batched_bench(args.batch_count)
+    if batched is not None:
+        note(f"batched: {batched['qrs_per_s']} QRs/s")
     if batched is not None and not batched["max_col_norm_rel_diff"] < 1e-12:
         print(json.dumps({"error": "batched factorisation check failed", "batched": batched}))
         sys.exit(3)
@@ -213,8 +222,11 @@ def main():
             lsq_res = lsq_bench()
         except Exception as e:  # the headline line must still print
             lsq_res = {"error": f"{type(e).__name__}: {e}"}
+        note("mixed least squares done")
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(nb, ib, args.seed)
+    if cpu is not None:
+        note(f"cpu baseline: {cpu['value']} TFLOP/s")
     out = {
         "metric": "QR TFLOP/s FP64 vs mixed+refined at FP64 backward error; batched QRs/sec",
         "value": round(tflops, 4),

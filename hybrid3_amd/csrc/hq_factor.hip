@@ -515,31 +515,40 @@ __device__ __forceinline__ void gram(Smem<NB> &S, int rb0) {
 }
 
 // T (upper, sb x sb) from G and tau, LAPACK dlarft('F','C') recursion
-//   T(0:c, c) = -tau_c T(0:c, 0:c) G(0:c, c),  T(c,c) = tau_c.
-// Lane p of wave 0 holds row p of T in registers; column c needs only the
-// lane's own row and the broadcast column c of G.  Writes T^T to S.TT, T to
-// global (ld = ib, block at column ii) and, if taug, tau to TAU.
+//   T(0:c, c) = -tau_c T(0:c, 0:c) G(0:c, c),  T(c,c) = tau_c,
+// in right-looking order: lane p of wave 0 holds row p of T; as soon as
+// T(p, q) is final its terms T(p,q) G(q,c) are added to every later column
+// c, so finishing a column waits on one product, not on a dot product of
+// growing length.  Row q+1 of G is read (broadcast LDS reads) while row q is
+// applied.  Each step is its own branch and scheduling region, which keeps
+// the register use of the unrolled recursion bounded.  Writes T^T to S.TT,
+// T to global (ld = ib, block at column ii) and, if taug, tau to TAU.
 template <int NB>
 __device__ __forceinline__ void build_t(Smem<NB> &S, int sb, double *Tg, int ib, double *taug) {
   constexpr int LDT = Smem<NB>::LDT;
   const int p = threadIdx.x;  // wave 0 only
   double Tr[HQ_IBMAX];
+  double gr[2][HQ_IBMAX];
 #pragma unroll
   for (int c = 0; c < HQ_IBMAX; c++) {
-    double val = 0.0;
-    if (c < sb) {
-      double t0 = 0.0, t1 = 0.0;
+    Tr[c] = 0.0;
+    gr[1][c] = 0.0;
+    gr[0][c] = c > 0 ? S.G[c] : 0.0;
+  }
 #pragma unroll
-      for (int q = 0; q < c; q++) {
-        if (q & 1)
-          t1 += Tr[q] * S.G[q * LDT + c];
-        else
-          t0 += Tr[q] * S.G[q * LDT + c];
+  for (int q = 0; q < HQ_IBMAX; q++) {
+    if (q < sb) {
+      if (q + 1 < HQ_IBMAX) {
+#pragma unroll
+        for (int c = q + 2; c < HQ_IBMAX; c++) gr[(q + 1) & 1][c] = S.G[(q + 1) * LDT + c];
       }
-      const double tc = S.tau[c];
-      val = (p < c) ? -tc * (t0 + t1) : (p == c ? tc : 0.0);
+      const double tq = S.tau[q];
+      Tr[q] = (p < q) ? -tq * Tr[q] : (p == q ? tq : 0.0);
+#pragma unroll
+      for (int c = q + 1; c < HQ_IBMAX; c++) Tr[c] += Tr[q] * gr[q & 1][c];
+    } else {
+      Tr[q] = 0.0;
     }
-    Tr[c] = val;
     __builtin_amdgcn_sched_barrier(0);
   }
   if (p < HQ_IBMAX) {
@@ -588,43 +597,154 @@ __device__ __forceinline__ void stage_columns(double *Vs, const double *tile, in
   }
 }
 
+// GEQRT and TSQRT are sequences over the ib blocks of the tile of
+//   panel b:     dgeqr2 / tsqr2 of the block's columns + dlarft into T, TAU
+//   trailing b:  the block reflector applied to the columns right of it
+// The whole-tile tasks run both in one workgroup; the DAG scheduler runs the
+// panel as one task and the trailing update as one task per 64-column slab,
+// so the flat-tree chain down a tile column (every TSQRT of a step updates
+// R(k,k)) advances one panel at a time instead of one tile at a time.  Each
+// column's arithmetic is the same either way.
+
+// GEQRT panel of columns ii .. ii+sb-1: R / V written back to A, T and TAU
+// of the block written; leaves V (unit lower trapezoidal) in S.V and T^T in
+// S.TT.
+template <int NB, int WB>
+__device__ __forceinline__ void geqrt_panel(Smem<NB> &S, double *A, double *T, double *TAU, int ib, int ii) {
+  constexpr int LDV = Smem<NB>::LDV;
+  const int tid = threadIdx.x, wave = tid >> 6;
+  const int sb = min(ib, NB - ii);
+  stage_columns<NB, WB>(S.V, A, ii, sb);
+  if (tid == 0) S.nref = 0;
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  panel_factor<NB, false>(S, ii, sb);
+  // write R / V back, then make the LDS copy unit lower trapezoidal
+  for (int e = tid; e < sb * NB; e += HQ_THREADS) {
+    int c = e / NB, r = e - c * NB;
+    if (r >= ii) {
+      A[(size_t)(ii + c) * NB + r] = S.V[c * LDV + r];
+      if (r <= ii + c) S.V[c * LDV + r] = (r == ii + c) ? 1.0 : 0.0;
+    }
+  }
+  __syncthreads();
+  gram<NB, WB>(S, ii / 16);
+  __syncthreads();
+  if (wave == 0) build_t<NB>(S, sb, T + (size_t)ii * ib, ib, TAU + ii);
+  __syncthreads();
+}
+
+// TSQRT panel of columns ii .. ii+sb-1 of [R(A1); A2]: V2 written to A2, the
+// R block to A1, T and TAU of the block written; leaves V2 in S.V and T^T in
+// S.TT.
+template <int NB, int WB>
+__device__ __forceinline__ void tsqrt_panel(Smem<NB> &S, double *A1, double *A2, double *T, double *TAU, int ib,
+                                            int ii) {
+  constexpr int LDV = Smem<NB>::LDV, LDT = Smem<NB>::LDT;
+  constexpr int IBP = 16 * WB;
+  const int tid = threadIdx.x, wave = tid >> 6;
+  const int sb = min(ib, NB - ii);
+  stage_columns<NB, WB>(S.V, A2, ii, sb);
+  if (tid == 0) S.nref = 0;
+  for (int e = tid; e < IBP * IBP; e += HQ_THREADS) {
+    int c = e / IBP, p = e - c * IBP;
+    S.A1[c * LDT + p] = (c < sb && p <= c) ? A1[(size_t)(ii + c) * NB + ii + p] : 0.0;
+  }
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  HQ_PHASE(0);
+  panel_factor<NB, true>(S, ii, sb);
+  HQ_PHASE(1);
+  for (int e = tid; e < sb * NB; e += HQ_THREADS) {
+    int c = e / NB, r = e - c * NB;
+    A2[(size_t)(ii + c) * NB + r] = S.V[c * LDV + r];
+  }
+  for (int e = tid; e < sb * sb; e += HQ_THREADS) {
+    int c = e / sb, p = e - c * sb;
+    if (p <= c) A1[(size_t)(ii + c) * NB + ii + p] = S.A1[c * LDT + p];
+  }
+  __syncthreads();
+  HQ_PHASE(2);
+  gram<NB, WB>(S, 0);
+  __syncthreads();
+  HQ_PHASE(3);
+  if (wave == 0) build_t<NB>(S, sb, T + (size_t)ii * ib, ib, TAU + ii);
+  __syncthreads();
+  HQ_PHASE(4);
+}
+
+// For a split trailing task: V of the block at column ii of tile Vt (GEQRT:
+// made unit lower trapezoidal, as the panel leaves it) and T^T from the T
+// array, into S.V / S.TT.
+template <int NB, int WB, bool TS>
+__device__ __forceinline__ void stage_panel(Smem<NB> &S, const double *Vt, const double *T, int ib, int ii) {
+  constexpr int LDV = Smem<NB>::LDV, LDT = Smem<NB>::LDT;
+  constexpr int IBP = 16 * WB;
+  const int tid = threadIdx.x;
+  const int sb = min(ib, NB - ii);
+  stage_columns<NB, WB>(S.V, Vt, ii, sb);
+  for (int e = tid; e < IBP * IBP; e += HQ_THREADS) {
+    const int c = e / IBP, p = e - c * IBP;
+    S.TT[c * LDT + p] = (c < sb && p <= c) ? T[(size_t)(ii + c) * ib + p] : 0.0;
+  }
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (!TS) {
+    for (int e = tid; e < sb * NB; e += HQ_THREADS) {
+      const int c = e / NB, r = e - c * NB;
+      if (r <= ii + c) S.V[c * LDV + r] = (r == ii + c) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+  }
+}
+
+// Trailing update of the block at column ii (V in S.V, T^T in S.TT) on the
+// 16-column groups c0 + 16*grp, grp = wave, wave + 4, ..., below c1; columns
+// of the block and left of it are not touched.  GEQRT (!TS): rows >= ii of
+// X = A.  TSQRT: [A1 rows ii..; X = A2].
+template <int NB, int WB, bool TS>
+__device__ __forceinline__ void panel_trailing(Smem<NB> &S, double *X, double *A1, int ib, int ii, int c0,
+                                               int c1) {
+  const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, wave = tid >> 6;
+  const int sb = min(ib, NB - ii), j0 = ii + sb;
+  const int rb0 = TS ? 0 : ii / 16;
+  for (int grp = wave; c0 + 16 * grp < c1; grp += 4) {
+    const int col = c0 + 16 * grp + m;
+    const bool ok = col >= j0 && col < c1;
+    d4 acc[NB / 16];
+    load_strip<NB>(acc, X, col, ok, rb0);
+    d4 A1v[WB];
+    if (TS) {
+      load_a1<WB>(A1v, A1 + (size_t)col * NB + ii, sb, ok);
+      apply_block<NB, WB, true>(acc, 0, S.V, S.TT, A1v, A1 + (size_t)col * NB + ii, sb, ok);
+      store_strip<NB>(acc, X, col, ok, 0);
+    } else {
+      apply_block<NB, WB, false>(acc, rb0, S.V, S.TT, A1v, nullptr, sb, ok);
+      // rows >= ii only: rows above it in the first row block are R rows of
+      // earlier blocks (unchanged here, and updated by other tasks meanwhile)
+      const int g = lane >> 4;
+      if (ok) {
+#pragma unroll
+        for (int rb = 0; rb < NB / 16; rb++) {
+          if (rb > rb0 || (rb == rb0 && (ii & 15) == 0)) {
+            *reinterpret_cast<d4 *>(X + (size_t)col * NB + 16 * rb + 4 * g) = acc[rb];
+          } else if (rb == rb0) {
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+              if (16 * rb + 4 * g + v >= ii) X[(size_t)col * NB + 16 * rb + 4 * g + v] = acc[rb][v];
+          }
+        }
+      }
+    }
+  }
+}
+
 // GEQRT = CORE_dgeqrt(nb, nb, ib, A, nb, T, ib, TAU)
 template <int NB, int WB>
 __device__ __forceinline__ void task_geqrt(Smem<NB> &S, double *A, double *T, double *TAU, int ib) {
-  constexpr int LDV = Smem<NB>::LDV;
-  constexpr int IBP = 16 * WB;
-  const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, wave = tid >> 6;
   for (int ii = 0; ii < NB; ii += ib) {
-    const int sb = min(ib, NB - ii);
-    stage_columns<NB, WB>(S.V, A, ii, sb);
-    if (tid == 0) S.nref = 0;
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    panel_factor<NB, false>(S, ii, sb);
-    // write R / V back, then make the LDS copy unit lower trapezoidal
-    for (int e = tid; e < sb * NB; e += HQ_THREADS) {
-      int c = e / NB, r = e - c * NB;
-      if (r >= ii) {
-        A[(size_t)(ii + c) * NB + r] = S.V[c * LDV + r];
-        if (r <= ii + c) S.V[c * LDV + r] = (r == ii + c) ? 1.0 : 0.0;
-      }
-    }
-    __syncthreads();
-    gram<NB, WB>(S, ii / 16);
-    __syncthreads();
-    if (wave == 0) build_t<NB>(S, sb, T + (size_t)ii * ib, ib, TAU + ii);
-    __syncthreads();
-    const int j0 = ii + sb;
-    const int ngrp = (NB - j0 + 15) / 16;
-    for (int grp = wave; grp < ngrp; grp += 4) {
-      int col = j0 + 16 * grp + m;
-      bool ok = col < NB;
-      d4 acc[NB / 16];
-      load_strip<NB>(acc, A, col, ok, ii / 16);
-      d4 A1v[WB];
-      apply_block<NB, WB, false>(acc, ii / 16, S.V, S.TT, A1v, nullptr, sb, ok);
-      store_strip<NB>(acc, A, col, ok, ii / 16);
-    }
+    geqrt_panel<NB, WB>(S, A, T, TAU, ib, ii);
+    panel_trailing<NB, WB, false>(S, A, nullptr, ib, ii, ii + min(ib, NB - ii), NB);
     __syncthreads();
   }
 }
@@ -632,53 +752,29 @@ __device__ __forceinline__ void task_geqrt(Smem<NB> &S, double *A, double *T, do
 // TSQRT = CORE_dtsqrt(nb, nb, ib, A1, nb, A2, nb, T, ib, TAU)
 template <int NB, int WB>
 __device__ __forceinline__ void task_tsqrt(Smem<NB> &S, double *A1, double *A2, double *T, double *TAU, int ib) {
-  constexpr int LDV = Smem<NB>::LDV, LDT = Smem<NB>::LDT;
-  constexpr int IBP = 16 * WB;
-  const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, wave = tid >> 6;
   for (int ii = 0; ii < NB; ii += ib) {
-    const int sb = min(ib, NB - ii);
-    stage_columns<NB, WB>(S.V, A2, ii, sb);
-    if (tid == 0) S.nref = 0;
-    for (int e = tid; e < IBP * IBP; e += HQ_THREADS) {
-      int c = e / IBP, p = e - c * IBP;
-      S.A1[c * LDT + p] = (c < sb && p <= c) ? A1[(size_t)(ii + c) * NB + ii + p] : 0.0;
-    }
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    HQ_PHASE(0);
-    panel_factor<NB, true>(S, ii, sb);
-    HQ_PHASE(1);
-    for (int e = tid; e < sb * NB; e += HQ_THREADS) {
-      int c = e / NB, r = e - c * NB;
-      A2[(size_t)(ii + c) * NB + r] = S.V[c * LDV + r];
-    }
-    for (int e = tid; e < sb * sb; e += HQ_THREADS) {
-      int c = e / sb, p = e - c * sb;
-      if (p <= c) A1[(size_t)(ii + c) * NB + ii + p] = S.A1[c * LDT + p];
-    }
-    __syncthreads();
-    HQ_PHASE(2);
-    gram<NB, WB>(S, 0);
-    __syncthreads();
-    HQ_PHASE(3);
-    if (wave == 0) build_t<NB>(S, sb, T + (size_t)ii * ib, ib, TAU + ii);
-    __syncthreads();
-    HQ_PHASE(4);
-    const int j0 = ii + sb;
-    const int ngrp = (NB - j0 + 15) / 16;
-    for (int grp = wave; grp < ngrp; grp += 4) {
-      int col = j0 + 16 * grp + m;
-      bool ok = col < NB;
-      d4 acc[NB / 16];
-      load_strip<NB>(acc, A2, col, ok, 0);
-      d4 A1v[WB];
-      load_a1<WB>(A1v, A1 + (size_t)col * NB + ii, sb, ok);
-      apply_block<NB, WB, true>(acc, 0, S.V, S.TT, A1v, A1 + (size_t)col * NB + ii, sb, ok);
-      store_strip<NB>(acc, A2, col, ok, 0);
-    }
+    tsqrt_panel<NB, WB>(S, A1, A2, T, TAU, ib, ii);
+    panel_trailing<NB, WB, true>(S, A2, A1, ib, ii, ii + min(ib, NB - ii), NB);
     __syncthreads();
     HQ_PHASE(5);
   }
+}
+
+// Split tasks of the DAG scheduler: panel b, and the trailing update of
+// block b on 64-column slab `slab` (V and T read back from memory).
+template <int NB, int WB>
+__device__ __forceinline__ void task_geqrt_update(Smem<NB> &S, double *A, const double *T, int ib, int b,
+                                                  int slab) {
+  const int ii = b * ib;
+  stage_panel<NB, WB, false>(S, A, T, ib, ii);
+  panel_trailing<NB, WB, false>(S, A, nullptr, ib, ii, slab * HQ_SLAB, min(NB, (slab + 1) * HQ_SLAB));
+}
+template <int NB, int WB>
+__device__ __forceinline__ void task_tsqrt_update(Smem<NB> &S, double *A1, double *A2, const double *T, int ib,
+                                                  int b, int slab) {
+  const int ii = b * ib;
+  stage_panel<NB, WB, true>(S, A2, T, ib, ii);
+  panel_trailing<NB, WB, true>(S, A2, A1, ib, ii, slab * HQ_SLAB, min(NB, (slab + 1) * HQ_SLAB));
 }
 
 // Staging of one ib block of a reflector tile and its T^T for the LARFB /
@@ -1148,6 +1244,222 @@ __device__ __forceinline__ void task_update_f(Smem<NB> &S, const double *V, cons
 }
 
 // --------------------------------------------------------------------------
+// Update pipeline with the products W_s = V_s T_s precomputed per reflector
+// tile (T_s the 16x16 diagonal block of T for sub-block s), for ib a
+// multiple of 16 (every sub-block 16 wide, s = 0 .. NB/16-1).  Since
+// T_s^T V_s^T X = W_s^T X, the accumulate phases of pass s produce
+// T_{s+1}^T (A1_{s+1} + V_{s+1}^T X) directly: the four T^T A1 MFMAs use no
+// result of the pass and run in its second phase, and the end of a pass is
+// one add and a negation instead of a chain of four dependent MFMAs.
+// Ring: V_s in slot s&1, W_s in slot 2+(s&1), T_s^T (SSRFB) in TT slot s&1.
+// Pass t loads V_{t+1} and W_{t+2} (direct-to-LDS, one op per phase).
+// V is the unit lower trapezoidal copy for LARFB (no fix-up in LDS).
+// --------------------------------------------------------------------------
+template <int NB>
+struct PassW {
+  const double *Vu, *Vw, *TTw;
+  double *D1, *D2;        // ring slots receiving this pass's loads
+  const double *G1, *G2;  // their source tiles
+  SubBlk s1, s2;
+};
+__device__ __forceinline__ SubBlk sub16(int s) {
+  SubBlk r;
+  r.i0 = 16 * s;
+  r.kb = 16;
+  r.p0 = 0;
+  return r;
+}
+// T_s^T of sub-block s (ib % 16 == 0) into TT slot layout c*17 + p: the raw
+// entry and the upper-triangle factor, as tt_raw / tt_mask.
+__device__ __forceinline__ double tt16_raw(const double *T, int ib, int s) {
+  const int c = threadIdx.x >> 4, p = threadIdx.x & 15;
+  return T[(size_t)(16 * s + c) * ib + ((16 * s) & (ib - 1)) + min(p, c)];
+}
+__device__ __forceinline__ double tt16_mask() {
+  const int c = threadIdx.x >> 4, p = threadIdx.x & 15;
+  return p <= c ? 1.0 : 0.0;
+}
+
+template <int NB, bool UNIT, bool DO_U, bool DO_A, int NDMA>
+__device__ __forceinline__ void fpass_w(d4 (&acc)[NB / 16], const d4 &W2, d4 (&Wn)[2], double (&o)[2][8],
+                                        const d4 &a1c, const PassW<NB> &P) {
+  constexpr int NG = NB / 32;
+  constexpr int NP = (DO_U ? NG : 0) + (DO_A ? NG : 0);
+  constexpr int NOPS = FDma<NB>::NOPS;
+  constexpr bool TA = !UNIT && DO_U && DO_A;
+  static_assert(NP >= NDMA * NOPS, "one direct-to-LDS op per phase");
+  auto kind = [](int p) -> int {  // 1 = update, 0 = accumulate
+    if (DO_U && !DO_A) return 1;
+    if (!DO_U) return 0;
+    if (p == 0) return 1;
+    if (p == 2 * NG - 1) return 0;
+    return (p & 1) ? 1 : 0;
+  };
+  auto grp = [](int p) -> int {
+    if (!(DO_U && DO_A)) return p;
+    if (p == 0) return 0;
+    if (p == 2 * NG - 1) return NG - 1;
+    return (p & 1) ? (p + 1) / 2 : p / 2 - 1;
+  };
+  const int lane = threadIdx.x & 63, g = lane >> 4, pm = arow(lane & 15);
+  double tto[4];
+#pragma unroll
+  for (int p = 0; p < NP; p++) {
+    if (NDMA >= 1 && p < NOPS)
+      dma_op<NB>(p, P.D1, P.G1, P.s1);
+    else if (NDMA >= 2 && p < 2 * NOPS)
+      dma_op<NB>(p - NOPS, P.D2, P.G2, P.s2);
+    if (TA && p == 0) {
+#pragma unroll
+      for (int sp = 0; sp < 4; sp++) tto[sp] = P.TTw[pm * LDT16 + 4 * g + sp];
+    }
+    if (p + 1 < NP) {
+      if (kind(p + 1))
+        ld_u<NB, UNIT>(o[(p + 1) & 1], P.Vu, grp(p + 1), 0, 16);
+      else
+        ld_a<NB, UNIT>(o[(p + 1) & 1], P.Vw, grp(p + 1), 0, 16);
+    }
+    if (TA && p == 1) {
+      // T^T A1 of the sub-block accumulated in this pass (independent of it)
+      Wn[1] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int sp = 0; sp < 4; sp++) Wn[1] = mfma(tto[sp], a1c[sp], Wn[1]);
+      __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+    }
+    const int G = grp(p);
+    if (kind(p)) {
+#pragma unroll
+      for (int sp = 0; sp < 4; sp++)
+#pragma unroll
+        for (int i = 0; i < 2; i++) acc[2 * G + i] = mfma(o[p & 1][2 * sp + i], W2[sp], acc[2 * G + i]);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int sp = 0; sp < 4; sp++) Wn[sp & 1] = mfma(o[p & 1][4 * i + sp], acc[2 * G + i][sp], Wn[sp & 1]);
+    }
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+    __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+    __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+    __builtin_amdgcn_sched_group_barrier(0x008, 6, 0);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// LARFB (UNIT: X <- Q^T X, V unit lower trapezoidal) or SSRFB ([A1; X] <-
+// Q^T [A1; X], V = [I; V2]) of one 64-column slab with W = V T16 given.
+template <int NB, bool UNIT>
+__device__ __forceinline__ void task_update_w(Smem<NB> &S, const double *V, const double *Wt, const double *T,
+                                              double *X, double *A1, int ib, int slab) {
+  static_assert(NB % HQ_SLAB == 0 && NB % 32 == 0, "slabs and row-block groups cover the tile");
+  constexpr int NSUB = NB / 16;
+  const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, g = lane >> 4, wave = tid >> 6;
+  const int pm = arow(m);
+  const int col = slab * HQ_SLAB + 16 * wave + m;
+  double *a1col = UNIT ? nullptr : A1 + (size_t)col * NB;
+  d4 acc[NB / 16];
+  const d4 zero = d4{0.0, 0.0, 0.0, 0.0};
+  d4 W2 = zero, Wn[2], a1c = zero, a1n = zero, a1o = zero;
+  double o[2][8];
+  double treg = 0.0;
+  // prologue: W_0, V_0, T_0^T and T_0^T A1_0, the strip
+  {
+#pragma unroll
+    for (int q = 0; q < FDma<NB>::NOPS; q++) dma_op<NB>(q, fvbuf(S, 2), Wt, sub16(0));
+#pragma unroll
+    for (int q = 0; q < FDma<NB>::NOPS; q++) dma_op<NB>(q, fvbuf(S, 0), V, sub16(0));
+    if (!UNIT) {
+      ftbuf(S, 0)[(tid >> 4) * LDT16 + (tid & 15)] = tt16_raw(T, ib, 0) * tt16_mask();
+      a1c = *reinterpret_cast<const d4 *>(a1col + 4 * g);
+      a1n = *reinterpret_cast<const d4 *>(a1col + 16 + 4 * g);
+      treg = tt16_raw(T, ib, 1);
+    }
+    load_strip<NB>(acc, X, col, true, 0);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    Wn[0] = zero;
+    Wn[1] = zero;
+    if (!UNIT) {
+      const double *tt = ftbuf(S, 0);
+#pragma unroll
+      for (int sp = 0; sp < 4; sp++) Wn[1] = mfma(tt[pm * LDT16 + 4 * g + sp], a1c[sp], Wn[1]);
+    }
+    ld_a<NB, UNIT>(o[0], fvbuf(S, 2), 0, 0, 16);
+  }
+  // after pass t-1: W2_t = -(W_t^T X + T_t^T A1_t); the new A1 rows of
+  // sub-block t (stored after the barrier); T_{t+1}^T into its TT slot
+  auto post = [&](int t) {
+    W2 = -(Wn[0] + Wn[1]);
+    if (!UNIT) a1o = a1c + W2;
+    ld_u<NB, UNIT>(o[0], fvbuf(S, t & 1), 0, 0, 16);
+    if (!UNIT) ftbuf(S, (t + 1) & 1)[(tid >> 4) * LDT16 + (tid & 15)] = treg * tt16_mask();
+    // every load of the pass has landed (direct-to-LDS loads are not taken
+    // to complete in order, so no counted wait)
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (!UNIT) *reinterpret_cast<d4 *>(a1col + 16 * t + 4 * g) = a1o;
+  };
+  // before pass t: A1 rows of sub-block t+1 (loaded one pass ahead) and the
+  // loads of rows t+2 and T_{t+2}^T (clamped to the last sub-block)
+  auto pre = [&](int t) {
+    if (!UNIT) {
+      const int tn = min(t + 2, NSUB - 1);
+      a1c = a1n;
+      a1n = *reinterpret_cast<const d4 *>(a1col + 16 * tn + 4 * g);
+      treg = tt16_raw(T, ib, tn);
+    }
+    Wn[0] = zero;
+    Wn[1] = zero;
+  };
+  {  // pass -1: W_0^T X; loads W_1
+    PassW<NB> P;
+    P.Vu = fvbuf(S, 0);
+    P.Vw = fvbuf(S, 2);
+    P.TTw = ftbuf(S, 0);
+    P.D1 = fvbuf(S, 3);
+    P.G1 = Wt;
+    P.s1 = sub16(1);
+    P.D2 = P.D1;
+    P.G2 = P.G1;
+    P.s2 = P.s1;
+    fpass_w<NB, UNIT, false, true, 1>(acc, W2, Wn, o, a1c, P);
+    post(0);
+  }
+  for (int t = 0; t + 1 < NSUB; t++) {
+    pre(t);
+    PassW<NB> P;
+    P.Vu = fvbuf(S, t & 1);
+    P.Vw = fvbuf(S, 2 + ((t + 1) & 1));
+    P.TTw = ftbuf(S, (t + 1) & 1);
+    P.D1 = fvbuf(S, (t + 1) & 1);
+    P.G1 = V;
+    P.s1 = sub16(t + 1);
+    P.D2 = fvbuf(S, 2 + (t & 1));
+    P.G2 = Wt;
+    P.s2 = sub16(min(t + 2, NSUB - 1));
+    fpass_w<NB, UNIT, true, true, 2>(acc, W2, Wn, o, a1c, P);
+    post(t + 1);
+  }
+  {  // last pass: update only
+    PassW<NB> P;
+    P.Vu = fvbuf(S, (NSUB - 1) & 1);
+    P.Vw = P.Vu;
+    P.TTw = ftbuf(S, 0);
+    P.D1 = nullptr;
+    P.G1 = nullptr;
+    P.s1 = sub16(0);
+    P.D2 = nullptr;
+    P.G2 = nullptr;
+    P.s2 = P.s1;
+    fpass_w<NB, UNIT, true, false, 0>(acc, W2, Wn, o, a1c, P);
+  }
+  // no direct-to-LDS load may still be in flight when the next task reuses LDS
+  __builtin_amdgcn_s_waitcnt(0);
+  store_strip<NB>(acc, X, col, true, 0);
+}
+
+// --------------------------------------------------------------------------
 // FP32 update path (mixed precision): LARFB / SSRFB of the same slab tasks
 // with FP32 operands and FP32 accumulation on v_mfma_f32_16x16x4f32; tiles
 // stay FP64 in memory (converted on load and on store), the panels that
@@ -1403,11 +1715,18 @@ template <int NB, int WB>
 __device__ __forceinline__ void run_task(Smem<NB> &S, const hq_geom &G, const hq_task &t) {
   switch (t.kind) {
     case HQ_GEQRT:
-      task_geqrt<NB, WB>(S, tile_ptr(G, t.k, t.k), t_ptr(G, t.k, t.k), tau_ptr(G, t.k, t.k), G.ib);
+      geqrt_panel<NB, WB>(S, tile_ptr(G, t.k, t.k), t_ptr(G, t.k, t.k), tau_ptr(G, t.k, t.k), G.ib, t.j * G.ib);
       break;
     case HQ_TSQRT:
-      task_tsqrt<NB, WB>(S, tile_ptr(G, t.k, t.k), tile_ptr(G, t.r, t.k), t_ptr(G, t.r, t.k),
-                         tau_ptr(G, t.r, t.k), G.ib);
+      tsqrt_panel<NB, WB>(S, tile_ptr(G, t.k, t.k), tile_ptr(G, t.r, t.k), t_ptr(G, t.r, t.k),
+                          tau_ptr(G, t.r, t.k), G.ib, t.j * G.ib);
+      break;
+    case HQ_GEQRT_U:
+      task_geqrt_update<NB, WB>(S, tile_ptr(G, t.k, t.k), t_ptr(G, t.k, t.k), G.ib, t.j, t.slab);
+      break;
+    case HQ_TSQRT_U:
+      task_tsqrt_update<NB, WB>(S, tile_ptr(G, t.k, t.k), tile_ptr(G, t.r, t.k), t_ptr(G, t.r, t.k), G.ib, t.j,
+                                t.slab);
       break;
     case HQ_LARFB:
       task_larfb<NB, WB>(S, tile_ptr(G, t.k, t.k), t_ptr(G, t.k, t.k), xtile_ptr(G, t.k, t.j), G.ib,
@@ -1444,7 +1763,11 @@ __global__ __launch_bounds__(HQ_THREADS) void sched_kernel(hq_geom G, const hq_g
     // structurizer into an inner loop that the other lanes never leave.)
     if (threadIdx.x == 0) {
       if (done >= 0) {
+        // every store of the task completed before the barrier that ended
+        // it; the L2 write-back of the release must complete too before the
+        // flag can be seen (the fence alone leaves no wait here)
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __builtin_amdgcn_s_waitcnt(0);
         __hip_atomic_store(&flags[done], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
       if (trace && done >= 0) trace[4 * done + 2] = wall_clock64();
@@ -1494,6 +1817,9 @@ __global__ __launch_bounds__(HQ_THREADS) void sched_kernel(hq_geom G, const hq_g
       run_task<NB, WB>(S, geoms[tk.mat], tk);
     else
       run_task<NB, WB>(S, G, tk);
+    // each wave's stores have completed (a barrier waits for LDS only), so
+    // none of them can land after the task's flag is published
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     done = t;
   }

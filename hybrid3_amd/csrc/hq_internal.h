@@ -7,11 +7,15 @@
 
 // Task kinds of the flat-tree tile QR DAG (Hybrid_tile_QR.c:960,1123,1254,1332)
 // plus the Q-application kinds used by dorgqr / dormqr.
+// GEQRT / TSQRT run as one panel task per ib block (field j = block) and
+// one trailing task per block and column slab (j = block, slab).
 enum hq_task_kind : int32_t {
-  HQ_GEQRT = 0,   // QR of diagonal tile (k,k)
-  HQ_TSQRT = 1,   // QR of [R(k,k); A(r,k)]
-  HQ_LARFB = 2,   // A(k,j) <- Q(k,k)^T A(k,j)            (one column slab)
-  HQ_SSRFB = 3,   // [A(k,j); A(r,j)] <- Q(r,k)^T [...]   (one column slab)
+  HQ_GEQRT = 0,    // QR of ib block j of diagonal tile (k,k)
+  HQ_TSQRT = 1,    // QR of ib block j of [R(k,k); A(r,k)]
+  HQ_LARFB = 2,    // A(k,j) <- Q(k,k)^T A(k,j)            (one column slab)
+  HQ_SSRFB = 3,    // [A(k,j); A(r,j)] <- Q(r,k)^T [...]   (one column slab)
+  HQ_GEQRT_U = 4,  // block j of GEQRT(k) applied to slab `slab` of tile (k,k)
+  HQ_TSQRT_U = 5,  // block j of TSQRT(k,r) applied to slab `slab` of [R(k,k); A(r,k)]
 };
 
 #define HQ_MAXDEP 6

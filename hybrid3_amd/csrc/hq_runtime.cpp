@@ -109,21 +109,26 @@ struct DagBuilder {
   int MT, NT, nb, ib, S, KT;
   bool apply;
   int XNT;
+  int NBLK;  // ib blocks per tile
   std::vector<hq_task> t;
-  std::vector<int> idg, idt, idl, ids;
+  std::vector<int> idl, ids;
+  // panel tasks of the current step k: (row r, block b); their trailing
+  // tasks: (row r, block b, slab s).  Row k is the GEQRT of the step.
+  std::vector<int> idp, idu;
   DagBuilder(int MT_, int NT_, int nb_, int ib_, bool apply_ = false, int XNT_ = 0)
       : MT(MT_), NT(NT_), nb(nb_), ib(ib_), apply(apply_), XNT(apply_ ? XNT_ : NT_) {
     S = (nb + HQ_SLAB - 1) / HQ_SLAB;
     KT = std::min(MT, NT);
-    idg.assign(KT, -1);
-    idt.assign((size_t)KT * MT, -1);
+    NBLK = (nb + ib - 1) / ib;
     idl.assign((size_t)KT * XNT * S, -1);
     ids.assign((size_t)KT * XNT * MT * S, -1);
+    idp.assign((size_t)MT * NBLK, -1);
+    idu.assign((size_t)MT * NBLK * S, -1);
   }
-  int &G(int k) { return idg[k]; }
-  int &Tq(int k, int r) { return idt[(size_t)k * MT + r]; }
   int &L(int k, int j, int s) { return idl[((size_t)k * XNT + j) * S + s]; }
   int &Sr(int k, int j, int r, int s) { return ids[(((size_t)k * XNT + j) * MT + r) * S + s]; }
+  int &P(int r, int b) { return idp[(size_t)r * NBLK + b]; }
+  int &U(int r, int b, int s) { return idu[((size_t)r * NBLK + b) * S + s]; }
   int add(int kind, int k, int r, int j, int s) {
     hq_task x{};
     x.kind = kind;
@@ -160,15 +165,46 @@ struct DagBuilder {
           }
     }
   }
+  // The panel QR of tile (r, k) (GEQRT for r == k, TSQRT below it) as one
+  // task per ib block b plus one trailing task per (b, slab) for the slabs
+  // right of the block.  Footprints: panel b writes the block's columns of
+  // tile (r,k) (rows >= b*ib for GEQRT) and, for TSQRT, the diagonal block
+  // (b,b) of R(k,k); trailing (b, s) reads them and writes tile (r,k) and,
+  // for TSQRT, R(k,k) rows of block b, in slab s right of the block.  So
+  //   panel(r,b)    <- panel(r-1,b) [R(b,b)], trailing(r,b-1,s) for the
+  //                    slabs s of block b (or the step k-1 update of them)
+  //   trailing(r,b,s) <- panel(r,b), trailing(r,b-1,s) (or step k-1),
+  //                    trailing(r-1,b,s) [R rows of block b in slab s]
+  // and the last panel of a tile closes its whole panel QR: every trailing
+  // task of the tile precedes it through the chains above.
+  void panel_tasks(int k, int r) {
+    const int kind = r == k ? HQ_GEQRT : HQ_TSQRT;
+    const int ukind = r == k ? HQ_GEQRT_U : HQ_TSQRT_U;
+    for (int b = 0; b < NBLK; b++) {
+      const int ii = b * ib, j0 = std::min(nb, ii + ib);
+      const int p = P(r, b) = add(kind, k, r, b, 0);
+      if (r > k) dep(p, P(r - 1, b));
+      for (int s = ii / HQ_SLAB; s <= (j0 - 1) / HQ_SLAB; s++)
+        dep(p, b == 0 ? (k > 0 ? Sr(k - 1, k, r, s) : -1) : U(r, b - 1, s));
+      if (j0 >= nb) continue;
+      for (int s = j0 / HQ_SLAB; s < S; s++) {
+        const int u = U(r, b, s) = add(ukind, k, r, b, s);
+        dep(u, p);
+        dep(u, b == 0 ? (k > 0 ? Sr(k - 1, k, r, s) : -1) : U(r, b - 1, s));
+        if (r > k) dep(u, U(r - 1, b, s));
+      }
+    }
+  }
   void build() {
     if (apply) {
       build_apply();
       return;
     }
     for (int k = 0; k < KT; k++) {
-      int g = G(k) = add(HQ_GEQRT, k, k, k, 0);
-      if (k > 0)
-        for (int s = 0; s < S; s++) dep(g, Sr(k - 1, k, k, s));
+      std::fill(idp.begin(), idp.end(), -1);
+      std::fill(idu.begin(), idu.end(), -1);
+      panel_tasks(k, k);
+      const int g = P(k, NBLK - 1);
       for (int j = k + 1; j < NT; j++)
         for (int s = 0; s < S; s++) {
           int l = L(k, j, s) = add(HQ_LARFB, k, k, j, s);
@@ -176,10 +212,8 @@ struct DagBuilder {
           if (k > 0) dep(l, Sr(k - 1, j, k, s));
         }
       for (int r = k + 1; r < MT; r++) {
-        int q = Tq(k, r) = add(HQ_TSQRT, k, r, k, 0);
-        dep(q, r == k + 1 ? g : Tq(k, r - 1));
-        if (k > 0)
-          for (int s = 0; s < S; s++) dep(q, Sr(k - 1, k, r, s));
+        panel_tasks(k, r);
+        const int q = P(r, NBLK - 1);
         for (int j = k + 1; j < NT; j++)
           for (int s = 0; s < S; s++) {
             int x = Sr(k, j, r, s) = add(HQ_SSRFB, k, r, j, s);
@@ -196,9 +230,12 @@ struct DagBuilder {
   double cost(const hq_task &x) const {
     const double f = nb / 256.0;
     const double w = (double)std::min(HQ_SLAB, nb) / 64.0;  // slab width / 64
+    const double c = std::min(ib, nb - x.j * ib) / 32.0;     // panel block width / 32
     switch (x.kind) {
-      case HQ_GEQRT: return 590.0 * f * f * f;
-      case HQ_TSQRT: return 630.0 * f * f * f;
+      case HQ_GEQRT: return 36.0 * f * c;
+      case HQ_TSQRT: return 38.0 * f * c;
+      case HQ_GEQRT_U: return 11.0 * f * c * w * (nb - x.j * ib) / nb;
+      case HQ_TSQRT_U: return 11.0 * f * c * w;
       case HQ_LARFB: return 99.0 * f * f * w;
       default: return 83.0 * f * f * w;
     }
@@ -692,9 +729,16 @@ int hq_wall_clock_khz(int *khz) {
 int hq_schedule_info(int MT, int NT, int nb, int ib, int *ntasks, int *grid) {
   if (!geometry_ok(MT, NT, nb, ib)) return g_err;
   int KT = std::min(MT, NT), S = (nb + HQ_SLAB - 1) / HQ_SLAB;
+  // tasks of one panel QR (GEQRT or TSQRT): a panel per ib block and a
+  // trailing task per slab right of the block
+  long per_tile = 0;
+  for (int ii = 0; ii < nb; ii += ib) {
+    const int j0 = std::min(nb, ii + ib);
+    per_tile += 1 + (j0 < nb ? S - j0 / HQ_SLAB : 0);
+  }
   long n = 0;
   for (int k = 0; k < KT; k++)
-    n += 1 + (MT - k - 1) + (long)(NT - k - 1) * S + (long)(NT - k - 1) * (MT - k - 1) * S;
+    n += (MT - k) * per_tile + (long)(NT - k - 1) * S + (long)(NT - k - 1) * (MT - k - 1) * S;
   if (ntasks) *ntasks = (int)n;
   if (grid) {
     int dev = 0, cu = 256;

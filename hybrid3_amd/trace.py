@@ -16,7 +16,7 @@ import json
 
 import numpy as np
 
-KINDS = ["GEQRT", "TSQRT", "LARFB", "SSRFB"]
+KINDS = ["GEQRT", "TSQRT", "LARFB", "SSRFB", "GEQRT_U", "TSQRT_U"]
 
 
 def run_traced(MT, NT, nb, ib, A, TAU, T=None, stream=None):
@@ -85,6 +85,7 @@ def main():
     ap.add_argument("--ib", type=int, default=32)
     ap.add_argument("--seed", type=int, default=20151130)
     ap.add_argument("--chrome", default="")
+    ap.add_argument("--raw", default="", help="write the task list and stamps (us) to this .npz")
     args = ap.parse_args()
     import torch
 
@@ -104,6 +105,8 @@ def main():
     print(json.dumps(s, indent=1))
     if args.chrome:
         chrome(tasks, us, args.chrome)
+    if args.raw:
+        np.savez_compressed(args.raw, tasks=tasks, us=us)
 
 
 if __name__ == "__main__":

@@ -2,11 +2,14 @@
 MEM / LIN / YTM dependency bits, Hybrid_tile_QR.c:36-40,496-601,1008-1440):
 claim order is topological, every pair of tasks whose footprints conflict is
 ordered, and executing the DAG in random topological orders with the
-oracle's tile kernels reproduces the sequential factorisation bit for bit."""
+oracle's tile kernels reproduces the sequential factorisation bit for bit.
+GEQRT / TSQRT appear as one panel task per ib block and one trailing task per
+(block, 64-column slab); the oracle runs them as the same LAPACK-style steps
+on the block's sub-matrices."""
 import numpy as np
 import pytest
 
-GEQRT, TSQRT, LARFB, SSRFB = 0, 1, 2, 3
+GEQRT, TSQRT, LARFB, SSRFB, GEQRT_U, TSQRT_U = 0, 1, 2, 3, 4, 5
 
 
 @pytest.fixture(scope="module")
@@ -26,9 +29,17 @@ def test_claim_order_is_topological_and_complete(hq, MT, NT, nb, XNT):
     if XNT > 0:
         expect = sum(W * S + (MT - k - 1) * W * S for k in range(KT))
     else:
-        expect = sum(1 + (MT - k - 1) + (NT - k - 1) * S + (NT - k - 1) * (MT - k - 1) * S
+        ib = 16
+        nblk = (nb + ib - 1) // ib
+        # panel tasks of one tile: a panel per block + a trailing task per
+        # slab right of the block
+        per_tile = sum(1 + (S - (b * ib + ib) // 64 if b * ib + ib < nb else 0)
+                       for b in range(nblk))
+        expect = sum((MT - k) * per_tile + (NT - k - 1) * S + (NT - k - 1) * (MT - k - 1) * S
                      for k in range(KT))
     assert n == expect
+    if XNT == 0:
+        assert hq.schedule_info(MT, NT, nb, 16)[0] == n
     for i in range(n):
         for d in deps[i]:
             assert d < i
@@ -36,32 +47,46 @@ def test_claim_order_is_topological_and_complete(hq, MT, NT, nb, XNT):
     assert len(seen) == n
 
 
-def footprint(t, nb, apply_mode):
-    """(array, ti, tj, part, cols, write) items; part U/L/A = upper incl.
-    diagonal / strictly lower / whole tile; cols = column slab range."""
+def footprint(t, nb, ib, apply_mode):
+    """(array, ti, tj, part, rows, cols, write) items: rows / cols are
+    half-open index ranges inside the tile; part U / L / A = the entries with
+    row <= col / row > col / all of that rectangle."""
     kind, k, r, j, s = t
     full = (0, nb)
     slab = (64 * s, min(nb, 64 * s + 64))
     X = "X" if apply_mode else "A"
+    if kind in (GEQRT, TSQRT, GEQRT_U, TSQRT_U):
+        ii, j0 = j * ib, min(nb, j * ib + ib)
+        blk = (ii, j0)
+        right = (max(slab[0], j0), slab[1])
     if kind == GEQRT:
-        return [("A", k, k, "A", full, True), ("T", k, k, "A", full, True)]
+        return [("A", k, k, "A", (ii, nb), blk, True), ("T", k, k, "A", full, blk, True)]
+    if kind == GEQRT_U:
+        return [("A", k, k, "L", (ii, nb), blk, False), ("T", k, k, "A", full, blk, False),
+                ("A", k, k, "A", (ii, nb), right, True)]
     if kind == TSQRT:
-        return [("A", k, k, "U", full, True), ("A", r, k, "A", full, True), ("T", r, k, "A", full, True)]
+        return [("A", k, k, "U", blk, blk, True), ("A", r, k, "A", full, blk, True),
+                ("T", r, k, "A", full, blk, True)]
+    if kind == TSQRT_U:
+        return [("A", r, k, "A", full, blk, False), ("T", r, k, "A", full, blk, False),
+                ("A", k, k, "U", blk, right, True), ("A", r, k, "A", full, right, True)]
     if kind == LARFB:
-        return [("A", k, k, "L", full, False), ("T", k, k, "A", full, False),
-                (X, k, j, "A", slab, True)]
-    return [("A", r, k, "A", full, False), ("T", r, k, "A", full, False),
-            (X, k, j, "A", slab, True), (X, r, j, "A", slab, True)]
+        return [("A", k, k, "L", full, full, False), ("T", k, k, "A", full, full, False),
+                (X, k, j, "A", full, slab, True)]
+    return [("A", r, k, "A", full, full, False), ("T", r, k, "A", full, full, False),
+            (X, k, j, "A", full, slab, True), (X, r, j, "A", full, slab, True)]
 
 
 def conflicts(fa, fb):
+    def meet(x, y):
+        return x[0] < y[1] and y[0] < x[1]
     for a in fa:
         for b in fb:
-            if a[:3] != b[:3] or not (a[4] or b[4]):
+            if a[:3] != b[:3] or not (a[6] or b[6]):
                 continue
             if a[3] != "A" and b[3] != "A" and a[3] != b[3]:
                 continue
-            if a[4][0] < b[4][1] and b[4][0] < a[4][1] and (a[5] or b[5]):
+            if meet(a[4], b[4]) and meet(a[5], b[5]):
                 return True
     return False
 
@@ -69,14 +94,15 @@ def conflicts(fa, fb):
 @pytest.mark.parametrize("MT,NT,nb,XNT", [(3, 3, 128, 0), (4, 2, 128, 0), (2, 4, 64, 0),
                                           (3, 3, 128, 3)])
 def test_every_conflicting_pair_is_ordered(hq, MT, NT, nb, XNT):
-    tasks, deps = hq.dag_host(MT, NT, nb, 16, XNT, 8)
+    ib = 16 if nb != 128 else 24
+    tasks, deps = hq.dag_host(MT, NT, nb, ib, XNT, 8)
     n = len(tasks)
     reach = [set() for _ in range(n)]   # ancestors
     for i in range(n):
         for d in deps[i]:
             if d >= 0:
                 reach[i] |= reach[d] | {d}
-    fps = [footprint(t, nb, XNT > 0) for t in tasks]
+    fps = [footprint(t, nb, ib, XNT > 0) for t in tasks]
     for i in range(n):
         for j in range(i):
             if conflicts(fps[i], fps[j]):
@@ -100,17 +126,29 @@ def run_task(orc, t, A, TAU, T, MT, nb, ib):
         return TAU[o:o + nb]
 
     cs = slice(64 * s, min(nb, 64 * s + 64))
-    if kind == GEQRT:
-        a, tt, tv = orc.dgeqrt(tile(k, k), ib)
-        tile(k, k)[:] = a
-        tf(k, k)[:] = tt
-        tau(k, k)[:] = tv
-    elif kind == TSQRT:
-        a1, a2, tt, tv = orc.dtsqrt(tile(k, k), tile(r, k), ib)
-        tile(k, k)[:] = a1
-        tile(r, k)[:] = a2
-        tf(r, k)[:] = tt
-        tau(r, k)[:] = tv
+    if kind in (GEQRT, TSQRT, GEQRT_U, TSQRT_U):
+        ii, j0 = j * ib, min(nb, j * ib + ib)
+        sb = j0 - ii
+        rc = slice(max(64 * s, j0), min(nb, 64 * s + 64))
+    if kind == GEQRT:      # dgeqr2 + dlarft of the block (rows ii..)
+        a, tt, tv = orc.dgeqrt(tile(k, k)[ii:, ii:j0], sb)
+        tile(k, k)[ii:, ii:j0] = a
+        tf(k, k)[:sb, ii:j0] = tt
+        tau(k, k)[ii:j0] = tv
+    elif kind == GEQRT_U:  # dlarfb of the block on the slab's columns right of it
+        tile(k, k)[ii:, rc] = orc.dormqr_lt(tile(k, k)[ii:, ii:j0], tf(k, k)[:sb, ii:j0],
+                                            tile(k, k)[ii:, rc], sb, k=sb)
+    elif kind == TSQRT:    # tsqr2 + T of the block of [R(k,k); A(r,k)]
+        a1, a2, tt, tv = orc.dtsqrt(tile(k, k)[ii:j0, ii:j0], tile(r, k)[:, ii:j0], sb)
+        tile(k, k)[ii:j0, ii:j0] = np.triu(a1) + np.tril(tile(k, k)[ii:j0, ii:j0], -1)
+        tile(r, k)[:, ii:j0] = a2
+        tf(r, k)[:sb, ii:j0] = tt
+        tau(r, k)[ii:j0] = tv
+    elif kind == TSQRT_U:  # parfb of the block on [R rows ii..j0; A(r,k)], slab columns
+        a1, a2 = orc.dtsmqr_lt(tile(k, k)[ii:j0, rc], tile(r, k)[:, rc], tile(r, k)[:, ii:j0],
+                               tf(r, k)[:sb, ii:j0], sb, k=sb)
+        tile(k, k)[ii:j0, rc] = a1
+        tile(r, k)[:, rc] = a2
     elif kind == LARFB:
         tile(k, j)[:, cs] = orc.dormqr_lt(tile(k, k), tf(k, k), tile(k, j)[:, cs], ib, k=nb)
     else:
