@@ -606,11 +606,34 @@ __device__ __forceinline__ void stage_columns(double *Vs, const double *tile, in
 // R(k,k)) advances one panel at a time instead of one tile at a time.  Each
 // column's arithmetic is the same either way.
 
+// W = V T_s for each 16-column sub-block s of the panel's block (T_s its
+// 16x16 diagonal block; ib % 16 == 0), from V in S.V and T^T in S.TT,
+// stored to columns ii.. of the W tile Wt (ld = NB).  MFMA (rb, h): rows
+// 16rb.. of sub-block h, contraction over its 16 reflectors.
+template <int NB>
+__device__ __forceinline__ void store_vt(Smem<NB> &S, double *Wt, int ii, int sb) {
+  constexpr int LDV = Smem<NB>::LDV, LDT = Smem<NB>::LDT;
+  const int lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int pm = arow(m);
+  const int nq = (sb / 16) * (NB / 16);
+  for (int q = wave; q < nq; q += HQ_THREADS / 64) {
+    const int h = q / (NB / 16), rb = q - h * (NB / 16);
+    d4 acc = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+      acc = mfma(S.V[(16 * h + 4 * g + s) * LDV + 16 * rb + pm], S.TT[(16 * h + m) * LDT + 16 * h + 4 * g + s],
+                 acc);
+    *reinterpret_cast<d4 *>(Wt + (size_t)(ii + 16 * h + m) * NB + 16 * rb + 4 * g) = acc;
+  }
+}
+
 // GEQRT panel of columns ii .. ii+sb-1: R / V written back to A, T and TAU
 // of the block written; leaves V (unit lower trapezoidal) in S.V and T^T in
 // S.TT.
 template <int NB, int WB>
-__device__ __forceinline__ void geqrt_panel(Smem<NB> &S, double *A, double *T, double *TAU, int ib, int ii) {
+__device__ __forceinline__ void geqrt_panel(Smem<NB> &S, double *A, double *T, double *TAU, int ib, int ii,
+                                            double *Wt = nullptr, double *Vd = nullptr) {
   constexpr int LDV = Smem<NB>::LDV;
   const int tid = threadIdx.x, wave = tid >> 6;
   const int sb = min(ib, NB - ii);
@@ -632,6 +655,13 @@ __device__ __forceinline__ void geqrt_panel(Smem<NB> &S, double *A, double *T, d
   __syncthreads();
   if (wave == 0) build_t<NB>(S, sb, T + (size_t)ii * ib, ib, TAU + ii);
   __syncthreads();
+  if (Wt) {
+    store_vt<NB>(S, Wt, ii, sb);
+    for (int e = tid; e < sb * NB; e += HQ_THREADS) {
+      const int c = e / NB, r = e - c * NB;
+      Vd[(size_t)(ii + c) * NB + r] = S.V[c * LDV + r];
+    }
+  }
 }
 
 // TSQRT panel of columns ii .. ii+sb-1 of [R(A1); A2]: V2 written to A2, the
@@ -1244,20 +1274,21 @@ __device__ __forceinline__ void task_update_f(Smem<NB> &S, const double *V, cons
 }
 
 // --------------------------------------------------------------------------
-// Update pipeline with the products W_s = V_s T_s precomputed per reflector
-// tile (T_s the 16x16 diagonal block of T for sub-block s), for ib a
-// multiple of 16 (every sub-block 16 wide, s = 0 .. NB/16-1).  Since
-// T_s^T V_s^T X = W_s^T X, the accumulate phases of pass s produce
-// T_{s+1}^T (A1_{s+1} + V_{s+1}^T X) directly: the four T^T A1 MFMAs use no
-// result of the pass and run in its second phase, and the end of a pass is
-// one add and a negation instead of a chain of four dependent MFMAs.
-// Ring: V_s in slot s&1, W_s in slot 2+(s&1), T_s^T (SSRFB) in TT slot s&1.
-// Pass t loads V_{t+1} and W_{t+2} (direct-to-LDS, one op per phase).
-// V is the unit lower trapezoidal copy for LARFB (no fix-up in LDS).
+// LARFB pipeline with W_s = V_s T_s precomputed for the diagonal tile (T_s
+// the 16x16 diagonal block of T for sub-block s; ib a multiple of 16, so
+// every sub-block is 16 wide, s = 0 .. NB/16-1).  Since T_s^T V_s^T X =
+// W_s^T X, the accumulate phases of pass s produce T_{s+1}^T V_{s+1}^T X
+// directly and the end of a pass is one add and a negation instead of a
+// chain of four dependent MFMAs.  V is the unit lower trapezoidal copy the
+// GEQRT panels write, so no fix-up pass in LDS either.
+// Ring: V_s in slot s&1, W_s in slot 2+(s&1); pass t loads V_{t+1} and
+// W_{t+2} (direct-to-LDS, one op per phase).
+// (SSRFB keeps the T multiplication: there, streaming W beside V doubles the
+// tile traffic of every slab task, which costs more than the chain.)
 // --------------------------------------------------------------------------
 template <int NB>
 struct PassW {
-  const double *Vu, *Vw, *TTw;
+  const double *Vu, *Vw;
   double *D1, *D2;        // ring slots receiving this pass's loads
   const double *G1, *G2;  // their source tiles
   SubBlk s1, s2;
@@ -1269,24 +1300,13 @@ __device__ __forceinline__ SubBlk sub16(int s) {
   r.p0 = 0;
   return r;
 }
-// T_s^T of sub-block s (ib % 16 == 0) into TT slot layout c*17 + p: the raw
-// entry and the upper-triangle factor, as tt_raw / tt_mask.
-__device__ __forceinline__ double tt16_raw(const double *T, int ib, int s) {
-  const int c = threadIdx.x >> 4, p = threadIdx.x & 15;
-  return T[(size_t)(16 * s + c) * ib + ((16 * s) & (ib - 1)) + min(p, c)];
-}
-__device__ __forceinline__ double tt16_mask() {
-  const int c = threadIdx.x >> 4, p = threadIdx.x & 15;
-  return p <= c ? 1.0 : 0.0;
-}
 
-template <int NB, bool UNIT, bool DO_U, bool DO_A, int NDMA>
+template <int NB, bool DO_U, bool DO_A, int NDMA>
 __device__ __forceinline__ void fpass_w(d4 (&acc)[NB / 16], const d4 &W2, d4 (&Wn)[2], double (&o)[2][8],
-                                        const d4 &a1c, const PassW<NB> &P) {
+                                        const PassW<NB> &P) {
   constexpr int NG = NB / 32;
   constexpr int NP = (DO_U ? NG : 0) + (DO_A ? NG : 0);
   constexpr int NOPS = FDma<NB>::NOPS;
-  constexpr bool TA = !UNIT && DO_U && DO_A;
   static_assert(NP >= NDMA * NOPS, "one direct-to-LDS op per phase");
   auto kind = [](int p) -> int {  // 1 = update, 0 = accumulate
     if (DO_U && !DO_A) return 1;
@@ -1301,30 +1321,17 @@ __device__ __forceinline__ void fpass_w(d4 (&acc)[NB / 16], const d4 &W2, d4 (&W
     if (p == 2 * NG - 1) return NG - 1;
     return (p & 1) ? (p + 1) / 2 : p / 2 - 1;
   };
-  const int lane = threadIdx.x & 63, g = lane >> 4, pm = arow(lane & 15);
-  double tto[4];
 #pragma unroll
   for (int p = 0; p < NP; p++) {
     if (NDMA >= 1 && p < NOPS)
       dma_op<NB>(p, P.D1, P.G1, P.s1);
     else if (NDMA >= 2 && p < 2 * NOPS)
       dma_op<NB>(p - NOPS, P.D2, P.G2, P.s2);
-    if (TA && p == 0) {
-#pragma unroll
-      for (int sp = 0; sp < 4; sp++) tto[sp] = P.TTw[pm * LDT16 + 4 * g + sp];
-    }
     if (p + 1 < NP) {
       if (kind(p + 1))
-        ld_u<NB, UNIT>(o[(p + 1) & 1], P.Vu, grp(p + 1), 0, 16);
+        ld_u<NB, true>(o[(p + 1) & 1], P.Vu, grp(p + 1), 0, 16);
       else
-        ld_a<NB, UNIT>(o[(p + 1) & 1], P.Vw, grp(p + 1), 0, 16);
-    }
-    if (TA && p == 1) {
-      // T^T A1 of the sub-block accumulated in this pass (independent of it)
-      Wn[1] = d4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int sp = 0; sp < 4; sp++) Wn[1] = mfma(tto[sp], a1c[sp], Wn[1]);
-      __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+        ld_a<NB, true>(o[(p + 1) & 1], P.Vw, grp(p + 1), 0, 16);
     }
     const int G = grp(p);
     if (kind(p)) {
@@ -1347,68 +1354,38 @@ __device__ __forceinline__ void fpass_w(d4 (&acc)[NB / 16], const d4 &W2, d4 (&W
   }
 }
 
-// LARFB (UNIT: X <- Q^T X, V unit lower trapezoidal) or SSRFB ([A1; X] <-
-// Q^T [A1; X], V = [I; V2]) of one 64-column slab with W = V T16 given.
-template <int NB, bool UNIT>
-__device__ __forceinline__ void task_update_w(Smem<NB> &S, const double *V, const double *Wt, const double *T,
-                                              double *X, double *A1, int ib, int slab) {
+// LARFB (X <- Q^T X) of one 64-column slab: V unit lower trapezoidal, W =
+// V T16 (both NB x NB, ld = NB).
+template <int NB>
+__device__ __forceinline__ void task_larfb_w(Smem<NB> &S, const double *V, const double *Wt, double *X, int slab) {
   static_assert(NB % HQ_SLAB == 0 && NB % 32 == 0, "slabs and row-block groups cover the tile");
   constexpr int NSUB = NB / 16;
-  const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, g = lane >> 4, wave = tid >> 6;
-  const int pm = arow(m);
+  const int lane = threadIdx.x & 63, m = lane & 15, wave = threadIdx.x >> 6;
   const int col = slab * HQ_SLAB + 16 * wave + m;
-  double *a1col = UNIT ? nullptr : A1 + (size_t)col * NB;
   d4 acc[NB / 16];
   const d4 zero = d4{0.0, 0.0, 0.0, 0.0};
-  d4 W2 = zero, Wn[2], a1c = zero, a1n = zero, a1o = zero;
+  d4 W2 = zero, Wn[2];
   double o[2][8];
-  double treg = 0.0;
-  // prologue: W_0, V_0, T_0^T and T_0^T A1_0, the strip
-  {
+  // prologue: W_0, V_0, the strip
 #pragma unroll
-    for (int q = 0; q < FDma<NB>::NOPS; q++) dma_op<NB>(q, fvbuf(S, 2), Wt, sub16(0));
+  for (int q = 0; q < FDma<NB>::NOPS; q++) dma_op<NB>(q, fvbuf(S, 2), Wt, sub16(0));
 #pragma unroll
-    for (int q = 0; q < FDma<NB>::NOPS; q++) dma_op<NB>(q, fvbuf(S, 0), V, sub16(0));
-    if (!UNIT) {
-      ftbuf(S, 0)[(tid >> 4) * LDT16 + (tid & 15)] = tt16_raw(T, ib, 0) * tt16_mask();
-      a1c = *reinterpret_cast<const d4 *>(a1col + 4 * g);
-      a1n = *reinterpret_cast<const d4 *>(a1col + 16 + 4 * g);
-      treg = tt16_raw(T, ib, 1);
-    }
-    load_strip<NB>(acc, X, col, true, 0);
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    Wn[0] = zero;
-    Wn[1] = zero;
-    if (!UNIT) {
-      const double *tt = ftbuf(S, 0);
-#pragma unroll
-      for (int sp = 0; sp < 4; sp++) Wn[1] = mfma(tt[pm * LDT16 + 4 * g + sp], a1c[sp], Wn[1]);
-    }
-    ld_a<NB, UNIT>(o[0], fvbuf(S, 2), 0, 0, 16);
-  }
-  // after pass t-1: W2_t = -(W_t^T X + T_t^T A1_t); the new A1 rows of
-  // sub-block t (stored after the barrier); T_{t+1}^T into its TT slot
+  for (int q = 0; q < FDma<NB>::NOPS; q++) dma_op<NB>(q, fvbuf(S, 0), V, sub16(0));
+  load_strip<NB>(acc, X, col, true, 0);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  Wn[0] = zero;
+  Wn[1] = zero;
+  ld_a<NB, true>(o[0], fvbuf(S, 2), 0, 0, 16);
+  // after pass t-1: W2_t = -W_t^T X
   auto post = [&](int t) {
     W2 = -(Wn[0] + Wn[1]);
-    if (!UNIT) a1o = a1c + W2;
-    ld_u<NB, UNIT>(o[0], fvbuf(S, t & 1), 0, 0, 16);
-    if (!UNIT) ftbuf(S, (t + 1) & 1)[(tid >> 4) * LDT16 + (tid & 15)] = treg * tt16_mask();
     // every load of the pass has landed (direct-to-LDS loads are not taken
-    // to complete in order, so no counted wait)
+    // to complete in order, so no counted wait); V_t came in during this
+    // pass, so its first operands are read only after the barrier
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    if (!UNIT) *reinterpret_cast<d4 *>(a1col + 16 * t + 4 * g) = a1o;
-  };
-  // before pass t: A1 rows of sub-block t+1 (loaded one pass ahead) and the
-  // loads of rows t+2 and T_{t+2}^T (clamped to the last sub-block)
-  auto pre = [&](int t) {
-    if (!UNIT) {
-      const int tn = min(t + 2, NSUB - 1);
-      a1c = a1n;
-      a1n = *reinterpret_cast<const d4 *>(a1col + 16 * tn + 4 * g);
-      treg = tt16_raw(T, ib, tn);
-    }
+    ld_u<NB, true>(o[0], fvbuf(S, t & 1), 0, 0, 16);
     Wn[0] = zero;
     Wn[1] = zero;
   };
@@ -1416,43 +1393,39 @@ __device__ __forceinline__ void task_update_w(Smem<NB> &S, const double *V, cons
     PassW<NB> P;
     P.Vu = fvbuf(S, 0);
     P.Vw = fvbuf(S, 2);
-    P.TTw = ftbuf(S, 0);
     P.D1 = fvbuf(S, 3);
     P.G1 = Wt;
     P.s1 = sub16(1);
     P.D2 = P.D1;
     P.G2 = P.G1;
     P.s2 = P.s1;
-    fpass_w<NB, UNIT, false, true, 1>(acc, W2, Wn, o, a1c, P);
+    fpass_w<NB, false, true, 1>(acc, W2, Wn, o, P);
     post(0);
   }
   for (int t = 0; t + 1 < NSUB; t++) {
-    pre(t);
     PassW<NB> P;
     P.Vu = fvbuf(S, t & 1);
     P.Vw = fvbuf(S, 2 + ((t + 1) & 1));
-    P.TTw = ftbuf(S, (t + 1) & 1);
     P.D1 = fvbuf(S, (t + 1) & 1);
     P.G1 = V;
     P.s1 = sub16(t + 1);
     P.D2 = fvbuf(S, 2 + (t & 1));
     P.G2 = Wt;
     P.s2 = sub16(min(t + 2, NSUB - 1));
-    fpass_w<NB, UNIT, true, true, 2>(acc, W2, Wn, o, a1c, P);
+    fpass_w<NB, true, true, 2>(acc, W2, Wn, o, P);
     post(t + 1);
   }
   {  // last pass: update only
     PassW<NB> P;
     P.Vu = fvbuf(S, (NSUB - 1) & 1);
     P.Vw = P.Vu;
-    P.TTw = ftbuf(S, 0);
     P.D1 = nullptr;
     P.G1 = nullptr;
     P.s1 = sub16(0);
     P.D2 = nullptr;
     P.G2 = nullptr;
     P.s2 = P.s1;
-    fpass_w<NB, UNIT, true, false, 0>(acc, W2, Wn, o, a1c, P);
+    fpass_w<NB, true, false, 0>(acc, W2, Wn, o, P);
   }
   // no direct-to-LDS load may still be in flight when the next task reuses LDS
   __builtin_amdgcn_s_waitcnt(0);
@@ -1674,10 +1647,13 @@ __device__ __forceinline__ void task_update_f32(Smem<NB> &S, const double *V, co
 
 template <int NB, int WB>
 __device__ __forceinline__ void task_larfb(Smem<NB> &S, const double *V, const double *T, double *C,
-                                           int ib, int slab, int prec = 0) {
+                                           int ib, int slab, int prec = 0, const double *Wt = nullptr,
+                                           const double *Vd = nullptr) {
   if constexpr (NB >= 128) {
     if (prec)
       task_update_f32<NB, true>(S, V, T, C, nullptr, ib, slab);
+    else if (Wt)
+      task_larfb_w<NB>(S, Vd, Wt, C, slab);
     else
       task_update_f<NB, true>(S, V, T, C, nullptr, ib, slab);
   } else {
@@ -1707,6 +1683,12 @@ __device__ __forceinline__ double *xtile_ptr(const hq_geom &G, int i, int j) {
 __device__ __forceinline__ double *t_ptr(const hq_geom &G, int i, int j) {
   return G.T + ((size_t)i + (size_t)j * G.MT) * (size_t)G.ib * G.nb;
 }
+__device__ __forceinline__ double *wd_ptr(const hq_geom &G, int k) {
+  return G.W ? G.W + (size_t)k * G.nb * G.nb : nullptr;
+}
+__device__ __forceinline__ double *vd_ptr(const hq_geom &G, int k) {
+  return G.W ? G.VD + (size_t)k * G.nb * G.nb : nullptr;
+}
 __device__ __forceinline__ double *tau_ptr(const hq_geom &G, int i, int j) {
   return G.TAU + ((size_t)i + (size_t)j * G.MT) * (size_t)G.nb;
 }
@@ -1715,7 +1697,8 @@ template <int NB, int WB>
 __device__ __forceinline__ void run_task(Smem<NB> &S, const hq_geom &G, const hq_task &t) {
   switch (t.kind) {
     case HQ_GEQRT:
-      geqrt_panel<NB, WB>(S, tile_ptr(G, t.k, t.k), t_ptr(G, t.k, t.k), tau_ptr(G, t.k, t.k), G.ib, t.j * G.ib);
+      geqrt_panel<NB, WB>(S, tile_ptr(G, t.k, t.k), t_ptr(G, t.k, t.k), tau_ptr(G, t.k, t.k), G.ib, t.j * G.ib,
+                          wd_ptr(G, t.k), vd_ptr(G, t.k));
       break;
     case HQ_TSQRT:
       tsqrt_panel<NB, WB>(S, tile_ptr(G, t.k, t.k), tile_ptr(G, t.r, t.k), t_ptr(G, t.r, t.k),
@@ -1730,7 +1713,7 @@ __device__ __forceinline__ void run_task(Smem<NB> &S, const hq_geom &G, const hq
       break;
     case HQ_LARFB:
       task_larfb<NB, WB>(S, tile_ptr(G, t.k, t.k), t_ptr(G, t.k, t.k), xtile_ptr(G, t.k, t.j), G.ib,
-                         t.slab, G.prec);
+                         t.slab, G.prec, wd_ptr(G, t.k), vd_ptr(G, t.k));
       break;
     case HQ_SSRFB:
       task_ssrfb<NB, WB>(S, xtile_ptr(G, t.k, t.j), xtile_ptr(G, t.r, t.j), tile_ptr(G, t.r, t.k),

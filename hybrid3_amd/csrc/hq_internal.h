@@ -47,6 +47,13 @@ struct hq_geom {
                 // the Q / C array (MT x XNT tiles) while applying Q^T
   int XNT;
   int prec;     // 0: FP64 everywhere; 1: LARFB / SSRFB in FP32 (panels FP64)
+  // Optional (FP64 factorisation, nb >= 128, ib % 16 == 0): for each
+  // diagonal tile k, W = V T16 at W + k*nb*nb (T16 = the 16x16 diagonal
+  // blocks of T) and its unit lower trapezoidal V at VD + k*nb*nb, written by
+  // the GEQRT panels and read by LARFB.  nullptr: LARFB multiplies by T
+  // itself.
+  double *W;
+  double *VD;
 };
 
 // One matrix of a batch for the layout kernels: m x n entries in an MT x NT

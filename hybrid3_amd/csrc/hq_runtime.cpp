@@ -319,12 +319,13 @@ struct Context {
   int num_cu = 0;
   hipStream_t stream = nullptr;
   DevBuf flags, counter, T, A, TAU, X, hg;
+  DevBuf W, VD;  // W = V T16 and unit lower V of the diagonal tiles (FP64 factorisation)
   int epoch = 0;
   std::map<std::tuple<int, int, int, int, int>, std::unique_ptr<CachedSched>> scheds;
   std::map<std::pair<int, int>, std::unique_ptr<DevBuf>> tilelists;  // (MT, NT) -> int2 list
   // batches: key = (nb, ib, MT_0, NT_0, MT_1, NT_1, ...)
   std::map<std::vector<int>, std::unique_ptr<CachedSched>> batch_scheds;
-  DevBuf geoms, bmats, Tbatch;
+  DevBuf geoms, bmats, Tbatch, Wbatch, VDbatch;
 };
 
 std::mutex g_mu;
@@ -439,6 +440,14 @@ int tile_qr_dev_impl(int MT, int NT, int nb, int ib, double *A, double *TAU, dou
   hipStream_t st = pick_stream(c, stream);
   if (ensure_T(c, MT, NT, nb, ib, &T) != HQ_OK) return g_err;
   hq_geom G{A, TAU, T, MT, NT, nb, ib, A, NT, prec};
+  if (!prec && nb >= 128 && ib % 16 == 0) {
+    // the GEQRT panels also write W = V T16 and the unit lower V of the
+    // diagonal tiles; LARFB then needs no T multiplication between passes
+    const size_t tiles = sizeof(double) * (size_t)nb * nb * std::min(MT, NT);
+    if (c->W.reserve(tiles) != HQ_OK || c->VD.reserve(tiles) != HQ_OK) return g_err;
+    G.W = (double *)c->W.p;
+    G.VD = (double *)c->VD.p;
+  }
   if (serial) {
     std::vector<hq_task> tl;
     host_schedule(MT, NT, nb, ib, c->num_cu, tl);
@@ -568,8 +577,20 @@ int batched_qr_impl(int count, int nb, int ib, const int *MT, const int *NT, dou
   for (int i = 0; i < count; i++)
     if (!T || !T[i]) tdoubles += (size_t)MT[i] * NT[i] * ib * nb;
   if (tdoubles && c->Tbatch.reserve(sizeof(double) * tdoubles) != HQ_OK) return g_err;
+  // W = V T16 and unit lower V of the diagonal tiles, as for one matrix
+  const bool wpath = nb >= 128 && ib % 16 == 0;
+  size_t wdoubles = 0, vddoubles = 0;
+  if (wpath) {
+    for (int i = 0; i < count; i++) {
+      wdoubles += (size_t)std::min(MT[i], NT[i]) * nb * nb;
+      vddoubles += (size_t)std::min(MT[i], NT[i]) * nb * nb;
+    }
+    if (c->Wbatch.reserve(sizeof(double) * wdoubles) != HQ_OK ||
+        c->VDbatch.reserve(sizeof(double) * vddoubles) != HQ_OK)
+      return g_err;
+  }
   std::vector<hq_geom> g(count);
-  size_t toff = 0;
+  size_t toff = 0, woff = 0, vdoff = 0;
   for (int i = 0; i < count; i++) {
     double *Ti = T ? T[i] : nullptr;
     if (!Ti) {
@@ -577,6 +598,12 @@ int batched_qr_impl(int count, int nb, int ib, const int *MT, const int *NT, dou
       toff += (size_t)MT[i] * NT[i] * ib * nb;
     }
     g[i] = hq_geom{A[i], TAU[i], Ti, MT[i], NT[i], nb, ib, A[i], NT[i]};
+    if (wpath) {
+      g[i].W = (double *)c->Wbatch.p + woff;
+      g[i].VD = (double *)c->VDbatch.p + vdoff;
+      woff += (size_t)std::min(MT[i], NT[i]) * nb * nb;
+      vdoff += (size_t)std::min(MT[i], NT[i]) * nb * nb;
+    }
   }
   CachedSched *cs;
   if (get_batch_schedule(c, count, nb, ib, MT, NT, &cs) != HQ_OK) return g_err;
