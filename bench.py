@@ -41,6 +41,7 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven-QR measurement")
     ap.add_argument("--batch-count", type=int, default=4000)
     ap.add_argument("--no-lsq", action="store_true", help="skip the mixed-precision least-squares part")
+    ap.add_argument("--no-tall", action="store_true", help="skip the 65536 x 4096 QR / dorgqr / dormqr part")
     ap.add_argument("--traffic-file",
                     default=os.path.join(ROOT, "profiles", "sched_kernel_traffic.txt"))
     return ap.parse_args()
@@ -88,6 +89,67 @@ def batched_bench(count):
         "tuner": report,
         "max_col_norm_rel_diff": err,
     }
+
+
+def tall_bench(m=65536, n=4096, nrhs=64, nb=256, ib=32, seed=20151130, reps=3):
+    """Tall FP64 tile QR (m x n), the Q^T array of dorgqr (Q^T applied to the
+    first n columns of I), and dormqr applying Q^T to an m x nrhs block (one
+    tile column, so the kernel does the work of nb columns), each timed with
+    HIP events, best of `reps`.  Rates use the algorithmic counts
+    2mn^2 - 2n^3/3, 4mnk - 2(m+n)k^2 + 4k^3/3 (k = n) and 4 m nrhs k - 2 nrhs k^2.
+    Check: Q^T A reproduces R in the top rows and zeros below."""
+    import torch
+
+    import hybrid3_amd as hq
+    MT, NT = m // nb, n // nb
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    pristine = torch.empty(MT * NT * nb * nb, dtype=torch.float64, device=dev)
+    hq.fill_uniform_dev(MT, NT, nb, seed, pristine)
+    A = torch.empty_like(pristine)
+    TAU = torch.zeros(MT * NT * nb, dtype=torch.float64, device=dev)
+    T = torch.zeros(MT * NT * ib * nb, dtype=torch.float64, device=dev)
+    Q = torch.empty_like(pristine)
+    C = torch.empty(MT * nb * nb, dtype=torch.float64, device=dev)
+
+    def timed(fn, before=None):
+        best = float("inf")
+        for _ in range(reps + 1):   # the first call builds the schedule
+            if before is not None:
+                before()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+            hq.sync()
+            torch.cuda.synchronize()
+            best = min(best, e0.elapsed_time(e1) * 1e-3)
+        return best
+
+    t_qr = timed(lambda: hq.tile_qr_dev(MT, NT, nb, ib, A, TAU, T), before=lambda: A.copy_(pristine))
+    t_q = timed(lambda: hq.tile_orgqr_dev(MT, NT, nb, ib, A, TAU, Q, T=T))
+    c0 = pristine[: MT * nb * nb]   # tile column 0 of the original matrix
+    t_c = timed(lambda: hq.tile_ormqr_dev(MT, NT, nb, ib, A, TAU, C, 1, T=T), before=lambda: C.copy_(c0))
+    # Q^T A(:, 0:nb) = [R(0:nb, 0:nb); 0]: tile (0,0) of the factor, zeros below
+    qc = C.view(MT, nb, nb)            # [tile row][col][row]
+    r00 = A[: nb * nb].view(nb, nb)    # tile (0,0): [col][row]
+    upper = torch.tril(torch.ones(nb, nb, dtype=torch.bool, device=dev))   # [col][row]: row <= col
+    rmax = r00.abs().max().item()
+    top_err = ((qc[0] - r00) * upper).abs().max().item() / rmax
+    below = torch.cat([(qc[0] * ~upper).reshape(-1), qc[1:].reshape(-1)]).abs().max().item() / rmax
+    k = n
+    f_qr = 2.0 * m * n * n - 2.0 * n ** 3 / 3.0
+    f_q = 4.0 * m * n * k - 2.0 * (m + n) * k * k + 4.0 / 3.0 * k ** 3
+    f_c = 4.0 * m * nrhs * k - 2.0 * nrhs * k * k
+    out = {"workload": f"tall FP64 tile QR {m}x{n} nb={nb} ib={ib}, dorgqr Q^T array, dormqr on {m}x{nrhs}",
+           "qr_ms": round(t_qr * 1e3, 3), "qr_tflops": round(f_qr / t_qr / 1e12, 3),
+           "orgqr_ms": round(t_q * 1e3, 3), "orgqr_tflops": round(f_q / t_q / 1e12, 3),
+           "ormqr_ms": round(t_c * 1e3, 3), "ormqr_tflops_algorithmic": round(f_c / t_c / 1e12, 3),
+           "ormqr_note": f"one {nb}-column tile column is updated for the {nrhs} columns",
+           "check_QtA_top_rel": top_err, "check_QtA_below_rel": below}
+    del pristine, A, TAU, T, Q, C
+    torch.cuda.empty_cache()
+    return out
 
 
 def lsq_bench(m=32768, n=8192, kappas=(1e2, 1e6, 1e10), nb=256, ib=32):
@@ -216,6 +278,14 @@ def main():
         print(json.dumps({"error": "batched factorisation check failed", "batched": batched}))
         sys.exit(3)
 
+    tall = None
+    if not args.no_tall:
+        tall = tall_bench(nb=nb, ib=ib)
+        note(f"tall: QR {tall['qr_tflops']} TFLOP/s, dorgqr {tall['orgqr_tflops']}, dormqr {tall['ormqr_ms']} ms")
+        if not (tall["check_QtA_top_rel"] < 1e-12 and tall["check_QtA_below_rel"] < 1e-12):
+            print(json.dumps({"error": "tall factorisation check failed", "tall": tall}))
+            sys.exit(3)
+
     lsq_res = None
     if not args.no_lsq:
         try:
@@ -251,6 +321,7 @@ def main():
         "cpu_baseline": cpu,
         "check": check,
         "batched": batched,
+        "tall": tall,
         "mixed_lsq": lsq_res,
         "step_ms": [round(x, 3) for x in fact_ms],
     }
