@@ -74,6 +74,9 @@ def batched_bench(count):
     bt = B.Batch(sizes, plan, seed0=1)
     t = B.time_factor(bt, reps=3)
     err = bt.colnorm_error()
+    tg = B.time_factor(bt, reps=3, graph=bt.capture())   # the same runs replayed from a hipGraph
+    err = max(err, bt.colnorm_error())
+    bt.release()
     tp = B.time_factor(bt, reps=1, per_matrix=True)
     fl = B.flops(sizes)
     return {
@@ -82,6 +85,7 @@ def batched_bench(count):
         "qrs_per_s": round(count / t, 1),
         "ms": round(t * 1e3, 3),
         "tflops_algorithmic": round(fl / t / 1e12, 3),
+        "graph_replay_qrs_per_s": round(count / tg, 1),
         "per_matrix_launch_qrs_per_s": round(count / tp, 1),
         "speedup_vs_per_matrix_launch": round(tp / t, 2),
         "launches": len(bt.groups),

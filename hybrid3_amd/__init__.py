@@ -73,6 +73,11 @@ SIGNATURES = {
     "hq_dag_host": (C.c_int, [C.c_int] * 6 + [C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int,
                                              C.POINTER(C.c_int)]),
     "hq_batched_tile_qr_dev": (C.c_int, [C.c_int] * 3 + [C.POINTER(C.c_int)] * 2 + [C.POINTER(_vp)] * 3 + [_vp]),
+    "hq_batched_plan_dev": (C.c_int, [C.c_int] * 3 + [C.POINTER(C.c_int)] * 2 + [C.POINTER(_vp)] * 3
+                            + [C.POINTER(C.c_int)]),
+    "hq_batched_run_dev": (C.c_int, [C.c_int, _vp]),
+    "hq_batched_plan_check": (C.c_int, [C.c_int, _vp]),
+    "hq_batched_plan_free": (C.c_int, [C.c_int]),
     "hq_batched_fill_uniform_dev": (C.c_int, [C.c_int] * 2 + [C.POINTER(C.c_int)] * 2
                                     + [C.POINTER(C.c_uint64), C.POINTER(_vp), _vp]),
     "hq_cm_to_tile_dev": (C.c_int, [C.c_int] * 2 + [C.POINTER(C.c_int)] * 2
@@ -185,6 +190,28 @@ def batched_tile_qr_dev(nb, ib, MT, NT, A, TAU, T=None, stream=None):
     Tp = None if T is None else _parr(T)
     check(lib().hq_batched_tile_qr_dev(n, nb, ib, _iarr(MT), _iarr(NT), _parr(A), _parr(TAU), Tp,
                                        _stream(stream)))
+
+
+def batched_plan_dev(nb, ib, MT, NT, A, TAU, T=None):
+    """Prepare a batch for repeated (graph-capturable) runs; returns the plan id."""
+    n = len(MT)
+    Tp = None if T is None else _parr(T)
+    plan = C.c_int(0)
+    check(lib().hq_batched_plan_dev(n, nb, ib, _iarr(MT), _iarr(NT), _parr(A), _parr(TAU), Tp, C.byref(plan)))
+    return plan.value
+
+
+def batched_run_dev(plan, stream=None):
+    """Enqueue one run of a prepared batch (kernel launches only)."""
+    check(lib().hq_batched_run_dev(plan, _stream(stream)))
+
+
+def batched_plan_check(plan, stream=None):
+    check(lib().hq_batched_plan_check(plan, _stream(stream)))
+
+
+def batched_plan_free(plan):
+    check(lib().hq_batched_plan_free(plan))
 
 
 def batched_fill_uniform_dev(nb, m, n, seeds, A, stream=None):

@@ -20,7 +20,8 @@ import time
 
 import numpy as np
 
-from . import batched_fill_uniform_dev, batched_tile_qr_dev, sync, tile_qr_dev
+from . import (batched_fill_uniform_dev, batched_plan_dev, batched_plan_free, batched_run_dev,
+               batched_tile_qr_dev, sync, tile_qr_dev)
 
 CANDIDATES = ((64, 16), (64, 32), (128, 32))
 BUCKETS = ((1, 256), (256, 512), (512, 1 << 30))  # half-open ranges of m
@@ -84,6 +85,28 @@ class Batch:
             batched_tile_qr_dev(nb, ib, [self.tiles[i][0] for i in idx], [self.tiles[i][1] for i in idx],
                                 [self.a_ptr[i] for i in idx], [self.t_ptr[i] for i in idx])
 
+    def capture(self):
+        """A hipGraph of one run per (nb, ib) group of prepared plans
+        (hq_batched_plan_dev): replaying it factors the batch in place."""
+        import torch
+        self.release()
+        self.plans = []
+        for (nb, ib), idx in self.groups.items():
+            self.plans.append(batched_plan_dev(nb, ib, [self.tiles[i][0] for i in idx],
+                                               [self.tiles[i][1] for i in idx],
+                                               [self.a_ptr[i] for i in idx], [self.t_ptr[i] for i in idx]))
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            for p in self.plans:
+                batched_run_dev(p)
+        return self.graph
+
+    def release(self):
+        for p in getattr(self, "plans", []):
+            batched_plan_free(p)
+        self.plans = []
+        self.graph = None
+
     def factor_per_matrix(self, idx=None):
         """The baseline: one hq_tile_qr_dev launch per matrix."""
         for i in (range(len(self.sizes)) if idx is None else idx):
@@ -114,15 +137,19 @@ class Batch:
         return worst
 
 
-def time_factor(batch, reps=3, per_matrix=False):
-    """Best-of-reps seconds for factoring the whole batch (restored first)."""
+def time_factor(batch, reps=3, per_matrix=False, graph=None):
+    """Best-of-reps seconds for factoring the whole batch (restored first):
+    one launch per (nb, ib) group, one launch per matrix, or a replay of a
+    captured graph."""
     import torch
     best = float("inf")
     for _ in range(reps):
         batch.restore()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        if per_matrix:
+        if graph is not None:
+            graph.replay()
+        elif per_matrix:
             batch.factor_per_matrix()
         else:
             batch.factor()

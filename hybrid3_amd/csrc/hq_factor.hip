@@ -771,9 +771,10 @@ __device__ __forceinline__ void panel_trailing(Smem<NB> &S, double *X, double *A
 
 // GEQRT = CORE_dgeqrt(nb, nb, ib, A, nb, T, ib, TAU)
 template <int NB, int WB>
-__device__ __forceinline__ void task_geqrt(Smem<NB> &S, double *A, double *T, double *TAU, int ib) {
+__device__ __forceinline__ void task_geqrt(Smem<NB> &S, double *A, double *T, double *TAU, int ib,
+                                           double *Wt = nullptr, double *Vd = nullptr) {
   for (int ii = 0; ii < NB; ii += ib) {
-    geqrt_panel<NB, WB>(S, A, T, TAU, ib, ii);
+    geqrt_panel<NB, WB>(S, A, T, TAU, ib, ii, Wt, Vd);
     panel_trailing<NB, WB, false>(S, A, nullptr, ib, ii, ii + min(ib, NB - ii), NB);
     __syncthreads();
   }
@@ -1696,13 +1697,21 @@ __device__ __forceinline__ double *tau_ptr(const hq_geom &G, int i, int j) {
 template <int NB, int WB>
 __device__ __forceinline__ void run_task(Smem<NB> &S, const hq_geom &G, const hq_task &t) {
   switch (t.kind) {
-    case HQ_GEQRT:
-      geqrt_panel<NB, WB>(S, tile_ptr(G, t.k, t.k), t_ptr(G, t.k, t.k), tau_ptr(G, t.k, t.k), G.ib, t.j * G.ib,
-                          wd_ptr(G, t.k), vd_ptr(G, t.k));
+    case HQ_GEQRT:  // j = ib block of the panel, or -1: the whole tile
+      if (t.j < 0)
+        task_geqrt<NB, WB>(S, tile_ptr(G, t.k, t.k), t_ptr(G, t.k, t.k), tau_ptr(G, t.k, t.k), G.ib,
+                           wd_ptr(G, t.k), vd_ptr(G, t.k));
+      else
+        geqrt_panel<NB, WB>(S, tile_ptr(G, t.k, t.k), t_ptr(G, t.k, t.k), tau_ptr(G, t.k, t.k), G.ib,
+                            t.j * G.ib, wd_ptr(G, t.k), vd_ptr(G, t.k));
       break;
     case HQ_TSQRT:
-      tsqrt_panel<NB, WB>(S, tile_ptr(G, t.k, t.k), tile_ptr(G, t.r, t.k), t_ptr(G, t.r, t.k),
-                          tau_ptr(G, t.r, t.k), G.ib, t.j * G.ib);
+      if (t.j < 0)
+        task_tsqrt<NB, WB>(S, tile_ptr(G, t.k, t.k), tile_ptr(G, t.r, t.k), t_ptr(G, t.r, t.k),
+                           tau_ptr(G, t.r, t.k), G.ib);
+      else
+        tsqrt_panel<NB, WB>(S, tile_ptr(G, t.k, t.k), tile_ptr(G, t.r, t.k), t_ptr(G, t.r, t.k),
+                            tau_ptr(G, t.r, t.k), G.ib, t.j * G.ib);
       break;
     case HQ_GEQRT_U:
       task_geqrt_update<NB, WB>(S, tile_ptr(G, t.k, t.k), t_ptr(G, t.k, t.k), G.ib, t.j, t.slab);
@@ -1734,9 +1743,14 @@ __device__ __forceinline__ void run_task(Smem<NB> &S, const hq_geom &G, const hq
 template <int NB, int WB>
 __global__ __launch_bounds__(HQ_THREADS) void sched_kernel(hq_geom G, const hq_geom *__restrict__ geoms,
                                                           const hq_task *__restrict__ tasks,
-                                                          int ntasks, int *flags, int epoch,
-                                                          int *counter, long long *trace) {
+                                                          int ntasks, int *flags, int epoch_arg,
+                                                          int *counter, long long *trace,
+                                                          const int *epoch_dev) {
   __shared__ Smem<NB> S;
+  // this run's flag value: the argument, or the device epoch advanced by the
+  // counter reset just before (graph replays)
+  const int epoch = epoch_dev ? __hip_atomic_load(epoch_dev, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                              : epoch_arg;
   int done = -1;  // task finished by this workgroup in the previous iteration
   long long t_claim = 0;
   for (;;) {
@@ -1909,8 +1923,14 @@ __global__ void zero_reflectors_kernel(double *A, int MT, int NT, int nb) {
   }
 }
 
-__global__ void reset_counter_kernel(int *counter) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
+// Claim counter reset before a scheduler launch; with epoch_dev (launches
+// that may be replayed from a captured graph) the run's epoch is advanced on
+// the device too, since a captured kernel argument would stay fixed.
+__global__ void reset_counter_kernel(int *counter, int *epoch_dev) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    *counter = 0;
+    if (epoch_dev) *epoch_dev = *epoch_dev + 1;
+  }
 }
 
 // SplitMix64 of (seed, global column-major index) -> uniform [-1, 1), written
@@ -1994,12 +2014,13 @@ static hipError_t hq_dispatch(int nb, int ib, Fn &&fn) {
   constexpr int WB = decltype(wbc)::value;
 
 hipError_t hq_launch_sched(const hq_geom &G, const hq_geom *geoms, const hq_task *tasks, int ntasks,
-                           int *flags, int epoch, int *counter, int grid, long long *trace, hipStream_t st) {
-  hipLaunchKernelGGL(hq::reset_counter_kernel, dim3(1), dim3(64), 0, st, counter);
+                           int *flags, int epoch, int *counter, int grid, long long *trace, hipStream_t st,
+                           int *epoch_dev) {
+  hipLaunchKernelGGL(hq::reset_counter_kernel, dim3(1), dim3(64), 0, st, counter, epoch_dev);
   return hq_dispatch(G.nb, G.ib, [&](auto nbc, auto wbc) {
     HQ_NBWB
     hipLaunchKernelGGL((hq::sched_kernel<NB, WB>), dim3(grid), dim3(HQ_THREADS), 0, st, G, geoms, tasks,
-                       ntasks, flags, epoch, counter, trace);
+                       ntasks, flags, epoch, counter, trace, (const int *)epoch_dev);
   });
 }
 

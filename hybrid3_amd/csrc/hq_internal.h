@@ -8,10 +8,11 @@
 // Task kinds of the flat-tree tile QR DAG (Hybrid_tile_QR.c:960,1123,1254,1332)
 // plus the Q-application kinds used by dorgqr / dormqr.
 // GEQRT / TSQRT run as one panel task per ib block (field j = block) and
-// one trailing task per block and column slab (j = block, slab).
+// one trailing task per block and column slab (j = block, slab), or, in
+// batches of many matrices, as one task per tile (j = -1).
 enum hq_task_kind : int32_t {
-  HQ_GEQRT = 0,    // QR of ib block j of diagonal tile (k,k)
-  HQ_TSQRT = 1,    // QR of ib block j of [R(k,k); A(r,k)]
+  HQ_GEQRT = 0,    // QR of ib block j (all blocks: j = -1) of diagonal tile (k,k)
+  HQ_TSQRT = 1,    // QR of ib block j (all blocks: j = -1) of [R(k,k); A(r,k)]
   HQ_LARFB = 2,    // A(k,j) <- Q(k,k)^T A(k,j)            (one column slab)
   HQ_SSRFB = 3,    // [A(k,j); A(r,j)] <- Q(r,k)^T [...]   (one column slab)
   HQ_GEQRT_U = 4,  // block j of GEQRT(k) applied to slab `slab` of tile (k,k)

@@ -21,7 +21,7 @@
 
 // launch wrappers (hq_factor.hip)
 hipError_t hq_launch_sched(const hq_geom &G, const hq_geom *geoms, const hq_task *tasks, int ntasks,
-                           int *flags, int epoch, int *counter, int grid, long long *trace, hipStream_t st);
+                           int *flags, int epoch, int *counter, int grid, long long *trace, hipStream_t st, int *epoch_dev = nullptr);
 hipError_t hq_launch_task(const hq_geom &G, const hq_task &tk, hipStream_t st);
 hipError_t hq_launch_geqrt(int nb, int ib, double *A, double *T, double *TAU, hipStream_t st);
 hipError_t hq_launch_tsqrt(int nb, int ib, double *A1, double *A2, double *T, double *TAU,
@@ -115,8 +115,9 @@ struct DagBuilder {
   // panel tasks of the current step k: (row r, block b); their trailing
   // tasks: (row r, block b, slab s).  Row k is the GEQRT of the step.
   std::vector<int> idp, idu;
-  DagBuilder(int MT_, int NT_, int nb_, int ib_, bool apply_ = false, int XNT_ = 0)
-      : MT(MT_), NT(NT_), nb(nb_), ib(ib_), apply(apply_), XNT(apply_ ? XNT_ : NT_) {
+  bool split;  // GEQRT / TSQRT as panel + trailing tasks (else one task per tile)
+  DagBuilder(int MT_, int NT_, int nb_, int ib_, bool apply_ = false, int XNT_ = 0, bool split_ = true)
+      : MT(MT_), NT(NT_), nb(nb_), ib(ib_), apply(apply_), XNT(apply_ ? XNT_ : NT_), split(split_) {
     S = (nb + HQ_SLAB - 1) / HQ_SLAB;
     KT = std::min(MT, NT);
     NBLK = (nb + ib - 1) / ib;
@@ -180,6 +181,14 @@ struct DagBuilder {
   void panel_tasks(int k, int r) {
     const int kind = r == k ? HQ_GEQRT : HQ_TSQRT;
     const int ukind = r == k ? HQ_GEQRT_U : HQ_TSQRT_U;
+    if (!split) {  // the whole panel QR of the tile in one task (j = -1)
+      const int p = add(kind, k, r, -1, 0);
+      if (r > k) dep(p, P(r - 1, 0));
+      if (k > 0)
+        for (int s = 0; s < S; s++) dep(p, Sr(k - 1, k, r, s));
+      for (int b = 0; b < NBLK; b++) P(r, b) = p;
+      return;
+    }
     for (int b = 0; b < NBLK; b++) {
       const int ii = b * ib, j0 = std::min(nb, ii + ib);
       const int p = P(r, b) = add(kind, k, r, b, 0);
@@ -231,6 +240,7 @@ struct DagBuilder {
     const double f = nb / 256.0;
     const double w = (double)std::min(HQ_SLAB, nb) / 64.0;  // slab width / 64
     const double c = std::min(ib, nb - x.j * ib) / 32.0;     // panel block width / 32
+    if ((x.kind == HQ_GEQRT || x.kind == HQ_TSQRT) && x.j < 0) return 540.0 * f * f * f;  // whole tile
     switch (x.kind) {
       case HQ_GEQRT: return 36.0 * f * c;
       case HQ_TSQRT: return 38.0 * f * c;
@@ -326,6 +336,8 @@ struct Context {
   // batches: key = (nb, ib, MT_0, NT_0, MT_1, NT_1, ...)
   std::map<std::vector<int>, std::unique_ptr<CachedSched>> batch_scheds;
   DevBuf geoms, bmats, Tbatch, Wbatch, VDbatch;
+  std::map<int, std::unique_ptr<struct BatchPlan>> plans;  // prepared batches
+  int next_plan = 1;
 };
 
 std::mutex g_mu;
@@ -536,7 +548,9 @@ int get_batch_schedule(Context *c, int count, int nb, int ib, const int *MT, con
   for (int i = 0; i < count; i++) {
     auto &dag = per_geom[std::make_pair(MT[i], NT[i])];
     if (dag.empty()) {
-      DagBuilder b(MT[i], NT[i], nb, ib);
+      // many independent matrices keep the machine busy without pipelining
+      // each tile column's panel chain: one task per panel QR
+      DagBuilder b(MT[i], NT[i], nb, ib, false, 0, false);
       b.build();
       dag = b.t;
     }
@@ -561,50 +575,61 @@ int get_batch_schedule(Context *c, int count, int nb, int ib, const int *MT, con
   return HQ_OK;
 }
 
-int batched_qr_impl(int count, int nb, int ib, const int *MT, const int *NT, double *const *A,
-                    double *const *TAU, double *const *T, void *stream) {
+int batch_args_ok(int count, int nb, int ib, const int *MT, const int *NT, double *const *A,
+                  double *const *TAU) {
   if (count < 1 || count > 65535) return set_error(HQ_ERR_ARG, "batch of %d matrices (1..65535)", count);
   if (!MT || !NT || !A || !TAU) return set_error(HQ_ERR_ARG, "MT, NT, A and TAU arrays are required");
   for (int i = 0; i < count; i++) {
     if (!geometry_ok(MT[i], NT[i], nb, ib)) return g_err;
     if (!A[i] || !TAU[i]) return set_error(HQ_ERR_ARG, "matrix %d: A and TAU must be device pointers", i);
   }
-  Context *c;
-  if (get_ctx(&c) != HQ_OK) return g_err;
-  hipStream_t st = pick_stream(c, stream);
-  // T scratch for the matrices that bring none
+  return HQ_OK;
+}
+
+// Per-matrix geometries of a batch.  T scratch for the matrices that bring
+// none, and the diagonal tiles' W = V T16 / unit lower V (as for one matrix),
+// are carved from Tb / Wb / VDb, reserved here.
+int batch_geoms(int count, int nb, int ib, const int *MT, const int *NT, double *const *A,
+                double *const *TAU, double *const *T, DevBuf &Tb, DevBuf &Wb, DevBuf &VDb,
+                std::vector<hq_geom> &g) {
   size_t tdoubles = 0;
   for (int i = 0; i < count; i++)
     if (!T || !T[i]) tdoubles += (size_t)MT[i] * NT[i] * ib * nb;
-  if (tdoubles && c->Tbatch.reserve(sizeof(double) * tdoubles) != HQ_OK) return g_err;
-  // W = V T16 and unit lower V of the diagonal tiles, as for one matrix
+  if (tdoubles && Tb.reserve(sizeof(double) * tdoubles) != HQ_OK) return g_err;
   const bool wpath = nb >= 128 && ib % 16 == 0;
-  size_t wdoubles = 0, vddoubles = 0;
+  size_t wdoubles = 0;
   if (wpath) {
-    for (int i = 0; i < count; i++) {
-      wdoubles += (size_t)std::min(MT[i], NT[i]) * nb * nb;
-      vddoubles += (size_t)std::min(MT[i], NT[i]) * nb * nb;
-    }
-    if (c->Wbatch.reserve(sizeof(double) * wdoubles) != HQ_OK ||
-        c->VDbatch.reserve(sizeof(double) * vddoubles) != HQ_OK)
+    for (int i = 0; i < count; i++) wdoubles += (size_t)std::min(MT[i], NT[i]) * nb * nb;
+    if (Wb.reserve(sizeof(double) * wdoubles) != HQ_OK || VDb.reserve(sizeof(double) * wdoubles) != HQ_OK)
       return g_err;
   }
-  std::vector<hq_geom> g(count);
-  size_t toff = 0, woff = 0, vdoff = 0;
+  g.resize(count);
+  size_t toff = 0, woff = 0;
   for (int i = 0; i < count; i++) {
     double *Ti = T ? T[i] : nullptr;
     if (!Ti) {
-      Ti = (double *)c->Tbatch.p + toff;
+      Ti = (double *)Tb.p + toff;
       toff += (size_t)MT[i] * NT[i] * ib * nb;
     }
     g[i] = hq_geom{A[i], TAU[i], Ti, MT[i], NT[i], nb, ib, A[i], NT[i]};
     if (wpath) {
-      g[i].W = (double *)c->Wbatch.p + woff;
-      g[i].VD = (double *)c->VDbatch.p + vdoff;
+      g[i].W = (double *)Wb.p + woff;
+      g[i].VD = (double *)VDb.p + woff;
       woff += (size_t)std::min(MT[i], NT[i]) * nb * nb;
-      vdoff += (size_t)std::min(MT[i], NT[i]) * nb * nb;
     }
   }
+  return HQ_OK;
+}
+
+int batched_qr_impl(int count, int nb, int ib, const int *MT, const int *NT, double *const *A,
+                    double *const *TAU, double *const *T, void *stream) {
+  if (batch_args_ok(count, nb, ib, MT, NT, A, TAU) != HQ_OK) return g_err;
+  Context *c;
+  if (get_ctx(&c) != HQ_OK) return g_err;
+  hipStream_t st = pick_stream(c, stream);
+  std::vector<hq_geom> g;
+  if (batch_geoms(count, nb, ib, MT, NT, A, TAU, T, c->Tbatch, c->Wbatch, c->VDbatch, g) != HQ_OK)
+    return g_err;
   CachedSched *cs;
   if (get_batch_schedule(c, count, nb, ib, MT, NT, &cs) != HQ_OK) return g_err;
   if (c->geoms.reserve(sizeof(hq_geom) * (size_t)count) != HQ_OK) return g_err;
@@ -613,6 +638,48 @@ int batched_qr_impl(int count, int nb, int ib, const int *MT, const int *NT, dou
   HQ_HIP(hq_launch_sched(g[0], (const hq_geom *)c->geoms.p, (const hq_task *)cs->tasks.p, cs->ntasks,
                          (int *)c->flags.p, c->epoch, (int *)c->counter.p, cs->grid, nullptr, st));
   return ok();
+}
+
+// A batch prepared once and launched many times (hq_batched_plan_dev /
+// hq_batched_run_dev): everything a run reads lives in device memory owned by
+// the plan, and a run enqueues two kernels and nothing else, so it can be
+// captured in a graph.  Counter block: [0] claim counter, [1] time-out
+// report, [2] device epoch (advanced by every run; the plan's own flags
+// start at 0).
+struct BatchPlan {
+  CachedSched *cs = nullptr;  // task list, owned by the context's schedule cache
+  hq_geom g0{};
+  DevBuf geoms, flags, counter, T, W, VD;
+};
+
+int batched_plan_impl(int count, int nb, int ib, const int *MT, const int *NT, double *const *A,
+                      double *const *TAU, double *const *T, int *plan) {
+  if (!plan) return set_error(HQ_ERR_ARG, "plan must not be NULL");
+  if (batch_args_ok(count, nb, ib, MT, NT, A, TAU) != HQ_OK) return g_err;
+  Context *c;
+  if (get_ctx(&c) != HQ_OK) return g_err;
+  std::unique_ptr<BatchPlan> p(new BatchPlan());
+  std::vector<hq_geom> g;
+  if (batch_geoms(count, nb, ib, MT, NT, A, TAU, T, p->T, p->W, p->VD, g) != HQ_OK) return g_err;
+  if (get_batch_schedule(c, count, nb, ib, MT, NT, &p->cs) != HQ_OK) return g_err;
+  if (p->geoms.reserve(sizeof(hq_geom) * (size_t)count) != HQ_OK ||
+      p->flags.reserve(sizeof(int) * (size_t)p->cs->ntasks) != HQ_OK || p->counter.reserve(64) != HQ_OK)
+    return g_err;
+  HQ_HIP(hipMemcpy(p->geoms.p, g.data(), sizeof(hq_geom) * (size_t)count, hipMemcpyHostToDevice));
+  HQ_HIP(hipMemset(p->flags.p, 0, sizeof(int) * (size_t)p->cs->ntasks));
+  HQ_HIP(hipMemset(p->counter.p, 0, 64));
+  p->g0 = g[0];
+  std::lock_guard<std::mutex> lk(g_mu);
+  const int id = c->next_plan++;
+  c->plans[id] = std::move(p);
+  *plan = id;
+  return ok();
+}
+
+BatchPlan *find_plan(Context *c, int plan) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = c->plans.find(plan);
+  return it == c->plans.end() ? nullptr : it->second.get();
 }
 
 // Layout operations on a batch (op 0: fill uniform, 1: column-major -> tile,
@@ -850,6 +917,46 @@ int hq_tile_make_q_dev(int MT, int NT, int nb, int ib, double *RY, const double 
 int hq_batched_tile_qr_dev(int count, int nb, int ib, const int *MT, const int *NT, double *const *A,
                            double *const *TAU, double *const *T, void *stream) {
   return batched_qr_impl(count, nb, ib, MT, NT, A, TAU, T, stream);
+}
+
+int hq_batched_plan_dev(int count, int nb, int ib, const int *MT, const int *NT, double *const *A,
+                        double *const *TAU, double *const *T, int *plan) {
+  return batched_plan_impl(count, nb, ib, MT, NT, A, TAU, T, plan);
+}
+
+int hq_batched_run_dev(int plan, void *stream) {
+  Context *c;
+  if (get_ctx(&c) != HQ_OK) return g_err;
+  BatchPlan *p = find_plan(c, plan);
+  if (!p) return set_error(HQ_ERR_ARG, "no batch plan %d", plan);
+  int *cnt = (int *)p->counter.p;
+  HQ_HIP(hq_launch_sched(p->g0, (const hq_geom *)p->geoms.p, (const hq_task *)p->cs->tasks.p, p->cs->ntasks,
+                         (int *)p->flags.p, 0, cnt, p->cs->grid, nullptr, pick_stream(c, stream), cnt + 2));
+  return ok();
+}
+
+int hq_batched_plan_check(int plan, void *stream) {
+  Context *c;
+  if (get_ctx(&c) != HQ_OK) return g_err;
+  BatchPlan *p = find_plan(c, plan);
+  if (!p) return set_error(HQ_ERR_ARG, "no batch plan %d", plan);
+  HQ_HIP(hipStreamSynchronize(pick_stream(c, stream)));
+  int st[2] = {0, 0};
+  HQ_HIP(hipMemcpy(st, p->counter.p, sizeof st, hipMemcpyDeviceToHost));
+  if (st[1] != 0) {
+    HQ_HIP(hipMemset((int *)p->counter.p + 1, 0, sizeof(int)));
+    return set_error(HQ_ERR_DEVICE, "scheduler dependency wait timed out at task %d", st[1] - 1);
+  }
+  return ok();
+}
+
+int hq_batched_plan_free(int plan) {
+  Context *c;
+  if (get_ctx(&c) != HQ_OK) return g_err;
+  HQ_HIP(hipDeviceSynchronize());
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!c->plans.erase(plan)) return set_error(HQ_ERR_ARG, "no batch plan %d", plan);
+  return ok();
 }
 
 int hq_batched_fill_uniform_dev(int count, int nb, const int *m, const int *n, const uint64_t *seeds,

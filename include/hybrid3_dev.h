@@ -146,6 +146,25 @@ int hq_batched_tile_qr_dev(int count, int nb, int ib, const int *MT, const int *
                            double *const *A, double *const *TAU, double *const *T,
                            void *stream);
 
+/* The same factorisation as a plan prepared once and run many times (no
+ * counterpart in the reference; for callers that capture the launch in a
+ * hipGraph and replay it, e.g. a serving loop over fixed buffers).
+ * hq_batched_plan_dev copies the task list and the per-matrix geometry to the
+ * device and allocates the plan's scratch, flags and counters; it records the
+ * pointers, which must stay valid while the plan is used.
+ * hq_batched_run_dev enqueues a counter reset (which advances the plan's
+ * device-side epoch) and the persistent scheduler on `stream`, allocates
+ * nothing and copies nothing, so it may be captured and replayed; each run
+ * factors the matrices in place at the recorded pointers.  Runs of one plan
+ * must not overlap.  hq_batched_plan_check synchronises `stream` and reports
+ * a scheduler time-out of the plan; hq_batched_plan_free releases it. */
+int hq_batched_plan_dev(int count, int nb, int ib, const int *MT, const int *NT,
+                        double *const *A, double *const *TAU, double *const *T,
+                        int *plan);
+int hq_batched_run_dev(int plan, void *stream);
+int hq_batched_plan_check(int plan, void *stream);
+int hq_batched_plan_free(int plan);
+
 /* A[i] (tile layout, padding zero) <- m[i] x n[i] uniform[-1,1) values:
  * entry (row, col) = SplitMix64 of (seeds[i], col*m[i] + row), the values of
  * a column-major fill of the unpadded matrix. */

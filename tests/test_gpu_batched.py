@@ -152,3 +152,51 @@ def test_autotuner_plan_and_batch(hq):
     bt = B.Batch(sizes, plan, seed0=3)
     B.time_factor(bt, reps=1)
     assert bt.colnorm_error() < 1e-12
+
+
+def test_plan_runs_and_graph_replays(hq):
+    """A batch prepared once (hq_batched_plan_dev) is run eagerly, then from a
+    captured graph replayed three times with other eager batched work and
+    synchronisations between the replays: every run gives the bits of the
+    one-launch hq_batched_tile_qr_dev call (same kernels, same per-tile update
+    order)."""
+    import torch
+    nb, ib = 128, 32
+    print("batched plan + graph replays", flush=True)
+    tiles, A = make_batch(hq, SIZES, nb, 950)
+    MTs, NTs = [t[0] for t in tiles], [t[1] for t in tiles]
+    A0 = [a.clone() for a in A]
+    TAU = [torch.zeros(MT * NT * nb, dtype=torch.float64, device="cuda") for MT, NT in tiles]
+    Aref = [a.clone() for a in A0]
+    TAUref = [torch.zeros_like(t) for t in TAU]
+    hq.batched_tile_qr_dev(nb, ib, MTs, NTs, Aref, TAUref)
+    hq.sync()
+    # other matrices factored eagerly between the replays
+    otiles, other = make_batch(hq, SIZES[:4], nb, 960)
+    other0 = [a.clone() for a in other]
+    otau = [torch.zeros(MT * NT * nb, dtype=torch.float64, device="cuda") for MT, NT in otiles]
+    plan = hq.batched_plan_dev(nb, ib, MTs, NTs, A, TAU)
+    try:
+        hq.batched_run_dev(plan)
+        hq.batched_plan_check(plan)
+        assert all(torch.equal(a, r) for a, r in zip(A, Aref))
+        assert all(torch.equal(t, r) for t, r in zip(TAU, TAUref))
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            hq.batched_run_dev(plan)
+        for rep in range(3):
+            for a, a0 in zip(A, A0):
+                a.copy_(a0)
+            for t in TAU:
+                t.zero_()
+            for a, a0 in zip(other, other0):
+                a.copy_(a0)
+            hq.batched_tile_qr_dev(nb, ib, [t[0] for t in otiles], [t[1] for t in otiles], other, otau)
+            hq.sync()
+            g.replay()
+            hq.batched_plan_check(plan)
+            torch.cuda.synchronize()
+            assert all(torch.equal(a, r) for a, r in zip(A, Aref)), rep
+            assert all(torch.equal(t, r) for t, r in zip(TAU, TAUref)), rep
+    finally:
+        hq.batched_plan_free(plan)
