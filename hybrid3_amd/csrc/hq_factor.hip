@@ -162,6 +162,15 @@ struct Smem {
   int nref;  // reflectors published in the current panel factorisation
 };
 
+// Global-address-space views of tile pointers.  Pointers read from the
+// batch's geometry table are generic to the compiler, which then emits flat
+// accesses; flat loads also count in the LDS counter, so every LDS operand
+// wait would wait for them too.
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gmem(T *p) {
+  return (__attribute__((address_space(1))) T *)p;
+}
+
 // --------------------------------------------------------------------------
 // accumulator-layout strip load/store: lane (g = lane/16, n = lane%16) holds
 // rows 16*rb + 4*g + {0..3} of column `col`.
@@ -173,7 +182,7 @@ __device__ __forceinline__ void load_strip(d4 (&acc)[NB / 16], const double *til
 #pragma unroll
   for (int rb = 0; rb < NB / 16; rb++) {
     if (ok && rb >= rb0)
-      acc[rb] = *reinterpret_cast<const d4 *>(tile + (size_t)col * NB + 16 * rb + 4 * g);
+      acc[rb] = *gmem(reinterpret_cast<const d4 *>(tile + (size_t)col * NB + 16 * rb + 4 * g));
     else
       acc[rb] = d4{0.0, 0.0, 0.0, 0.0};
   }
@@ -186,7 +195,7 @@ __device__ __forceinline__ void store_strip(const d4 (&acc)[NB / 16], double *ti
   if (!ok) return;
 #pragma unroll
   for (int rb = 0; rb < NB / 16; rb++)
-    if (rb >= rb0) *reinterpret_cast<d4 *>(tile + (size_t)col * NB + 16 * rb + 4 * g) = acc[rb];
+    if (rb >= rb0) *gmem(reinterpret_cast<d4 *>(tile + (size_t)col * NB + 16 * rb + 4 * g)) = acc[rb];
 }
 
 // --------------------------------------------------------------------------
@@ -991,6 +1000,10 @@ __device__ __forceinline__ double *fvbuf(Smem<NB> &S, int i) {
   return (i < 2 ? S.V : S.V1) + (i & 1) * 16 * LDV;
 }
 constexpr int LDT16 = 17;
+// s_waitcnt immediate: vector memory count <= vm, LDS / scalar count 0, no
+// export wait (gfx9 encoding: vm[3:0] at 3:0, vm[5:4] at 15:14, exp 6:4,
+// lgkm 11:8)
+constexpr int waitcnt_vm_lgkm0(int vm) { return (vm & 15) | ((vm >> 4) << 14) | (7 << 4); }
 template <int NB>
 __device__ __forceinline__ double *ftbuf(Smem<NB> &S, int i) {
   return (i < 2 ? S.TT : S.TT1) + (i & 1) * 16 * LDT16;
@@ -1059,9 +1072,10 @@ struct PassArgs {
 // operands of the next one, the last the T^T operands (tto) when DO_A.
 // Branch-free: LARFB runs the MFMAs of rows above its diagonal block too,
 // on the zeros its staging put there.
-template <int NB, bool UNIT, bool DO_U, bool DO_A, bool DO_DMA>
+template <int NB, bool UNIT, bool DO_U, bool DO_A, bool DO_DMA, bool DO_ST, class Hook>
 __device__ __forceinline__ void fpass(d4 (&acc)[NB / 16], const d4 &W2, d4 (&Wn)[2],
-                                      double (&o)[2][8], double (&tto)[4], const PassArgs<NB, UNIT> &P) {
+                                      double (&o)[2][8], double (&tto)[4], const PassArgs<NB, UNIT> &P,
+                                      double *xcol, Hook &&hook) {
   constexpr int NG = NB / 32;
   constexpr int NP = (DO_U ? NG : 0) + (DO_A ? NG : 0);
   static_assert(!DO_DMA || NP >= FDma<NB>::NOPS, "one direct-to-LDS op per phase");
@@ -1115,6 +1129,20 @@ __device__ __forceinline__ void fpass(d4 (&acc)[NB / 16], const d4 &W2, d4 (&Wn)
     __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
     __builtin_amdgcn_sched_group_barrier(0x008, 6, 0);
     __builtin_amdgcn_sched_barrier(0);
+    // work of the pass that no MFMA of it waits for, behind the first phase
+    if (p == 0) hook();
+    // last pass: the rows of the previous phase are final; their stores go
+    // out behind this phase's MFMAs instead of all at the end of the task
+    if (DO_ST && p > 0) {
+      const int Gp = grp(p - 1);
+#pragma unroll
+      for (int i = 0; i < 2; i++) *gmem(reinterpret_cast<d4 *>(xcol + 16 * (2 * Gp + i) + 4 * g)) = acc[2 * Gp + i];
+    }
+  }
+  if (DO_ST) {
+    const int Gp = grp(NP - 1);
+#pragma unroll
+    for (int i = 0; i < 2; i++) *gmem(reinterpret_cast<d4 *>(xcol + 16 * (2 * Gp + i) + 4 * g)) = acc[2 * Gp + i];
   }
 }
 
@@ -1136,7 +1164,7 @@ __device__ __forceinline__ void zero_pad(double *Vb, const SubBlk &sb) {
 __device__ __forceinline__ double tt_raw(const double *T, int ib, const SubBlk &sb) {
   const int c = threadIdx.x >> 4, p = threadIdx.x & 15;
   const int cc = min(c, sb.kb - 1);
-  return T[(size_t)(sb.i0 + cc) * ib + sb.p0 + min(p, cc)];
+  return *gmem(T + (size_t)(sb.i0 + cc) * ib + sb.p0 + min(p, cc));
 }
 __device__ __forceinline__ double tt_mask(const SubBlk &sb) {
   const int c = threadIdx.x >> 4, p = threadIdx.x & 15;
@@ -1179,10 +1207,16 @@ __device__ __forceinline__ void task_update_f(Smem<NB> &S, const double *V, cons
 #pragma unroll
       for (int q = 0; q < FDma<NB>::NOPS; q++) dma_op<NB>(q, fvbuf(S, j), V, sj);
       if (sj.kb < 16) zero_pad<NB>(fvbuf(S, j), sj);
-      ftbuf(S, j)[(tid >> 4) * LDT16 + (tid & 15)] = tt_raw(T, ib, sj) * tt_mask(sj);
+      ftbuf(S, j)[(tid >> 4) * LDT16 + (tid & 15)] = tt_raw(T, ib, sj) * -tt_mask(sj);
     }
+    // the strip's loads are issued last and stay in flight: the wait covers
+    // the direct-to-LDS loads and the LDS stores above (vector memory loads
+    // complete in issue order), and the first pass waits for each row group
+    // of the strip where it reads it
+    __builtin_amdgcn_sched_barrier(0);
     load_strip<NB>(acc, X, col, ok, 0);
-    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_s_waitcnt(waitcnt_vm_lgkm0(2 * (NB / 16)));
     __syncthreads();
     if (UNIT) {
       fix_unit<NB>(fvbuf(S, 0), w0);
@@ -1194,13 +1228,6 @@ __device__ __forceinline__ void task_update_f(Smem<NB> &S, const double *V, cons
   auto pre = [&](int s, PassArgs<NB, UNIT> &P, SubBlk &sw) {
     const SubBlk su = w0;
     sw = w1;
-    treg = tt_raw(T, ib, w2);
-    if (!UNIT) {
-      // rows past a ragged sub-block load a clamped neighbour: their W rows
-      // meet only zero rows and columns of T and are never stored
-#pragma unroll
-      for (int v = 0; v < 4; v++) a1v[v] = a1col[min(sw.i0 + 4 * g + v, NB - 1)];
-    }
     P.sd = w3;
     P.Vd = fvbuf(S, (s + 3) & 3);
     P.Vg = V;
@@ -1212,36 +1239,48 @@ __device__ __forceinline__ void task_update_f(Smem<NB> &S, const double *V, cons
     P.TTw = ftbuf(S, (s + 1) & 3);
     P.i0u = su.i0; P.kbu = su.kb; P.i0w = sw.i0; P.kbw = sw.kb;
   };
-  // after pass s: W2_{s+1}, its A1 rows (stored once the barrier is passed:
-  // the wait before the barrier then does not cover them), T^T of s+2 into
+  // In the first phase of pass s, behind its MFMAs: (SSRFB) the A1 rows of
+  // the update sub-block, a1 + W2 with W2 = -T^T W from the end of the
+  // previous pass, then the A1 rows of the accumulate sub-block; T of s+2.
+  auto hook = [&](bool upd, bool acc_next, const PassArgs<NB, UNIT> &P, const SubBlk &sw) {
+    return [&, upd, acc_next]() {
+      if (!UNIT && upd) {
+        a1o = a1v + W2;
+        if (P.kbu == 16 && (P.i0u & 3) == 0) {
+          *gmem(reinterpret_cast<d4 *>(a1col + P.i0u + 4 * g)) = a1o;
+        } else {
+#pragma unroll
+          for (int v = 0; v < 4; v++)
+            if (4 * g + v < P.kbu) *gmem(a1col + P.i0u + 4 * g + v) = a1o[v];
+        }
+      }
+      if (!UNIT && acc_next) {
+        // rows past a ragged sub-block load a clamped neighbour: their W rows
+        // meet only zero rows and columns of T and are never stored
+#pragma unroll
+        for (int v = 0; v < 4; v++) a1v[v] = *gmem(a1col + min(sw.i0 + 4 * g + v, NB - 1));
+      }
+      if (acc_next) treg = tt_raw(T, ib, w2);
+    };
+  };
+  // after pass s: W2_{s+1} = -T^T W (the T ring holds -T^T), T^T of s+2 into
   // the ring
   auto post = [&](int s, const PassArgs<NB, UNIT> &P, const SubBlk &sw) {
     const d4 W = UNIT ? Wn[0] + Wn[1] : (Wn[0] + Wn[1]) + a1v;
     d4 W2n = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int sp = 0; sp < 4; sp++) W2n = mfma(tto[sp], W[sp], W2n);
-    a1o = a1v - W2n;
-    W2 = -W2n;
+    W2 = W2n;
     ld_u<NB, UNIT>(o[0], P.Vw, 0, sw.i0, sw.kb);
-    ftbuf(S, (s + 2) & 3)[(tid >> 4) * LDT16 + (tid & 15)] = treg * tt_mask(w2);
+    ftbuf(S, (s + 2) & 3)[(tid >> 4) * LDT16 + (tid & 15)] = treg * -tt_mask(w2);
     // Every load of the pass has landed, sub-block s+3 included (issued in
-    // the first half of the pass).  A counted wait (vmcnt(NOPS), relying on
-    // in-order completion) let results at n = 16384 go wrong: direct-to-LDS
-    // loads are not taken to complete in order here.
+    // the first half of the pass); direct-to-LDS loads are not taken to
+    // complete in order with the others, so no counted wait here.
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (UNIT && s + 2 < nsub) {
       fix_unit<NB>(fvbuf(S, (s + 2) & 3), w2);
       __syncthreads();
-    }
-    if (!UNIT) {
-      if (sw.kb == 16 && (sw.i0 & 3) == 0) {
-        *reinterpret_cast<d4 *>(a1col + sw.i0 + 4 * g) = a1o;
-      } else {
-#pragma unroll
-        for (int v = 0; v < 4; v++)
-          if (4 * g + v < sw.kb) a1col[sw.i0 + 4 * g + v] = a1o[v];
-      }
     }
     // advance the sub-block window
     w0 = w1;
@@ -1253,25 +1292,26 @@ __device__ __forceinline__ void task_update_f(Smem<NB> &S, const double *V, cons
     PassArgs<NB, UNIT> P;
     SubBlk sw;
     pre(-1, P, sw);
-    fpass<NB, UNIT, false, true, true>(acc, W2, Wn, o, tto, P);
+    fpass<NB, UNIT, false, true, true, false>(acc, W2, Wn, o, tto, P, nullptr, hook(false, true, P, sw));
     post(-1, P, sw);
   }
   for (int s = 0; s + 1 < nsub; s++) {
     PassArgs<NB, UNIT> P;
     SubBlk sw;
     pre(s, P, sw);
-    fpass<NB, UNIT, true, true, true>(acc, W2, Wn, o, tto, P);
+    fpass<NB, UNIT, true, true, true, false>(acc, W2, Wn, o, tto, P, nullptr, hook(true, true, P, sw));
     post(s, P, sw);
   }
   {
     PassArgs<NB, UNIT> P;
     SubBlk sw;
     pre(nsub - 1, P, sw);
-    fpass<NB, UNIT, true, false, false>(acc, W2, Wn, o, tto, P);
+    // the strip is stored row group by row group during the pass
+    fpass<NB, UNIT, true, false, false, true>(acc, W2, Wn, o, tto, P, X + (size_t)col * NB,
+                                              hook(true, false, P, sw));
   }
   // no direct-to-LDS load may still be in flight when the next task reuses LDS
   __builtin_amdgcn_s_waitcnt(0);
-  store_strip<NB>(acc, X, col, ok, 0);
 }
 
 // --------------------------------------------------------------------------
