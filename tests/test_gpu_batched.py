@@ -200,3 +200,25 @@ def test_plan_runs_and_graph_replays(hq):
             assert all(torch.equal(t, r) for t, r in zip(TAU, TAUref)), rep
     finally:
         hq.batched_plan_free(plan)
+
+
+def test_plan_errors(hq):
+    """Unknown or released plans are rejected with an argument error; a plan
+    rejects invalid geometry at creation (nothing allocated)."""
+    import torch
+    nb, ib = 64, 16
+    tiles, A = make_batch(hq, [(64, 64)], nb, 5)
+    TAU = [torch.zeros(nb, dtype=torch.float64, device="cuda")]
+    with pytest.raises(hq.HQError):
+        hq.batched_run_dev(987654)
+    with pytest.raises(hq.HQError):
+        hq.batched_plan_dev(nb, 33, [1], [1], A, TAU)   # ib > 32
+    plan = hq.batched_plan_dev(nb, ib, [1], [1], A, TAU)
+    hq.batched_run_dev(plan)
+    hq.batched_plan_check(plan)
+    hq.batched_plan_free(plan)
+    for fn in (lambda: hq.batched_run_dev(plan), lambda: hq.batched_plan_check(plan),
+               lambda: hq.batched_plan_free(plan)):
+        with pytest.raises(hq.HQError):
+            fn()
+    assert hq.lib().hq_last_error() == 1   # HQ_ERR_ARG
