@@ -19,7 +19,11 @@ DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                 # keep MFMA accumulators in ordinary VGPRs: the strip of the
                 # updated tile is both an MFMA accumulator and an MFMA operand,
                 # and the AGPR form copies it between register files each block
-                "-mllvm", "-amdgpu-mfma-vgpr-form=1"]
+                "-mllvm", "-amdgpu-mfma-vgpr-form=1",
+                # every atomic read-modify-write here is issued by one lane; the
+                # optimizer's wave-wide rewrite would wait for the look-ahead
+                # claim at once instead of phases later
+                "-mllvm", "-amdgpu-atomic-optimizer-strategy=None"]
 
 
 def hipcc() -> str:

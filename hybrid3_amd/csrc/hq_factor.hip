@@ -159,7 +159,8 @@ struct Smem {
   };
   double tau[HQ_IBMAX];
   int task;
-  int nref;  // reflectors published in the current panel factorisation
+  int nref;     // reflectors published in the current panel factorisation
+  hq_task next; // descriptor of the task the workgroup runs next (scheduler)
 };
 
 // Global-address-space views of tile pointers.  Pointers read from the
@@ -1129,8 +1130,8 @@ __device__ __forceinline__ void fpass(d4 (&acc)[NB / 16], const d4 &W2, d4 (&Wn)
     __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
     __builtin_amdgcn_sched_group_barrier(0x008, 6, 0);
     __builtin_amdgcn_sched_barrier(0);
-    // work of the pass that no MFMA of it waits for, behind the first phase
-    if (p == 0) hook();
+    // work of the pass that no MFMA of it waits for, behind its phases
+    hook(p);
     // last pass: the rows of the previous phase are final; their stores go
     // out behind this phase's MFMAs instead of all at the end of the task
     if (DO_ST && p > 0) {
@@ -1184,9 +1185,65 @@ __device__ __forceinline__ void fix_unit(double *Vb, const SubBlk &sb) {
   }
 }
 
+// Look-ahead of the scheduler during the last pass of a slab task: wave 0,
+// in wave-uniform code, claims the next task (phase 0), reads its dependency
+// list (a few phases later) and its dependency flags (later still), each
+// behind MFMAs that do not wait for them.  The scheduler then takes the
+// claimed task and skips its wait when every flag was already complete; a
+// flag seen incomplete (including one of the task just finished) sends it
+// to the normal wait.  Flags only move to the current epoch, so "complete"
+// is never wrong.
+// Field by field: an aggregate copy of a descriptor goes through scratch.
+__device__ __forceinline__ void copy_task(hq_task &d, const hq_task &s) {
+  d.kind = s.kind;
+  d.k = s.k;
+  d.r = s.r;
+  d.j = s.j;
+  d.slab = s.slab;
+  d.mat = s.mat;
+  d.ndep = s.ndep;
+#pragma unroll
+  for (int i = 0; i < HQ_MAXDEP; i++) d.dep[i] = s.dep[i];
+}
+
+struct Ahead {  // passed by value; counter == nullptr: no look-ahead
+  int *counter;
+  const hq_task *tasks;
+  const int *flags;
+  int ntasks, epoch;
+};
+// Stage p of the look-ahead (wave 0 only); the state lives in the caller's
+// registers.
+template <int NP>
+__device__ __forceinline__ void ahead_stage(const Ahead &ah, int p, int &v, int &t, hq_task &tk,
+                                            int (&f)[HQ_MAXDEP], int &stop) {
+  constexpr int P_DEPS = NP / 2 - 1, P_FLAGS = NP - 2;
+  static_assert(P_DEPS > 0 && P_FLAGS > P_DEPS, "stages in distinct phases after the claim");
+  if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) != 0) return;
+  const int lane = threadIdx.x & 63;
+  // one lane claims; its result is read a few phases later, not waited for here
+  if (p == 0 && lane == 0) v = __hip_atomic_fetch_add(ah.counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (p == P_DEPS) {
+    t = __builtin_amdgcn_readlane(v, 0);
+    tk.ndep = 0;
+    if (t < ah.ntasks) copy_task(tk, ah.tasks[t]);
+  }
+  if (p == P_FLAGS) {
+#pragma unroll
+    for (int i = 0; i < HQ_MAXDEP; i++)
+      f[i] = i < tk.ndep ? __hip_atomic_load(ah.flags + tk.dep[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                         : ah.epoch;
+    // the scheduler's time-out word, read with the flags
+    stop = __hip_atomic_load(ah.counter + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Returns the task claimed by the look-ahead (x, -1: none; valid in wave 0,
+// its descriptor left in S.next); y bit 0: all its dependency flags were
+// seen complete, bit 1: the scheduler's time-out word was set.
 template <int NB, bool UNIT>
-__device__ __forceinline__ void task_update_f(Smem<NB> &S, const double *V, const double *T, double *X,
-                                              double *A1, int ib, int slab) {
+__device__ __forceinline__ int2 task_update_f(Smem<NB> &S, const double *V, const double *T, double *X,
+                                              double *A1, int ib, int slab, Ahead ah = Ahead{}) {
   static_assert(NB % HQ_SLAB == 0, "slabs cover the tile exactly");
   const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, g = lane >> 4, wave = tid >> 6;
   const int col = slab * HQ_SLAB + 16 * wave + m;
@@ -1242,8 +1299,16 @@ __device__ __forceinline__ void task_update_f(Smem<NB> &S, const double *V, cons
   // In the first phase of pass s, behind its MFMAs: (SSRFB) the A1 rows of
   // the update sub-block, a1 + W2 with W2 = -T^T W from the end of the
   // previous pass, then the A1 rows of the accumulate sub-block; T of s+2.
+  int la_v = 0, la_t = -1, la_stop = 0, la_f[HQ_MAXDEP];
+  hq_task la_tk;
+  la_tk.ndep = 0;
+#pragma unroll
+  for (int i = 0; i < HQ_MAXDEP; i++) la_f[i] = 0;
+  const bool look = ah.counter != nullptr;
   auto hook = [&](bool upd, bool acc_next, const PassArgs<NB, UNIT> &P, const SubBlk &sw) {
-    return [&, upd, acc_next]() {
+    return [&, upd, acc_next](int p) {
+      if (look && !acc_next) ahead_stage<NB / 32>(ah, p, la_v, la_t, la_tk, la_f, la_stop);
+      if (p != 0) return;
       if (!UNIT && upd) {
         a1o = a1v + W2;
         if (P.kbu == 16 && (P.i0u & 3) == 0) {
@@ -1312,6 +1377,12 @@ __device__ __forceinline__ void task_update_f(Smem<NB> &S, const double *V, cons
   }
   // no direct-to-LDS load may still be in flight when the next task reuses LDS
   __builtin_amdgcn_s_waitcnt(0);
+  bool ready = look;
+#pragma unroll
+  for (int i = 0; i < HQ_MAXDEP; i++) ready = ready && la_f[i] == ah.epoch;
+  // the claimed task's descriptor for every thread of the next iteration
+  if (look && __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0 && la_t < ah.ntasks) copy_task(S.next, la_tk);
+  return make_int2(look ? la_t : -1, (ready ? 1 : 0) | (la_stop != 0 ? 2 : 0));
 }
 
 // --------------------------------------------------------------------------
@@ -1687,32 +1758,35 @@ __device__ __forceinline__ void task_update_f32(Smem<NB> &S, const double *V, co
 }
 
 template <int NB, int WB>
-__device__ __forceinline__ void task_larfb(Smem<NB> &S, const double *V, const double *T, double *C,
+__device__ __forceinline__ int2 task_larfb(Smem<NB> &S, const double *V, const double *T, double *C,
                                            int ib, int slab, int prec = 0, const double *Wt = nullptr,
-                                           const double *Vd = nullptr) {
+                                           const double *Vd = nullptr, Ahead ah = Ahead{}) {
   if constexpr (NB >= 128) {
     if (prec)
       task_update_f32<NB, true>(S, V, T, C, nullptr, ib, slab);
     else if (Wt)
       task_larfb_w<NB>(S, Vd, Wt, C, slab);
     else
-      task_update_f<NB, true>(S, V, T, C, nullptr, ib, slab);
+      return task_update_f<NB, true>(S, V, T, C, nullptr, ib, slab, ah);
   } else {
     task_update<NB, WB, true>(S, V, T, C, nullptr, ib, slab);
   }
+  return make_int2(-1, 0);
 }
 
 template <int NB, int WB>
-__device__ __forceinline__ void task_ssrfb(Smem<NB> &S, double *A1, double *A2, const double *V,
-                                           const double *T, int ib, int slab, int prec = 0) {
+__device__ __forceinline__ int2 task_ssrfb(Smem<NB> &S, double *A1, double *A2, const double *V,
+                                           const double *T, int ib, int slab, int prec = 0,
+                                           Ahead ah = Ahead{}) {
   if constexpr (NB >= 128) {
     if (prec)
       task_update_f32<NB, false>(S, V, T, A2, A1, ib, slab);
     else
-      task_update_f<NB, false>(S, V, T, A2, A1, ib, slab);
+      return task_update_f<NB, false>(S, V, T, A2, A1, ib, slab, ah);
   } else {
     task_update<NB, WB, false>(S, V, T, A2, A1, ib, slab);
   }
+  return make_int2(-1, 0);
 }
 
 __device__ __forceinline__ double *tile_ptr(const hq_geom &G, int i, int j) {
@@ -1734,8 +1808,9 @@ __device__ __forceinline__ double *tau_ptr(const hq_geom &G, int i, int j) {
   return G.TAU + ((size_t)i + (size_t)j * G.MT) * (size_t)G.nb;
 }
 
+// Returns the look-ahead of a slab task (see task_update_f), else (-1, 0).
 template <int NB, int WB>
-__device__ __forceinline__ void run_task(Smem<NB> &S, const hq_geom &G, const hq_task &t) {
+__device__ __forceinline__ int2 run_task(Smem<NB> &S, const hq_geom &G, const hq_task &t, Ahead ah = Ahead{}) {
   switch (t.kind) {
     case HQ_GEQRT:  // j = ib block of the panel, or -1: the whole tile
       if (t.j < 0)
@@ -1761,16 +1836,15 @@ __device__ __forceinline__ void run_task(Smem<NB> &S, const hq_geom &G, const hq
                                 t.slab);
       break;
     case HQ_LARFB:
-      task_larfb<NB, WB>(S, tile_ptr(G, t.k, t.k), t_ptr(G, t.k, t.k), xtile_ptr(G, t.k, t.j), G.ib,
-                         t.slab, G.prec, wd_ptr(G, t.k), vd_ptr(G, t.k));
-      break;
+      return task_larfb<NB, WB>(S, tile_ptr(G, t.k, t.k), t_ptr(G, t.k, t.k), xtile_ptr(G, t.k, t.j), G.ib,
+                                t.slab, G.prec, wd_ptr(G, t.k), vd_ptr(G, t.k), ah);
     case HQ_SSRFB:
-      task_ssrfb<NB, WB>(S, xtile_ptr(G, t.k, t.j), xtile_ptr(G, t.r, t.j), tile_ptr(G, t.r, t.k),
-                         t_ptr(G, t.r, t.k), G.ib, t.slab, G.prec);
-      break;
+      return task_ssrfb<NB, WB>(S, xtile_ptr(G, t.k, t.j), xtile_ptr(G, t.r, t.j), tile_ptr(G, t.r, t.k),
+                                t_ptr(G, t.r, t.k), G.ib, t.slab, G.prec, ah);
     default:
       break;
   }
+  return make_int2(-1, 0);
 }
 
 // Persistent DAG scheduler.  Workgroups claim task indices in order from one
@@ -1793,6 +1867,8 @@ __global__ __launch_bounds__(HQ_THREADS) void sched_kernel(hq_geom G, const hq_g
                               : epoch_arg;
   int done = -1;  // task finished by this workgroup in the previous iteration
   long long t_claim = 0;
+  const Ahead ah{counter, tasks, flags, ntasks, epoch};
+  int2 la = make_int2(-1, 0);  // the next task, claimed during the last pass of a slab task
   for (;;) {
     // Thread 0 publishes the previous task and claims the next one in ONE
     // divergent region.  (Two separate thread-0 regions at the end and the
@@ -1808,13 +1884,19 @@ __global__ __launch_bounds__(HQ_THREADS) void sched_kernel(hq_geom G, const hq_g
         __hip_atomic_store(&flags[done], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
       if (trace && done >= 0) trace[4 * done + 2] = wall_clock64();
-      int t = atomicAdd(counter, 1);
+      int t = la.x >= 0 ? la.x : atomicAdd(counter, 1);
+      const bool seen_ready = la.x >= 0 && (la.y & 1) != 0;
       if (trace) t_claim = wall_clock64();
-      // counter[1] != 0: a wait timed out somewhere; drain without work
-      if (__hip_atomic_load(&counter[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+      // counter[1] != 0: a wait timed out somewhere; drain without work (a
+      // look-ahead read it already)
+      if (la.x >= 0 ? (la.y & 2) != 0
+                    : __hip_atomic_load(&counter[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
         t = ntasks;
-      if (t < ntasks) {
+      if (t < ntasks && !seen_ready) {
+        // (a look-ahead that saw the task ready left its descriptor in S.next
+        // and read nothing else this needs)
         const hq_task &tk = tasks[t];
+        if (la.x < 0) copy_task(S.next, tk);
         const int nd = tk.ndep;
         unsigned spins = 0;
         for (;;) {
@@ -1835,8 +1917,8 @@ __global__ __launch_bounds__(HQ_THREADS) void sched_kernel(hq_geom G, const hq_g
             break;
           }
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       }
+      if (t < ntasks) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       S.task = t;
     }
     __syncthreads();
@@ -1844,16 +1926,24 @@ __global__ __launch_bounds__(HQ_THREADS) void sched_kernel(hq_geom G, const hq_g
     // are wave-uniform for the compiler
     const int t = __builtin_amdgcn_readfirstlane(S.task);
     if (t >= ntasks) break;
-    const hq_task tk = tasks[t];
+    // the descriptor from LDS, each field made wave-uniform like the index
+    hq_task tk;
+    tk.kind = __builtin_amdgcn_readfirstlane(S.next.kind);
+    tk.k = __builtin_amdgcn_readfirstlane(S.next.k);
+    tk.r = __builtin_amdgcn_readfirstlane(S.next.r);
+    tk.j = __builtin_amdgcn_readfirstlane(S.next.j);
+    tk.slab = __builtin_amdgcn_readfirstlane(S.next.slab);
+    tk.mat = __builtin_amdgcn_readfirstlane(S.next.mat);
+    tk.ndep = 0;  // (dependencies are the scheduler's business only)
     if (trace && threadIdx.x == 0) {
       trace[4 * t + 0] = t_claim;
       trace[4 * t + 1] = wall_clock64();
       trace[4 * t + 3] = blockIdx.x;
     }
     if (geoms)
-      run_task<NB, WB>(S, geoms[tk.mat], tk);
+      la = run_task<NB, WB>(S, geoms[tk.mat], tk, ah);
     else
-      run_task<NB, WB>(S, G, tk);
+      la = run_task<NB, WB>(S, G, tk, ah);
     // each wave's stores have completed (a barrier waits for LDS only), so
     // none of them can land after the task's flag is published
     __builtin_amdgcn_s_waitcnt(0);
