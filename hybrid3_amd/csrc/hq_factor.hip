@@ -1236,6 +1236,10 @@ __device__ __forceinline__ void ahead_stage(const Ahead &ah, int p, int &v, int 
     // the scheduler's time-out word, read with the flags
     stop = __hip_atomic_load(ah.counter + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
+  // acquire after the flag loads (in program order), behind the last phase:
+  // if they show the task ready the scheduler starts it without a fence of
+  // its own; the pass itself reads nothing more from global memory
+  if (p == NP - 1) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
 // Returns the task claimed by the look-ahead (x, -1: none; valid in wave 0,
@@ -1918,7 +1922,7 @@ __global__ __launch_bounds__(HQ_THREADS) void sched_kernel(hq_geom G, const hq_g
           }
         }
       }
-      if (t < ntasks) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      if (t < ntasks && !seen_ready) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       S.task = t;
     }
     __syncthreads();
