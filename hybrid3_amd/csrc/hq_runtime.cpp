@@ -233,21 +233,23 @@ struct DagBuilder {
       }
     }
   }
-  // Estimated durations (us) of each task kind for the priority computation,
-  // from the measured per-task times at nb = 256 (GEQRT 590, TSQRT 630,
-  // LARFB 99, SSRFB 83 us per 64-column slab) scaled by the work of the kind.
+  // Estimated durations (us) of each task kind for the priority computation:
+  // the per-task means measured by the trace build at nb = 256, ib = 32
+  // (GEQRT panel 58, TSQRT panel 43, trailing tasks 22 per 64-column slab of
+  // the full tile height, LARFB 80 and SSRFB 85 per slab), scaled by the
+  // work of the kind; a whole-tile panel QR (batches) ≈ 540.
   double cost(const hq_task &x) const {
     const double f = nb / 256.0;
     const double w = (double)std::min(HQ_SLAB, nb) / 64.0;  // slab width / 64
+    if ((x.kind == HQ_GEQRT || x.kind == HQ_TSQRT) && x.j < 0) return 540.0 * f * f * f;
     const double c = std::min(ib, nb - x.j * ib) / 32.0;     // panel block width / 32
-    if ((x.kind == HQ_GEQRT || x.kind == HQ_TSQRT) && x.j < 0) return 540.0 * f * f * f;  // whole tile
     switch (x.kind) {
-      case HQ_GEQRT: return 36.0 * f * c;
-      case HQ_TSQRT: return 38.0 * f * c;
-      case HQ_GEQRT_U: return 11.0 * f * c * w * (nb - x.j * ib) / nb;
-      case HQ_TSQRT_U: return 11.0 * f * c * w;
-      case HQ_LARFB: return 99.0 * f * f * w;
-      default: return 83.0 * f * f * w;
+      case HQ_GEQRT: return 58.0 * f * c;
+      case HQ_TSQRT: return 43.0 * f * c;
+      case HQ_GEQRT_U: return 40.0 * f * c * w * (nb - x.j * ib) / nb;
+      case HQ_TSQRT_U: return 22.0 * f * c * w;
+      case HQ_LARFB: return 80.0 * f * f * w;
+      default: return 85.0 * f * f * w;
     }
   }
   // List-scheduling simulation on P workers with bottom-level priorities;
