@@ -372,15 +372,17 @@ __device__ __forceinline__ void gen_reflector(Smem<NB> &S, int c, int d, double 
     ss = two[0];
     alpha = two[1];
   }
-  // LAPACK dlarfg: beta = -sign(alpha) * ||(alpha, x)||
-  double xnorm = sqrt(ss);
+  // LAPACK dlarfg: beta = -sign(alpha) * ||(alpha, x)||.  ss is already an
+  // unscaled sum of squares, so the norm is taken directly as
+  // sqrt(alpha^2 + ss) (dlapy2's scaling would add no range here and costs a
+  // division and a second square root on the per-column chain).
   double tau, beta, scal;
-  if (xnorm == 0.0) {
+  if (ss == 0.0) {
     tau = 0.0;
     beta = alpha;
     scal = 1.0;
   } else {
-    beta = -copysign(dlapy2(alpha, xnorm), alpha);
+    beta = -copysign(sqrt(fma(alpha, alpha, ss)), alpha);
     tau = (beta - alpha) / beta;
     scal = 1.0 / (alpha - beta);
   }
@@ -405,10 +407,39 @@ __device__ __forceinline__ void lds_wait_geq(int *flag, int val) {
   }
 }
 
+// Apply reflector c (v, tau) to this wave's panel columns j = 4t + wave with
+// j > c + 1, for the register slots t >= T0 (earlier slots are finished).
+template <int NB, bool TS, int T0>
+__device__ __forceinline__ void panel_apply(double (&x)[HQ_IBMAX / 4][NB / 64],
+                                            double (&a1)[HQ_IBMAX / 4], const double (&v)[NB / 64],
+                                            double tau, int c, int sb, int wave, int lane) {
+  constexpr int NQ = NB / 64, JW = HQ_IBMAX / 4, N = JW - T0;
+  double dot[N];
+#pragma unroll
+  for (int t = 0; t < N; t++) {
+    dot[t] = 0.0;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) dot[t] += v[q] * x[T0 + t][q];
+  }
+  wave_sums<N>(dot);
+#pragma unroll
+  for (int t = T0; t < JW; t++) {
+    const int j = 4 * t + wave;
+    if (j > c + 1 && j < sb) {
+      const double top = TS ? readlane_d(a1[t], c) : 0.0;
+      const double w = tau * (dot[t - T0] + top);
+#pragma unroll
+      for (int q = 0; q < NQ; q++) x[t][q] -= v[q] * w;
+      if (TS && lane == c) a1[t] = top - w;
+    }
+  }
+}
+
 template <int NB, bool TS>
 __device__ __forceinline__ void panel_factor(Smem<NB> &S, int ii, int sb) {
   constexpr int LDV = Smem<NB>::LDV, LDT = Smem<NB>::LDT;
   constexpr int NQ = NB / 64, JW = HQ_IBMAX / 4;
+  static_assert(JW == 8, "panel_factor: the live-slot dispatch assumes 8 register slots");
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   double x[JW][NQ], a1[JW];
@@ -461,25 +492,14 @@ __device__ __forceinline__ void panel_factor(Smem<NB> &S, int ii, int sb) {
       __hip_atomic_store(&S.nref, c + 2, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     if (c + 2 < sb) {
-      double dot[JW];
-#pragma unroll
-      for (int t = 0; t < JW; t++) {
-        dot[t] = 0.0;
-#pragma unroll
-        for (int q = 0; q < NQ; q++) dot[t] += v[q] * x[t][q];
-      }
-      wave_sums<JW>(dot);
-#pragma unroll
-      for (int t = 0; t < JW; t++) {
-        const int j = 4 * t + wave;
-        if (j > c + 1 && j < sb) {
-          const double top = TS ? readlane_d(a1[t], c) : 0.0;
-          const double w = tau * (dot[t] + top);
-#pragma unroll
-          for (int q = 0; q < NQ; q++) x[t][q] -= v[q] * w;
-          if (TS && lane == c) a1[t] = top - w;
-        }
-      }
+      // this wave's columns j = 4t + wave still to update are those with
+      // j > c + 1, i.e. t >= tmin; reduce only the slots from the last
+      // power-of-two group covering them
+      const int live = JW - ((c + 5 - wave) >> 2);
+      if (live > 4) panel_apply<NB, TS, 0>(x, a1, v, tau, c, sb, wave, lane);
+      else if (live > 2) panel_apply<NB, TS, JW - 4>(x, a1, v, tau, c, sb, wave, lane);
+      else if (live == 2) panel_apply<NB, TS, JW - 2>(x, a1, v, tau, c, sb, wave, lane);
+      else if (live == 1) panel_apply<NB, TS, JW - 1>(x, a1, v, tau, c, sb, wave, lane);
     }
   }
   __syncthreads();
