@@ -6,7 +6,7 @@ files, Hybrid_tile_QR.c:64-74): one factorisation is run with device
 wall-clock stamps (claim / start / end per task, and the workgroup that ran
 it), then summarised per task kind and optionally written as a Chrome trace.
 
-    python -m hybrid3_amd.trace --n 16384 --nb 256 --ib 32 [--chrome out.json]
+    python -m hybrid3_amd.trace --n 16384 --nb 256 --ib 32 [--chrome out.json] [--critical-path]
 """
 from __future__ import annotations
 
@@ -68,6 +68,32 @@ def summarize(tasks, us, grid):
     return out
 
 
+def critical_path(tasks, us, MT, NT, nb, ib, grid):
+    """Longest dependency chain (us) with the measured task durations, and the
+    last time a task became ready and then waited more than 5 us to start
+    (claim and acquire take 1-3 us; after this time the run is limited by
+    the DAG, not by the claim order).  None if the host DAG's order
+    differs from the traced one."""
+    import hybrid3_amd as hq
+    t2, deps = hq.dag_host(MT, NT, nb, ib, 0, grid)
+    if t2.shape != tasks.shape or not (t2 == tasks).all():
+        return None
+    dur = us[:, 2] - us[:, 1]
+    fin = np.zeros(len(tasks))
+    ready = np.zeros(len(tasks))
+    for i, d in enumerate(deps):
+        d = d[d >= 0]
+        if len(d):
+            fin[i] = fin[d].max()
+            ready[i] = us[d, 2].max()
+        fin[i] += dur[i]
+    late = us[:, 1] - ready > 5.0          # ready, but held behind other claims
+    tail = float(ready[late].max()) if late.any() else 0.0
+    return {"critical_path_us": float(fin.max()),
+            "work_per_workgroup_us": float(dur.sum() / grid),
+            "last_held_ready_task_us": tail}
+
+
 def chrome(tasks, us, path):
     ev = []
     for (kind, k, r, j, s), (c, b, e, wg) in zip(tasks, us):
@@ -85,6 +111,8 @@ def main():
     ap.add_argument("--ib", type=int, default=32)
     ap.add_argument("--seed", type=int, default=20151130)
     ap.add_argument("--chrome", default="")
+    ap.add_argument("--critical-path", action="store_true",
+                    help="also report the DAG's critical path with the measured durations")
     ap.add_argument("--raw", default="", help="write the task list and stamps (us) to this .npz")
     args = ap.parse_args()
     import torch
@@ -102,6 +130,8 @@ def main():
     s = summarize(tasks, us, grid)
     flops = 4.0 / 3.0 * args.n ** 3 if m == args.n else 2.0 * m * args.n ** 2 - 2.0 * args.n ** 3 / 3
     s["tflops_traced"] = flops / (s["makespan_us"] * 1e-6) / 1e12
+    if args.critical_path:
+        s["critical_path"] = critical_path(tasks, us, MT, NT, args.nb, args.ib, grid)
     print(json.dumps(s, indent=1))
     if args.chrome:
         chrome(tasks, us, args.chrome)

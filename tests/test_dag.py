@@ -177,3 +177,38 @@ def test_random_interleavings_reproduce_sequential_result(hq, orc, seed):
                 ready.append(q)
     assert done.all()
     assert np.array_equal(A, ref) and np.array_equal(TAU, tref) and np.array_equal(T, Tref)
+
+
+def test_trace_critical_path_matches_brute_force(hq):
+    """hybrid3_amd.trace.critical_path on synthetic stamps: the longest
+    dependency chain equals an independent longest-path search (memoised
+    recursion over predecessors), and a task started late is reported."""
+    import sys
+    from hybrid3_amd import trace
+    MT, NT, nb, ib, P = 3, 3, 64, 16, 4
+    tasks, deps = hq.dag_host(MT, NT, nb, ib, 0, P)
+    n = len(tasks)
+    rng = np.random.default_rng(7)
+    dur = rng.uniform(1.0, 50.0, n)
+    # a valid schedule: each task starts 2 us after its last dependency ends
+    us = np.zeros((n, 4))
+    for i in range(n):
+        d = deps[i][deps[i] >= 0]
+        st = (us[d, 2].max() if len(d) else 0.0) + 2.0
+        us[i] = (st - 1.0, st, st + dur[i], i % P)
+    late = n - 1
+    us[late, 1:3] += 40.0                       # the last task is held back
+    sys.setrecursionlimit(10000)
+    memo = {}
+
+    def longest(i):
+        if i not in memo:
+            d = deps[i][deps[i] >= 0]
+            memo[i] = (us[i, 2] - us[i, 1]) + max((longest(int(j)) for j in d), default=0.0)
+        return memo[i]
+    want = max(longest(i) for i in range(n))
+    got = trace.critical_path(tasks, us, MT, NT, nb, ib, P)
+    assert abs(got["critical_path_us"] - want) < 1e-9
+    assert abs(got["work_per_workgroup_us"] - (us[:, 2] - us[:, 1]).sum() / P) < 1e-9
+    d = deps[late][deps[late] >= 0]
+    assert got["last_held_ready_task_us"] == us[d, 2].max()
