@@ -126,15 +126,6 @@ __device__ __forceinline__ double wave_sum1(double x) {
   return v[0];
 }
 
-__device__ __forceinline__ double dlapy2(double x, double y) {
-  double xa = fabs(x), ya = fabs(y);
-  double w = xa > ya ? xa : ya;
-  double z = xa < ya ? xa : ya;
-  if (z == 0.0) return w;
-  double q = z / w;
-  return w * sqrt(1.0 + q * q);
-}
-
 template <int NB>
 struct Smem {
   // column stride NB+2: each column starts 16-byte aligned (direct-to-LDS loads
