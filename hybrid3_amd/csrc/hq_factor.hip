@@ -346,22 +346,23 @@ __device__ __forceinline__ void gen_reflector(Smem<NB> &S, int c, int d, double 
                                               double &a1) {
   constexpr int LDV = Smem<NB>::LDV;
   const int lane = threadIdx.x & 63;
-  double ss = 0.0, xd = 0.0;
+  double ss = 0.0;
 #pragma unroll
   for (int q = 0; q < NB / 64; q++) {
     int r = lane + 64 * q;
     if (r > d) ss += x[q] * x[q];
-    if (!TS && r == d) xd = x[q];
   }
+  ss = wave_sum1(ss);
   double alpha;
   if (TS) {
-    ss = wave_sum1(ss);
     alpha = readlane_d(a1, c);
   } else {
-    double two[2] = {ss, xd};
-    wave_sums<2>(two);
-    ss = two[0];
-    alpha = two[1];
+    // the diagonal element: register d / 64 of lane d % 64 (d is uniform)
+    double xq = x[0];
+#pragma unroll
+    for (int q = 1; q < NB / 64; q++)
+      if (q == (d >> 6)) xq = x[q];
+    alpha = readlane_d(xq, d & 63);
   }
   // LAPACK dlarfg: beta = -sign(alpha) * ||(alpha, x)||.  ss is already an
   // unscaled sum of squares, so the norm is taken directly as
