@@ -381,11 +381,10 @@ __device__ __forceinline__ void gen_reflector(Smem<NB> &S, int c, int d, double 
 #pragma unroll
   for (int q = 0; q < NB / 64; q++) {
     int r = lane + 64 * q;
-    if (r > d) {
-      x[q] *= scal;
-      S.V[c * LDV + r] = x[q];
-    }
-    if (!TS && r == d) x[q] = beta;
+    // branch-free: every reader of this LDS column masks rows <= d itself,
+    // and the panel's final write-back overwrites the whole column
+    x[q] = r > d ? x[q] * scal : ((!TS && r == d) ? beta : x[q]);
+    S.V[c * LDV + r] = x[q];
   }
   if (TS && lane == c) a1 = beta;
   if (lane == 0) S.tau[c] = tau;
