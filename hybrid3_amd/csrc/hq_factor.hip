@@ -408,7 +408,8 @@ __device__ __forceinline__ void panel_apply(double (&x)[HQ_IBMAX / 4][NB / 64],
   double dot[N];
 #pragma unroll
   for (int t = 0; t < N; t++) {
-    dot[t] = 0.0;
+    // (TSQRT: the triangle's row c, held by lane c, joins the sum)
+    dot[t] = (TS && lane == c) ? a1[T0 + t] : 0.0;
 #pragma unroll
     for (int q = 0; q < NQ; q++) dot[t] += v[q] * x[T0 + t][q];
   }
@@ -417,11 +418,10 @@ __device__ __forceinline__ void panel_apply(double (&x)[HQ_IBMAX / 4][NB / 64],
   for (int t = T0; t < JW; t++) {
     const int j = 4 * t + wave;
     if (j > c + 1 && j < sb) {
-      const double top = TS ? readlane_d(a1[t], c) : 0.0;
-      const double w = tau * (dot[t - T0] + top);
+      const double w = tau * dot[t - T0];
 #pragma unroll
       for (int q = 0; q < NQ; q++) x[t][q] -= v[q] * w;
-      if (TS && lane == c) a1[t] = top - w;
+      if (TS && lane == c) a1[t] -= w;
     }
   }
 }
@@ -468,15 +468,14 @@ __device__ __forceinline__ void panel_factor(Smem<NB> &S, int ii, int sb) {
 #pragma unroll
       for (int t = 0; t < JW; t++) {
         if (t == ((c + 1) >> 2)) {
-          double dt = 0.0;
+          double dt = (TS && lane == c) ? a1[t] : 0.0;
 #pragma unroll
           for (int q = 0; q < NQ; q++) dt += v[q] * x[t][q];
           dt = wave_sum1(dt);
-          const double top = TS ? readlane_d(a1[t], c) : 0.0;
-          const double w = tau * (dt + top);
+          const double w = tau * dt;
 #pragma unroll
           for (int q = 0; q < NQ; q++) x[t][q] -= v[q] * w;
-          if (TS && lane == c) a1[t] = top - w;
+          if (TS && lane == c) a1[t] -= w;
           gen_reflector<NB, TS>(S, c + 1, TS ? -1 : ii + c + 1, x[t], a1[t]);
         }
       }
