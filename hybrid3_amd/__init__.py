@@ -85,6 +85,20 @@ SIGNATURES = {
     "hq_tile_to_cm_dev": (C.c_int, [C.c_int] * 2 + [C.POINTER(C.c_int)] * 2
                           + [C.POINTER(_vp), C.POINTER(_vp), C.POINTER(C.c_int), _vp]),
     "hq_tile_qr_mixed_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp]),
+    "hq_tile_qr_mn_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp]),
+    # mat_prod.h harness helpers
+    "mat_make": (None, [C.POINTER(mat_stat_t), _dp, C.c_uint, C.c_int]),
+    "mat_remv": (None, [C.POINTER(mat_stat_t), _dp, C.c_uint, C.c_int]),
+    "mat_comp3": (C.c_double, [C.POINTER(mat_stat_t), _dp, C.c_int, C.c_int]),
+    "mat_dprint": (None, [C.POINTER(mat_stat_t), _dp, _dp]),
+    "mat_print": (None, [C.POINTER(mat_stat_t), _vp, _dp, C.c_char_p]),
+    # core_blas.h (PLASMA core_blas shim)
+    "CORE_dgeqrt": (C.c_int, [C.c_int] * 3 + [_dp, C.c_int, _dp, C.c_int, _dp, _dp]),
+    "CORE_dtsqrt": (C.c_int, [C.c_int] * 3 + [_dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _dp]),
+    "CORE_dormqr": (C.c_int, [C.c_int] * 6 + [_dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int]),
+    "CORE_dtsmqr": (C.c_int, [C.c_int] * 8 + [_dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp,
+                                             C.c_int]),
+    "hq_set_wait_limit": (C.c_int, [C.c_uint]),
     "hq_sync": (C.c_int, [_vp]),
     "hq_finalize": (None, []),
 }
@@ -169,6 +183,11 @@ def tile_qr_dev(MT, NT, nb, ib, A, TAU, T=None, stream=None, serial=False):
 def tile_qr_mixed_dev(MT, NT, nb, ib, A, TAU, T=None, stream=None):
     """FP64 panels, FP32 LARFB / SSRFB updates (mixed precision)."""
     check(lib().hq_tile_qr_mixed_dev(MT, NT, nb, ib, _ptr(A), _ptr(TAU), _ptr(T), _stream(stream)))
+
+
+def tile_qr_mn_dev(m, n, nb, ib, A, TAU, T=None, stream=None):
+    """Tile QR of an m x n matrix with edge tiles (padding zeroed by the call)."""
+    check(lib().hq_tile_qr_mn_dev(m, n, nb, ib, _ptr(A), _ptr(TAU), _ptr(T), _stream(stream)))
 
 
 def fill_uniform_dev(MT, NT, nb, seed, A, stream=None):

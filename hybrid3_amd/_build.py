@@ -10,7 +10,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libhybrid3.so")
-SOURCES = ["hq_factor.hip", "hq_homegrown.hip", "hq_runtime.cpp"]
+MAIN = os.path.join(HERE, "hybrid3")  # `hybrid3 N NB [IB]`, the reference's main.c
+SOURCES = ["hq_factor.hip", "hq_homegrown.hip", "hq_runtime.cpp", "hq_harness.cpp"]
 ARCH = os.environ.get("HQ_OFFLOAD_ARCH", "gfx950")
 
 # Packed FP32 ops are switched off for every device compile (the hardware
@@ -57,8 +58,17 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("link of libhybrid3.so failed")
     os.replace(tmp, LIB)
+    # the reference's driver program on the library (plain C, gcc)
+    cmd = ["gcc", "-O2", "-std=c11", "-D_POSIX_C_SOURCE=199309L", "-Wall", "-I", os.path.join(HERE, "..", "include"),
+           os.path.join(CSRC, "hq_main.c"), "-o", MAIN + ".tmp", "-L", HERE, "-lhybrid3",
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build of the hybrid3 driver failed")
+    os.replace(MAIN + ".tmp", MAIN)
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", MAIN)
     return LIB
 
 

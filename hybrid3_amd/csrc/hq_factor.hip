@@ -390,11 +390,22 @@ __device__ __forceinline__ void gen_reflector(Smem<NB> &S, int c, int d, double 
   if (lane == 0) S.tau[c] = tau;
 }
 
-// Wait (bounded) until the LDS counter reaches val; acquire at workgroup scope.
+// Device fault word of the library: nonzero after a bounded wait inside a
+// task gave up (the task then finishes on whatever it read, so its results
+// are wrong).  Read and cleared by the host after every synchronisation of a
+// host-pointer entry point and by hq_sync.
+__device__ int g_fault;
+
+// Wait (bounded) until the LDS counter reaches val; acquire at workgroup
+// scope.  Giving up records the fault word (1 + the awaited count).
 __device__ __forceinline__ void lds_wait_geq(int *flag, int val) {
-  for (unsigned spins = 0; spins < (1u << 22); spins++) {
+  for (unsigned spins = 0;; spins++) {
     int f = __hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (__builtin_amdgcn_readfirstlane(f) >= val) break;
+    if (spins >= (1u << 22)) {
+      if ((threadIdx.x & 63) == 0) __hip_atomic_store(&g_fault, 1 + val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      break;
+    }
   }
 }
 
@@ -1873,7 +1884,7 @@ __global__ __launch_bounds__(HQ_THREADS) void sched_kernel(hq_geom G, const hq_g
                                                           const hq_task *__restrict__ tasks,
                                                           int ntasks, int *flags, int epoch_arg,
                                                           int *counter, long long *trace,
-                                                          const int *epoch_dev) {
+                                                          const int *epoch_dev, unsigned wait_limit) {
   __shared__ Smem<NB> S;
   // this run's flag value: the argument, or the device epoch advanced by the
   // counter reset just before (graph replays)
@@ -1925,7 +1936,7 @@ __global__ __launch_bounds__(HQ_THREADS) void sched_kernel(hq_geom G, const hq_g
           for (int i = 0; i < HQ_MAXDEP; i++) ready = ready && f[i] == epoch;
           if (ready) break;
           __builtin_amdgcn_s_sleep(2);
-          if (++spins > (1u << 22)) {  // seconds: a dependency never completed
+          if (++spins > wait_limit) {  // (default 2^22: seconds) a dependency never completed
             __hip_atomic_store(&counter[1], 1 + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             t = ntasks;
             break;
@@ -2039,6 +2050,108 @@ __global__ __launch_bounds__(HQ_THREADS) void tbuild_kernel(hq_geom G, const int
   gram<NB, WB>(S, diag ? i0 / 16 : 0);
   __syncthreads();
   if (threadIdx.x < 64) build_t<NB>(S, kb, t_ptr(G, r, k) + (size_t)i0 * ib, ib, nullptr);
+}
+
+// T of one block of kb <= ib reflectors wider than the LDS-staged builder
+// takes (ib > 32; PLASMA's T layout with ld = ib for any ib <= nb): the
+// block's columns i0 .. i0+kb-1 of reflector tile V (ld = NB; diag: unit lower
+// with the diagonal at row i0 + c, else the V2 of a TSQRT tile under an
+// identity), tau of the block, T block at Tb (column c at Tb + c*ib).
+//   1. G(p, c) = v_p^T v_c for p < c, one wave per pair, into T's strict
+//      upper part;
+//   2. dlarft('F','C'): column by column, T(0:c, c) = -tau_c T(0:c, 0:c)
+//      G(0:c, c), T(c, c) = tau_c; column c of G is read into LDS before the
+//      column is overwritten.  Rows c < p < ib are written as zero.
+template <int NB>
+__device__ void tbuild_wide_body(const double *V, const double *tau, double *Tb, int ib, int i0, int kb,
+                                 bool diag, bool drop_last) {
+  __shared__ double gcol[NB];
+  __shared__ double tv[NB];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int c = tid; c < kb; c += HQ_THREADS) {
+    double t = tau[c];
+    if (drop_last && diag && i0 + c == NB - 1) t = 0.0;
+    tv[c] = t;
+  }
+  const int npair = kb * (kb - 1) / 2;
+  for (int e = wave; e < npair; e += HQ_THREADS / 64) {
+    // pair e -> (p, c), p < c: c = largest with c(c-1)/2 <= e
+    int c = (int)((1.0 + sqrt(1.0 + 8.0 * (double)e)) * 0.5);
+    while (c * (c - 1) / 2 > e) c--;
+    while ((c + 1) * c / 2 <= e) c++;
+    const int p = e - c * (c - 1) / 2;
+    const double *vp = V + (size_t)(i0 + p) * NB, *vc = V + (size_t)(i0 + c) * NB;
+    double s = 0.0;
+    for (int row = lane; row < NB; row += 64) {
+      double a = vp[row], b = vc[row];
+      if (diag) {
+        const int dp = i0 + p, dc = i0 + c;
+        a = row > dp ? a : (row == dp ? 1.0 : 0.0);
+        b = row > dc ? b : (row == dc ? 1.0 : 0.0);
+      }
+      s += a * b;
+    }
+    s = wave_sum1(s);
+    if (lane == 0) Tb[(size_t)c * ib + p] = s;
+  }
+  __syncthreads();
+  for (int c = 0; c < kb; c++) {
+    for (int q = tid; q < c; q += HQ_THREADS) gcol[q] = Tb[(size_t)c * ib + q];
+    __syncthreads();
+    const double tc = tv[c];
+    for (int p = tid; p < ib; p += HQ_THREADS) {
+      double val = 0.0;
+      if (p < c) {
+        double s = 0.0;
+        for (int q = p; q < c; q++) s += Tb[(size_t)q * ib + p] * gcol[q];
+        val = -tc * s;
+      } else if (p == c) {
+        val = tc;
+      }
+      Tb[(size_t)c * ib + p] = val;
+    }
+    __syncthreads();
+  }
+}
+
+// Every reflector tile of a tile grid (grid.x = ib block, grid.y = entry of
+// the lower tile list), as tbuild_kernel for wide blocks.
+template <int NB>
+__global__ __launch_bounds__(HQ_THREADS) void tbuild_wide_kernel(hq_geom G, const int2 *tiles, int drop_last) {
+  const int2 rk = tiles[blockIdx.y];
+  const int r = rk.x, k = rk.y, ib = G.ib, i0 = blockIdx.x * ib;
+  if (i0 >= NB) return;
+  tbuild_wide_body<NB>(tile_ptr(G, r, k), tau_ptr(G, r, k) + i0, t_ptr(G, r, k) + (size_t)i0 * ib, ib, i0,
+                       min(ib, NB - i0), r == k, drop_last != 0);
+}
+// One tile given by pointers (grid.x = ib block).
+template <int NB>
+__global__ __launch_bounds__(HQ_THREADS) void tbuild_wide_tile_kernel(const double *V, const double *tau,
+                                                                     double *T, int ib, int diag) {
+  const int i0 = blockIdx.x * ib;
+  if (i0 >= NB) return;
+  tbuild_wide_body<NB>(V, tau + i0, T + (size_t)i0 * ib, ib, i0, min(ib, NB - i0), diag != 0, false);
+}
+
+// Zero the entries of an MT x NT tile grid past row m / column n (the
+// padding of edge tiles), touching only the last tile row and column.
+__global__ void pad_zero_kernel(double *A, int MT, int NT, int nb, int m, int n) {
+  const size_t ts = (size_t)nb * nb;
+  const size_t total = ts * (size_t)(MT + NT);
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t t = e / ts, w = e - t * ts;
+    const size_t c = w / nb, rr = w - c * nb;
+    size_t ti, tj;
+    if (t < (size_t)NT) {  // last tile row, tile column t
+      ti = MT - 1;
+      tj = t;
+    } else {               // last tile column, tile row t - NT
+      ti = t - NT;
+      tj = NT - 1;
+    }
+    const size_t row = ti * nb + rr, col = tj * nb + c;
+    if (row >= (size_t)m || col >= (size_t)n) A[(ti + tj * (size_t)MT) * ts + w] = 0.0;
+  }
 }
 
 // X <- identity (full = true: every entry written) or only ones on the
@@ -2159,12 +2272,12 @@ static hipError_t hq_dispatch(int nb, int ib, Fn &&fn) {
 
 hipError_t hq_launch_sched(const hq_geom &G, const hq_geom *geoms, const hq_task *tasks, int ntasks,
                            int *flags, int epoch, int *counter, int grid, long long *trace, hipStream_t st,
-                           int *epoch_dev) {
+                           int *epoch_dev, unsigned wait_limit) {
   hipLaunchKernelGGL(hq::reset_counter_kernel, dim3(1), dim3(64), 0, st, counter, epoch_dev);
   return hq_dispatch(G.nb, G.ib, [&](auto nbc, auto wbc) {
     HQ_NBWB
     hipLaunchKernelGGL((hq::sched_kernel<NB, WB>), dim3(grid), dim3(HQ_THREADS), 0, st, G, geoms, tasks,
-                       ntasks, flags, epoch, counter, trace, (const int *)epoch_dev);
+                       ntasks, flags, epoch, counter, trace, (const int *)epoch_dev, wait_limit);
   });
 }
 
@@ -2232,6 +2345,40 @@ hipError_t hq_launch_tbuild(const hq_geom &G, const int2 *tiles, int ntiles, hip
 static unsigned grid_for(size_t total) {
   size_t blocks = (total + 255) / 256;
   return (unsigned)(blocks > 65536 ? 65536 : (blocks ? blocks : 1));
+}
+
+hipError_t hq_launch_tbuild_wide(const hq_geom &G, const int2 *tiles, int ntiles, int drop_last, hipStream_t st) {
+  return hq_dispatch(G.nb, G.ib, [&](auto nbc, auto wbc) {
+    HQ_NBWB
+    (void)WB;
+    hipLaunchKernelGGL((hq::tbuild_wide_kernel<NB>), dim3((NB + G.ib - 1) / G.ib, ntiles), dim3(HQ_THREADS), 0,
+                       st, G, tiles, drop_last);
+  });
+}
+
+hipError_t hq_launch_tbuild_wide_tile(int nb, int ib, const double *V, const double *tau, double *T, int diag,
+                                      hipStream_t st) {
+  return hq_dispatch(nb, ib, [&](auto nbc, auto wbc) {
+    HQ_NBWB
+    (void)WB;
+    hipLaunchKernelGGL((hq::tbuild_wide_tile_kernel<NB>), dim3((NB + ib - 1) / ib), dim3(HQ_THREADS), 0, st, V,
+                       tau, T, ib, diag);
+  });
+}
+
+hipError_t hq_launch_pad_zero(double *A, int MT, int NT, int nb, int m, int n, hipStream_t st) {
+  hipLaunchKernelGGL(hq::pad_zero_kernel, dim3(grid_for((size_t)(MT + NT) * nb * nb)), dim3(256), 0, st, A, MT,
+                     NT, nb, m, n);
+  return hipGetLastError();
+}
+
+// Read the device fault word of the current device (after the work that may
+// have raised it has completed) and clear it.
+hipError_t hq_fault_take(int *value) {
+  hipError_t e = hipMemcpyFromSymbol(value, HIP_SYMBOL(hq::g_fault), sizeof(int), 0, hipMemcpyDeviceToHost);
+  if (e != hipSuccess || *value == 0) return e;
+  const int zero = 0;
+  return hipMemcpyToSymbol(HIP_SYMBOL(hq::g_fault), &zero, sizeof(int), 0, hipMemcpyHostToDevice);
 }
 
 hipError_t hq_launch_identity(double *X, int MT, int XNT, int nb, int full, hipStream_t st) {

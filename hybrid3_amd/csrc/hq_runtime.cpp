@@ -16,12 +16,14 @@
 #include <vector>
 
 #include "../../include/Hybrid_tile_QR.h"
+#include "../../include/core_blas.h"
 #include "../../include/hybrid3_dev.h"
 #include "hq_internal.h"
 
 // launch wrappers (hq_factor.hip)
 hipError_t hq_launch_sched(const hq_geom &G, const hq_geom *geoms, const hq_task *tasks, int ntasks,
-                           int *flags, int epoch, int *counter, int grid, long long *trace, hipStream_t st, int *epoch_dev = nullptr);
+                           int *flags, int epoch, int *counter, int grid, long long *trace, hipStream_t st, int *epoch_dev = nullptr,
+                           unsigned wait_limit = 1u << 22);
 hipError_t hq_launch_task(const hq_geom &G, const hq_task &tk, hipStream_t st);
 hipError_t hq_launch_geqrt(int nb, int ib, double *A, double *T, double *TAU, hipStream_t st);
 hipError_t hq_launch_tsqrt(int nb, int ib, double *A1, double *A2, double *T, double *TAU,
@@ -36,6 +38,11 @@ hipError_t hq_launch_batched_layout(int op, const hq_bmat *mats, int count, int 
 hipError_t hq_launch_tbuild(const hq_geom &G, const int2 *tiles, int ntiles, hipStream_t st);
 hipError_t hq_launch_identity(double *X, int MT, int XNT, int nb, int full, hipStream_t st);
 hipError_t hq_launch_zero_reflectors(double *A, int MT, int NT, int nb, hipStream_t st);
+hipError_t hq_launch_tbuild_wide(const hq_geom &G, const int2 *tiles, int ntiles, int drop_last, hipStream_t st);
+hipError_t hq_launch_tbuild_wide_tile(int nb, int ib, const double *V, const double *tau, double *T, int diag,
+                                      hipStream_t st);
+hipError_t hq_launch_pad_zero(double *A, int MT, int NT, int nb, int m, int n, hipStream_t st);
+hipError_t hq_fault_take(int *value);
 
 // homegrown tile_kernel.h API and verification helpers (hq_homegrown.hip)
 hipError_t hqh_launch_dgeqrt(int row, int col, int block, double *A, int lda, double *TAU,
@@ -78,14 +85,21 @@ int ok() {
       return set_error(HQ_ERR_DEVICE, "%s failed: %s", #call, hipGetErrorString(e_));     \
   } while (0)
 
+bool nb_ok(int nb) { return nb == 64 || nb == 128 || nb == 256; }
+
 bool geometry_ok(int MT, int NT, int nb, int ib) {
   if (MT < 1 || NT < 1) return set_error(HQ_ERR_ARG, "tile grid %d x %d is empty", MT, NT), false;
-  if (nb != 64 && nb != 128 && nb != 256)
-    return set_error(HQ_ERR_ARG, "nb=%d unsupported (64, 128 or 256)", nb), false;
-  if (ib < 1 || ib > HQ_IBMAX)
-    return set_error(HQ_ERR_ARG, "ib=%d unsupported (1..%d)", ib, HQ_IBMAX), false;
+  if (!nb_ok(nb)) return set_error(HQ_ERR_ARG, "nb=%d unsupported (64, 128 or 256)", nb), false;
+  if (ib < 1 || ib > nb) return set_error(HQ_ERR_ARG, "ib=%d unsupported (1..nb=%d)", ib, nb), false;
   return true;
 }
+
+// Inner block width the kernels run with.  The panels factor at most
+// HQ_IBMAX columns at a time; R, V and TAU do not depend on the inner
+// blocking (every ib applies the same reflectors, only grouped differently,
+// so results agree to rounding), and T for a wider ib is rebuilt from V and
+// TAU (dlarft per ib block) where a caller asks for it.
+int ibi(int ib) { return ib < HQ_IBMAX ? ib : HQ_IBMAX; }
 
 // ---------------------------------------------------------------- DAG
 // Canonical task enumeration (step-major, a topological order):
@@ -326,13 +340,34 @@ struct CachedSched {
   int grid = 0;
 };
 
+// A stream and a scratch buffer for one host-pointer tile-kernel call at a
+// time; calls from different host threads take different lanes, so
+// tile_kernel.h calls are reentrant as the reference's are (it calls them
+// from inside `omp for` loops, Hybrid_dorgqr.c:92-108,124-137).
+struct Lane {
+  hipStream_t stream = nullptr;
+  DevBuf buf;
+};
+
 struct Context {
   int device = -1;
   int num_cu = 0;
   hipStream_t stream = nullptr;
-  DevBuf flags, counter, T, A, TAU, X, hg;
+  DevBuf flags, counter, T, A, TAU, X, T1;
   DevBuf W, VD;  // W = V T16 and unit lower V of the diagonal tiles (FP64 factorisation)
   int epoch = 0;
+  // Device-pointer calls share this context's buffers (claim counter,
+  // flags, T / W / VD scratch, batch tables).  Each call waits for `busy`
+  // (recorded after the previous call's launches) on its own stream before
+  // enqueueing, so calls on different streams run one after another on the
+  // device instead of overwriting each other's scratch.
+  hipEvent_t busy = nullptr;
+  unsigned wait_limit = 1u << 22;  // spins of a scheduler dependency wait before it gives up
+  // host-pointer entry points (Hybrid_*) use A / TAU / X and `stream`: one
+  // such call per device at a time
+  std::mutex host_mu;
+  std::mutex lane_mu;
+  std::vector<std::unique_ptr<Lane>> lanes;  // idle lanes
   std::map<std::tuple<int, int, int, int, int>, std::unique_ptr<CachedSched>> scheds;
   std::map<std::pair<int, int>, std::unique_ptr<DevBuf>> tilelists;  // (MT, NT) -> int2 list
   // batches: key = (nb, ib, MT_0, NT_0, MT_1, NT_1, ...)
@@ -355,10 +390,39 @@ int get_ctx(Context **out) {
     c->device = dev;
     HQ_HIP(hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, dev));
     HQ_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HQ_HIP(hipEventCreateWithFlags(&c->busy, hipEventDisableTiming));
     if (c->counter.reserve(256) != HQ_OK) return g_err;
     HQ_HIP(hipMemset(c->counter.p, 0, 256));
   }
   *out = c.get();
+  return HQ_OK;
+}
+
+// Order this call's launches on `st` after the previous call's (see busy).
+int order_in(Context *c, hipStream_t st) {
+  HQ_HIP(hipStreamWaitEvent(st, c->busy, 0));
+  return HQ_OK;
+}
+int order_out(Context *c, hipStream_t st) {
+  HQ_HIP(hipEventRecord(c->busy, st));
+  return HQ_OK;
+}
+
+// After the work of a call has completed: a dependency wait of the scheduler
+// that timed out (counter[1], the task index + 1) or a bounded wait inside a
+// task that gave up (the device fault word) is reported and cleared, so the
+// next call starts clean.
+int take_device_faults(int *counter) {
+  int st[2] = {0, 0};
+  HQ_HIP(hipMemcpy(st, counter, sizeof st, hipMemcpyDeviceToHost));
+  int fault = 0;
+  HQ_HIP(hq_fault_take(&fault));
+  if (st[1] != 0) {
+    HQ_HIP(hipMemset(counter + 1, 0, sizeof(int)));
+    return set_error(HQ_ERR_DEVICE, "scheduler dependency wait timed out at task %d", st[1] - 1);
+  }
+  if (fault != 0)
+    return set_error(HQ_ERR_DEVICE, "a panel's reflector wait timed out (awaited count %d)", fault - 1);
   return HQ_OK;
 }
 
@@ -444,6 +508,8 @@ int ensure_T(Context *c, int MT, int NT, int nb, int ib, double **T) {
   return HQ_OK;
 }
 
+int tile_list(Context *c, int MT, int NT, const int2 **out, int *n);
+
 int tile_qr_dev_impl(int MT, int NT, int nb, int ib, double *A, double *TAU, double *T,
                      void *stream, bool serial, long long *trace = nullptr, int prec = 0) {
   if (!geometry_ok(MT, NT, nb, ib)) return g_err;
@@ -452,9 +518,15 @@ int tile_qr_dev_impl(int MT, int NT, int nb, int ib, double *A, double *TAU, dou
   Context *c;
   if (get_ctx(&c) != HQ_OK) return g_err;
   hipStream_t st = pick_stream(c, stream);
-  if (ensure_T(c, MT, NT, nb, ib, &T) != HQ_OK) return g_err;
-  hq_geom G{A, TAU, T, MT, NT, nb, ib, A, NT, prec};
-  if (!prec && nb >= 128 && ib % 16 == 0) {
+  // the run itself uses the inner block width; a caller's T for a wider ib
+  // is rebuilt from V and TAU after it
+  const int ik = ibi(ib);
+  double *Tout = ik != ib ? T : nullptr;
+  if (ik != ib) T = nullptr;
+  if (ensure_T(c, MT, NT, nb, ik, &T) != HQ_OK) return g_err;
+  if (order_in(c, st) != HQ_OK) return g_err;
+  hq_geom G{A, TAU, T, MT, NT, nb, ik, A, NT, prec};
+  if (!prec && nb >= 128 && ik % 16 == 0) {
     // the GEQRT panels also write W = V T16 and the unit lower V of the
     // diagonal tiles; LARFB then needs no T multiplication between passes
     const size_t tiles = sizeof(double) * (size_t)nb * nb * std::min(MT, NT);
@@ -464,15 +536,23 @@ int tile_qr_dev_impl(int MT, int NT, int nb, int ib, double *A, double *TAU, dou
   }
   if (serial) {
     std::vector<hq_task> tl;
-    host_schedule(MT, NT, nb, ib, c->num_cu, tl);
+    host_schedule(MT, NT, nb, ik, c->num_cu, tl);
     for (const hq_task &tk : tl) HQ_HIP(hq_launch_task(G, tk, st));
-    return ok();
+  } else {
+    CachedSched *cs;
+    if (get_schedule(c, MT, NT, nb, ik, &cs) != HQ_OK) return g_err;
+    c->epoch++;
+    HQ_HIP(hq_launch_sched(G, nullptr, (const hq_task *)cs->tasks.p, cs->ntasks, (int *)c->flags.p,
+                           c->epoch, (int *)c->counter.p, cs->grid, trace, st, nullptr, c->wait_limit));
   }
-  CachedSched *cs;
-  if (get_schedule(c, MT, NT, nb, ib, &cs) != HQ_OK) return g_err;
-  c->epoch++;
-  HQ_HIP(hq_launch_sched(G, nullptr, (const hq_task *)cs->tasks.p, cs->ntasks, (int *)c->flags.p,
-                         c->epoch, (int *)c->counter.p, cs->grid, trace, st));
+  if (Tout) {
+    const int2 *tiles;
+    int n;
+    if (tile_list(c, MT, NT, &tiles, &n) != HQ_OK) return g_err;
+    hq_geom Go{A, TAU, Tout, MT, NT, nb, ib, nullptr, 0};
+    HQ_HIP(hq_launch_tbuild_wide(Go, tiles, n, 0, st));
+  }
+  if (order_out(c, st) != HQ_OK) return g_err;
   return ok();
 }
 
@@ -499,11 +579,18 @@ int build_t_impl(Context *c, int MT, int NT, int nb, int ib, const double *RY, c
   int n;
   if (tile_list(c, MT, NT, &tiles, &n) != HQ_OK) return g_err;
   hq_geom G{(double *)RY, (double *)TAU, T, MT, NT, nb, ib, nullptr, 0};
-  HQ_HIP(hq_launch_tbuild(G, tiles, n, st));
+  if (ib > HQ_IBMAX)
+    HQ_HIP(hq_launch_tbuild_wide(G, tiles, n, 1, st));
+  else
+    HQ_HIP(hq_launch_tbuild(G, tiles, n, st));
   return HQ_OK;
 }
 
 // X (MT x XNT tiles) <- Q^T X, Q from the reflectors of RY (MT x NT tiles).
+// A caller's T is applied with its own ib (the update kernels take the 16 x 16
+// diagonal blocks of T for any ib); without T (or with nb = 64, whose update
+// kernel stages whole ib blocks of at most HQ_IBMAX), T is rebuilt from RY
+// and TAU with the inner block width.
 int apply_qt_impl(int MT, int NT, int nb, int ib, const double *RY, const double *TAU,
                   const double *T, double *X, int XNT, void *stream) {
   if (!geometry_ok(MT, NT, nb, ib)) return g_err;
@@ -512,17 +599,22 @@ int apply_qt_impl(int MT, int NT, int nb, int ib, const double *RY, const double
   Context *c;
   if (get_ctx(&c) != HQ_OK) return g_err;
   hipStream_t st = pick_stream(c, stream);
+  if (order_in(c, st) != HQ_OK) return g_err;
   double *Tw = (double *)T;
+  int iq = ib;
+  if (Tw && nb < 128 && ib > HQ_IBMAX) Tw = nullptr;
   if (!Tw) {
-    if (ensure_T(c, MT, NT, nb, ib, &Tw) != HQ_OK) return g_err;
-    if (build_t_impl(c, MT, NT, nb, ib, RY, TAU, Tw, st) != HQ_OK) return g_err;
+    iq = ibi(ib);
+    if (ensure_T(c, MT, NT, nb, iq, &Tw) != HQ_OK) return g_err;
+    if (build_t_impl(c, MT, NT, nb, iq, RY, TAU, Tw, st) != HQ_OK) return g_err;
   }
   CachedSched *cs;
-  if (get_schedule(c, MT, NT, nb, ib, &cs, XNT) != HQ_OK) return g_err;
-  hq_geom G{(double *)RY, (double *)TAU, Tw, MT, NT, nb, ib, X, XNT};
+  if (get_schedule(c, MT, NT, nb, iq, &cs, XNT) != HQ_OK) return g_err;
+  hq_geom G{(double *)RY, (double *)TAU, Tw, MT, NT, nb, iq, X, XNT};
   c->epoch++;
   HQ_HIP(hq_launch_sched(G, nullptr, (const hq_task *)cs->tasks.p, cs->ntasks, (int *)c->flags.p,
-                         c->epoch, (int *)c->counter.p, cs->grid, nullptr, st));
+                         c->epoch, (int *)c->counter.p, cs->grid, nullptr, st, nullptr, c->wait_limit));
+  if (order_out(c, st) != HQ_OK) return g_err;
   return ok();
 }
 
@@ -581,6 +673,7 @@ int batch_args_ok(int count, int nb, int ib, const int *MT, const int *NT, doubl
                   double *const *TAU) {
   if (count < 1 || count > 65535) return set_error(HQ_ERR_ARG, "batch of %d matrices (1..65535)", count);
   if (!MT || !NT || !A || !TAU) return set_error(HQ_ERR_ARG, "MT, NT, A and TAU arrays are required");
+  if (ib > HQ_IBMAX) return set_error(HQ_ERR_ARG, "batches take ib <= %d (ib=%d)", HQ_IBMAX, ib);
   for (int i = 0; i < count; i++) {
     if (!geometry_ok(MT[i], NT[i], nb, ib)) return g_err;
     if (!A[i] || !TAU[i]) return set_error(HQ_ERR_ARG, "matrix %d: A and TAU must be device pointers", i);
@@ -629,6 +722,7 @@ int batched_qr_impl(int count, int nb, int ib, const int *MT, const int *NT, dou
   Context *c;
   if (get_ctx(&c) != HQ_OK) return g_err;
   hipStream_t st = pick_stream(c, stream);
+  if (order_in(c, st) != HQ_OK) return g_err;
   std::vector<hq_geom> g;
   if (batch_geoms(count, nb, ib, MT, NT, A, TAU, T, c->Tbatch, c->Wbatch, c->VDbatch, g) != HQ_OK)
     return g_err;
@@ -638,7 +732,9 @@ int batched_qr_impl(int count, int nb, int ib, const int *MT, const int *NT, dou
   HQ_HIP(hipMemcpyAsync(c->geoms.p, g.data(), sizeof(hq_geom) * (size_t)count, hipMemcpyHostToDevice, st));
   c->epoch++;
   HQ_HIP(hq_launch_sched(g[0], (const hq_geom *)c->geoms.p, (const hq_task *)cs->tasks.p, cs->ntasks,
-                         (int *)c->flags.p, c->epoch, (int *)c->counter.p, cs->grid, nullptr, st));
+                         (int *)c->flags.p, c->epoch, (int *)c->counter.p, cs->grid, nullptr, st, nullptr,
+                         c->wait_limit));
+  if (order_out(c, st) != HQ_OK) return g_err;
   return ok();
 }
 
@@ -717,9 +813,11 @@ int batched_layout_impl(int op, int count, int nb, const int *m, const int *n, c
   Context *c;
   if (get_ctx(&c) != HQ_OK) return g_err;
   hipStream_t st = pick_stream(c, stream);
+  if (order_in(c, st) != HQ_OK) return g_err;
   if (c->bmats.reserve(sizeof(hq_bmat) * (size_t)count) != HQ_OK) return g_err;
   HQ_HIP(hipMemcpyAsync(c->bmats.p, v.data(), sizeof(hq_bmat) * (size_t)count, hipMemcpyHostToDevice, st));
   HQ_HIP(hq_launch_batched_layout(op, (const hq_bmat *)c->bmats.p, count, nb, max_tiles, st));
+  if (order_out(c, st) != HQ_OK) return g_err;
   return ok();
 }
 
@@ -756,6 +854,20 @@ int hq_tile_qr_dev(int MT, int NT, int nb, int ib, double *A, double *TAU, doubl
   return tile_qr_dev_impl(MT, NT, nb, ib, A, TAU, T, stream, false);
 }
 
+int hq_tile_qr_mn_dev(int m, int n, int nb, int ib, double *A, double *TAU, double *T, void *stream) {
+  if (m < 1 || n < 1 || !nb_ok(nb)) return set_error(HQ_ERR_ARG, "bad m=%d n=%d nb=%d", m, n, nb);
+  if (!A || !TAU) return set_error(HQ_ERR_ARG, "A and TAU must be device pointers");
+  const int MT = (m + nb - 1) / nb, NT = (n + nb - 1) / nb;
+  if (!geometry_ok(MT, NT, nb, ib)) return g_err;
+  Context *c;
+  if (get_ctx(&c) != HQ_OK) return g_err;
+  hipStream_t st = pick_stream(c, stream);
+  // padding entries are zero, so every reflector is zero in the padded rows
+  // and the padded columns get tau = 0 (their reflector is the identity)
+  if (m % nb || n % nb) HQ_HIP(hq_launch_pad_zero(A, MT, NT, nb, m, n, st));
+  return tile_qr_dev_impl(MT, NT, nb, ib, A, TAU, T, stream, false);
+}
+
 int hq_tile_qr_mixed_dev(int MT, int NT, int nb, int ib, double *A, double *TAU, double *T,
                          void *stream) {
   return tile_qr_dev_impl(MT, NT, nb, ib, A, TAU, T, stream, false, nullptr, 1);
@@ -777,7 +889,7 @@ int hq_schedule_tasks(int MT, int NT, int nb, int ib, int *out, int capacity) {
   Context *c;
   if (get_ctx(&c) != HQ_OK) return g_err;
   CachedSched *cs;
-  if (get_schedule(c, MT, NT, nb, ib, &cs) != HQ_OK) return g_err;
+  if (get_schedule(c, MT, NT, nb, ibi(ib), &cs) != HQ_OK) return g_err;
   if (capacity < cs->ntasks * 5) return set_error(HQ_ERR_ARG, "need capacity >= 5*ntasks");
   std::vector<hq_task> tl(cs->ntasks);
   HQ_HIP(hipMemcpy(tl.data(), cs->tasks.p, sizeof(hq_task) * tl.size(), hipMemcpyDeviceToHost));
@@ -795,7 +907,7 @@ int hq_dag_host(int MT, int NT, int nb, int ib, int XNT, int P, int *tasks, int 
                 int capacity, int *ntasks) {
   if (!geometry_ok(MT, NT, nb, ib)) return g_err;
   if (P < 1) return set_error(HQ_ERR_ARG, "P must be >= 1");
-  DagBuilder b(MT, NT, nb, ib, XNT >= 1, XNT);
+  DagBuilder b(MT, NT, nb, ibi(ib), XNT >= 1, XNT);
   b.build();
   const int n = (int)b.t.size();
   if (ntasks) *ntasks = n;
@@ -824,6 +936,7 @@ int hq_wall_clock_khz(int *khz) {
 
 int hq_schedule_info(int MT, int NT, int nb, int ib, int *ntasks, int *grid) {
   if (!geometry_ok(MT, NT, nb, ib)) return g_err;
+  ib = ibi(ib);
   int KT = std::min(MT, NT), S = (nb + HQ_SLAB - 1) / HQ_SLAB;
   // tasks of one panel QR (GEQRT or TSQRT): a panel per ib block and a
   // trailing task per slab right of the block
@@ -847,26 +960,50 @@ int hq_schedule_info(int MT, int NT, int nb, int ib, int *ntasks, int *grid) {
   return ok();
 }
 
-#define HQ_TILE_ARGS_OK(nb, ib)                                                      \
-  if ((nb != 64 && nb != 128 && nb != 256) || ib < 1 || ib > HQ_IBMAX)              \
-    return set_error(HQ_ERR_ARG, "unsupported nb=%d / ib=%d", nb, ib);
+// Single-tile kernels: any 1 <= ib <= nb.  The panels (GEQRT / TSQRT) run
+// with the inner block width and T for a wider ib is rebuilt from V and TAU;
+// the updates take T with its own ib, except at nb = 64, whose update stages
+// whole ib blocks (ib <= HQ_IBMAX there).
+static int tile_args_ok(int nb, int ib, bool consumer) {
+  if (!nb_ok(nb) || ib < 1 || ib > nb) return set_error(HQ_ERR_ARG, "unsupported nb=%d / ib=%d", nb, ib);
+  if (consumer && nb < 128 && ib > HQ_IBMAX)
+    return set_error(HQ_ERR_ARG, "nb=%d takes ib <= %d in LARFB / SSRFB (ib=%d)", nb, HQ_IBMAX, ib);
+  return HQ_OK;
+}
+
+// GEQRT (ts = false: A1 = the tile) or TSQRT (A1 on A2) of one tile.
+static int panel_tile(int nb, int ib, double *A1, double *A2, double *T, double *TAU, void *stream, bool ts) {
+  if (tile_args_ok(nb, ib, false) != HQ_OK) return g_err;
+  if (!A1 || !T || !TAU || (ts && !A2)) return set_error(HQ_ERR_ARG, "tile, T and TAU must be device pointers");
+  Context *c;
+  if (get_ctx(&c) != HQ_OK) return g_err;
+  hipStream_t st = pick_stream(c, stream);
+  const int ik = ibi(ib);
+  double *Tk = T;
+  if (ik != ib) {  // inner T in the context's scratch, then T for ib
+    if (order_in(c, st) != HQ_OK) return g_err;
+    if (c->T1.reserve(sizeof(double) * (size_t)nb * ik) != HQ_OK) return g_err;
+    Tk = (double *)c->T1.p;
+  }
+  if (ts)
+    HQ_HIP(hq_launch_tsqrt(nb, ik, A1, A2, Tk, TAU, st));
+  else
+    HQ_HIP(hq_launch_geqrt(nb, ik, A1, Tk, TAU, st));
+  if (ik != ib) {
+    HQ_HIP(hq_launch_tbuild_wide_tile(nb, ib, ts ? A2 : A1, TAU, T, ts ? 0 : 1, st));
+    if (order_out(c, st) != HQ_OK) return g_err;
+  }
+  return ok();
+}
 
 int hq_dgeqrt_dev(int nb, int ib, double *A, double *T, double *TAU, void *stream) {
-  HQ_TILE_ARGS_OK(nb, ib);
-  Context *c;
-  if (get_ctx(&c) != HQ_OK) return g_err;
-  HQ_HIP(hq_launch_geqrt(nb, ib, A, T, TAU, pick_stream(c, stream)));
-  return ok();
+  return panel_tile(nb, ib, A, nullptr, T, TAU, stream, false);
 }
 int hq_dtsqrt_dev(int nb, int ib, double *A1, double *A2, double *T, double *TAU, void *stream) {
-  HQ_TILE_ARGS_OK(nb, ib);
-  Context *c;
-  if (get_ctx(&c) != HQ_OK) return g_err;
-  HQ_HIP(hq_launch_tsqrt(nb, ib, A1, A2, T, TAU, pick_stream(c, stream)));
-  return ok();
+  return panel_tile(nb, ib, A1, A2, T, TAU, stream, true);
 }
 int hq_dormqr_dev(int nb, int ib, const double *V, const double *T, double *C, void *stream) {
-  HQ_TILE_ARGS_OK(nb, ib);
+  if (tile_args_ok(nb, ib, true) != HQ_OK) return g_err;
   Context *c;
   if (get_ctx(&c) != HQ_OK) return g_err;
   HQ_HIP(hq_launch_larfb(nb, ib, V, T, C, pick_stream(c, stream)));
@@ -874,7 +1011,7 @@ int hq_dormqr_dev(int nb, int ib, const double *V, const double *T, double *C, v
 }
 int hq_dtsmqr_dev(int nb, int ib, double *A1, double *A2, const double *V, const double *T,
                   void *stream) {
-  HQ_TILE_ARGS_OK(nb, ib);
+  if (tile_args_ok(nb, ib, true) != HQ_OK) return g_err;
   Context *c;
   if (get_ctx(&c) != HQ_OK) return g_err;
   HQ_HIP(hq_launch_ssrfb(nb, ib, A1, A2, V, T, pick_stream(c, stream)));
@@ -943,12 +1080,7 @@ int hq_batched_plan_check(int plan, void *stream) {
   BatchPlan *p = find_plan(c, plan);
   if (!p) return set_error(HQ_ERR_ARG, "no batch plan %d", plan);
   HQ_HIP(hipStreamSynchronize(pick_stream(c, stream)));
-  int st[2] = {0, 0};
-  HQ_HIP(hipMemcpy(st, p->counter.p, sizeof st, hipMemcpyDeviceToHost));
-  if (st[1] != 0) {
-    HQ_HIP(hipMemset((int *)p->counter.p + 1, 0, sizeof(int)));
-    return set_error(HQ_ERR_DEVICE, "scheduler dependency wait timed out at task %d", st[1] - 1);
-  }
+  if (take_device_faults((int *)p->counter.p) != HQ_OK) return g_err;
   return ok();
 }
 
@@ -985,17 +1117,20 @@ int hq_fill_uniform_dev(int MT, int NT, int nb, uint64_t seed, double *A, void *
   return ok();
 }
 
+int hq_set_wait_limit(unsigned spins) {
+  Context *c;
+  if (get_ctx(&c) != HQ_OK) return g_err;
+  c->wait_limit = spins ? spins : 1u << 22;
+  return ok();
+}
+
 int hq_sync(void *stream) {
   Context *c;
   if (get_ctx(&c) != HQ_OK) return g_err;
   HQ_HIP(hipStreamSynchronize(pick_stream(c, stream)));
-  // counter[1] is raised by the scheduler when a dependency wait timed out
-  int st[2] = {0, 0};
-  HQ_HIP(hipMemcpy(st, c->counter.p, sizeof st, hipMemcpyDeviceToHost));
-  if (st[1] != 0) {
-    HQ_HIP(hipMemset((int *)c->counter.p + 1, 0, sizeof(int)));
-    return set_error(HQ_ERR_DEVICE, "scheduler dependency wait timed out at task %d", st[1] - 1);
-  }
+  // counter[1] is raised by the scheduler when a dependency wait timed out,
+  // the fault word by a panel wait that gave up
+  if (take_device_faults((int *)c->counter.p) != HQ_OK) return g_err;
   return ok();
 }
 
@@ -1045,6 +1180,7 @@ void Hybrid_tile_QR(mat_stat_t *status, double *A, double *TAU, int my_rank, int
     return;
   }
   const size_t nA = (size_t)MT * NT * nb * nb, nT = (size_t)MT * NT * nb;
+  std::lock_guard<std::mutex> lk(c->host_mu);
   auto run = [&]() -> int {
     if (c->A.reserve(nA * sizeof(double)) != HQ_OK) return g_err;
     if (c->TAU.reserve(nT * sizeof(double)) != HQ_OK) return g_err;
@@ -1056,6 +1192,7 @@ void Hybrid_tile_QR(mat_stat_t *status, double *A, double *TAU, int my_rank, int
     HQ_HIP(hipMemcpyAsync(A, c->A.p, nA * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HQ_HIP(hipMemcpyAsync(TAU, c->TAU.p, nT * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HQ_HIP(hipStreamSynchronize(c->stream));
+    if (take_device_faults((int *)c->counter.p) != HQ_OK) return g_err;
     return ok();
   };
   run();
@@ -1079,6 +1216,7 @@ static int host_q_call(mat_stat_t *status, double *X, double *RY, double *TAU, i
   Context *c;
   if (get_ctx(&c) != HQ_OK) return g_err;
   const size_t nA = (size_t)MT * NT * nb * nb, nT = (size_t)MT * NT * nb;
+  std::lock_guard<std::mutex> lk(c->host_mu);
   if (c->A.reserve(nA * sizeof(double)) != HQ_OK) return g_err;
   if (c->TAU.reserve(nT * sizeof(double)) != HQ_OK) return g_err;
   if (c->X.reserve(nA * sizeof(double)) != HQ_OK) return g_err;
@@ -1090,6 +1228,7 @@ static int host_q_call(mat_stat_t *status, double *X, double *RY, double *TAU, i
   HQ_HIP(hipMemcpyAsync(X, dX, nA * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   if (ry_out) HQ_HIP(hipMemcpyAsync(RY, dRY, nA * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HQ_HIP(hipStreamSynchronize(c->stream));
+  if (take_device_faults((int *)c->counter.p) != HQ_OK) return g_err;
   return ok();
 }
 
@@ -1143,27 +1282,58 @@ struct HArr {
   size_t n() const { return cols <= 0 || rows <= 0 ? 0 : (size_t)(cols - 1) * ld + rows; }
 };
 
+// Lane pool of a context: take an idle lane (or make one), give it back.
+Lane *take_lane(Context *c) {
+  {
+    std::lock_guard<std::mutex> lk(c->lane_mu);
+    if (!c->lanes.empty()) {
+      Lane *l = c->lanes.back().release();
+      c->lanes.pop_back();
+      return l;
+    }
+  }
+  std::unique_ptr<Lane> l(new Lane());
+  if (hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error(HQ_ERR_DEVICE, "hipStreamCreateWithFlags failed");
+    return nullptr;
+  }
+  return l.release();
+}
+void give_lane(Context *c, Lane *l) {
+  std::lock_guard<std::mutex> lk(c->lane_mu);
+  c->lanes.emplace_back(l);
+}
+
 template <typename Fn>
 int homegrown_call(const char *fn_name, std::vector<HArr> arrs, size_t work_elems, Fn &&fn) {
   Context *c;
   if (get_ctx(&c) != HQ_OK) return g_err;
+  Lane *lane = take_lane(c);
+  if (!lane) return g_err;
+  struct Back {  // the lane returns to the pool on every exit path
+    Context *c;
+    Lane *l;
+    ~Back() { give_lane(c, l); }
+  } back{c, lane};
   size_t total = work_elems;
   for (auto &a : arrs) total += (a.n() + 31) & ~(size_t)31;
-  if (c->hg.reserve(total * sizeof(double)) != HQ_OK) return g_err;
-  double *base = (double *)c->hg.p;
+  if (lane->buf.reserve(total * sizeof(double)) != HQ_OK) return g_err;
+  double *base = (double *)lane->buf.p;
+  hipStream_t st = lane->stream;
   size_t off = 0;
   for (auto &a : arrs) {
     a.d = base + off;
     off += (a.n() + 31) & ~(size_t)31;
     if (a.in && a.n())
-      HQ_HIP(hipMemcpyAsync(a.d, a.h, a.n() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      HQ_HIP(hipMemcpyAsync(a.d, a.h, a.n() * sizeof(double), hipMemcpyHostToDevice, st));
   }
   double *work = base + off;
-  HQ_HIP(fn(arrs, work, c->stream));
+  HQ_HIP(fn(arrs, work, st));
   for (auto &a : arrs)
     if (a.out && a.n())
-      HQ_HIP(hipMemcpyAsync(a.h, a.d, a.n() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HQ_HIP(hipStreamSynchronize(c->stream));
+      HQ_HIP(hipMemcpyAsync(a.h, a.d, a.n() * sizeof(double), hipMemcpyDeviceToHost, st));
+  HQ_HIP(hipStreamSynchronize(st));
   (void)fn_name;
   return ok();
 }
@@ -1309,6 +1479,170 @@ void Hybrid_dgemm(mat_stat_t *status, double *A, double *B, double *Cm, int my_r
                    return hqh_launch_tile_gemm_tn(a[0].d, a[1].d, a[2].d, MT, NT, nb, st);
                  });
   report("Hybrid_dgemm");
+}
+
+}  // extern "C"
+
+// =====================================================================
+// PLASMA core_blas shim: CORE_dgeqrt / CORE_dtsqrt / CORE_dormqr / CORE_dtsmqr
+// with PLASMA 2.x argument lists, as the reference calls them
+// (Hybrid_tile_QR.c:985-990,1135-1141,1276-1283,1356-1363), on the GPU.
+// Host pointers with any leading dimension; every array is packed to the
+// kernels' layout (ld = nb, T ld = ib) on a lane's stream and unpacked after.
+// Square nb x nb tiles (nb in {64, 128, 256}), K = nb, side = Left,
+// trans = Trans (the only forms the reference uses).  Return values follow
+// PLASMA: 0 on success, -i when argument i is illegal or outside what this
+// shim supports (message in hq_last_error_message()).
+// =====================================================================
+namespace {
+
+struct PArr {
+  double *h;
+  int rows, cols, ld;
+  bool in, out;
+  double *d = nullptr;
+};
+
+template <typename Fn>
+int packed_call(std::vector<PArr> arrs, Fn &&fn) {
+  Context *c;
+  if (get_ctx(&c) != HQ_OK) return g_err;
+  Lane *lane = take_lane(c);
+  if (!lane) return g_err;
+  struct Back {
+    Context *c;
+    Lane *l;
+    ~Back() { give_lane(c, l); }
+  } back{c, lane};
+  size_t total = 0;
+  for (auto &a : arrs) total += ((size_t)a.rows * a.cols + 31) & ~(size_t)31;
+  if (lane->buf.reserve(total * sizeof(double)) != HQ_OK) return g_err;
+  hipStream_t st = lane->stream;
+  size_t off = 0;
+  for (auto &a : arrs) {
+    a.d = (double *)lane->buf.p + off;
+    off += ((size_t)a.rows * a.cols + 31) & ~(size_t)31;
+    if (a.in)
+      HQ_HIP(hipMemcpy2DAsync(a.d, sizeof(double) * a.rows, a.h, sizeof(double) * a.ld, sizeof(double) * a.rows,
+                              a.cols, hipMemcpyHostToDevice, st));
+  }
+  if (fn(arrs, st) != HQ_OK) return g_err;
+  for (auto &a : arrs)
+    if (a.out)
+      HQ_HIP(hipMemcpy2DAsync(a.h, sizeof(double) * a.ld, a.d, sizeof(double) * a.rows, sizeof(double) * a.rows,
+                              a.cols, hipMemcpyDeviceToHost, st));
+  HQ_HIP(hipStreamSynchronize(st));
+  if (take_device_faults((int *)c->counter.p) != HQ_OK) return g_err;
+  return ok();
+}
+
+// -i for the first failing check, with the message kept for hq_last_error
+int core_bad(int i, const char *fn, const char *what) {
+  set_error(HQ_ERR_ARG, "%s: argument %d: %s", fn, i, what);
+  return -i;
+}
+
+}  // namespace
+
+extern "C" {
+
+int CORE_dgeqrt(int M, int N, int IB, double *A, int LDA, double *T, int LDT, double *TAU, double *WORK) {
+  (void)WORK;
+  const char *fn = "CORE_dgeqrt";
+  if (M < 0) return core_bad(1, fn, "M < 0");
+  if (N < 0) return core_bad(2, fn, "N < 0");
+  if (IB < 0 || (IB == 0 && M > 0 && N > 0)) return core_bad(3, fn, "illegal IB");
+  if (LDA < (M > 1 ? M : 1)) return core_bad(5, fn, "LDA < max(1, M)");
+  if (LDT < (IB > 1 ? IB : 1)) return core_bad(7, fn, "LDT < max(1, IB)");
+  if (M == 0 || N == 0 || IB == 0) return (ok(), 0);
+  if (M != N || !nb_ok(M)) return core_bad(1, fn, "square tiles of 64, 128 or 256 only");
+  if (IB > N) return core_bad(3, fn, "IB > N");
+  const int nb = M;
+  const int rc = packed_call({{A, nb, nb, LDA, true, true}, {T, IB, nb, LDT, false, true}, {TAU, nb, 1, nb, false, true}},
+                             [&](std::vector<PArr> &a, hipStream_t st) {
+                               return hq_dgeqrt_dev(nb, IB, a[0].d, a[1].d, a[2].d, st);
+                             });
+  return rc == HQ_OK ? 0 : -1000 - rc;
+}
+
+int CORE_dtsqrt(int M, int N, int IB, double *A1, int LDA1, double *A2, int LDA2, double *T, int LDT, double *TAU,
+                double *WORK) {
+  (void)WORK;
+  const char *fn = "CORE_dtsqrt";
+  if (M < 0) return core_bad(1, fn, "M < 0");
+  if (N < 0) return core_bad(2, fn, "N < 0");
+  if (IB < 0) return core_bad(3, fn, "IB < 0");
+  if (LDA1 < (N > 1 ? N : 1)) return core_bad(5, fn, "LDA1 < max(1, N)");
+  if (LDA2 < (M > 1 ? M : 1)) return core_bad(7, fn, "LDA2 < max(1, M)");
+  if (LDT < (IB > 1 ? IB : 1)) return core_bad(9, fn, "LDT < max(1, IB)");
+  if (M == 0 || N == 0 || IB == 0) return (ok(), 0);
+  if (M != N || !nb_ok(M)) return core_bad(1, fn, "square tiles of 64, 128 or 256 only");
+  if (IB > N) return core_bad(3, fn, "IB > N");
+  const int nb = M;
+  const int rc = packed_call({{A1, nb, nb, LDA1, true, true}, {A2, nb, nb, LDA2, true, true},
+                              {T, IB, nb, LDT, false, true}, {TAU, nb, 1, nb, false, true}},
+                             [&](std::vector<PArr> &a, hipStream_t st) {
+                               return hq_dtsqrt_dev(nb, IB, a[0].d, a[1].d, a[2].d, a[3].d, st);
+                             });
+  return rc == HQ_OK ? 0 : -1000 - rc;
+}
+
+int CORE_dormqr(int side, int trans, int M, int N, int K, int IB, const double *A, int LDA, const double *T, int LDT,
+                double *C, int LDC, double *WORK, int LDWORK) {
+  (void)WORK;
+  (void)LDWORK;
+  const char *fn = "CORE_dormqr";
+  if (side != HQ_PLASMA_LEFT) return core_bad(1, fn, "side must be PlasmaLeft");
+  if (trans != HQ_PLASMA_TRANS) return core_bad(2, fn, "trans must be PlasmaTrans");
+  if (M < 0) return core_bad(3, fn, "M < 0");
+  if (N < 0) return core_bad(4, fn, "N < 0");
+  if (K < 0 || K > M) return core_bad(5, fn, "K outside 0..M");
+  if (IB < 0) return core_bad(6, fn, "IB < 0");
+  if (LDA < (M > 1 ? M : 1)) return core_bad(8, fn, "LDA < max(1, M)");
+  if (LDT < (IB > 1 ? IB : 1)) return core_bad(10, fn, "LDT < max(1, IB)");
+  if (LDC < (M > 1 ? M : 1)) return core_bad(12, fn, "LDC < max(1, M)");
+  if (M == 0 || N == 0 || K == 0) return (ok(), 0);
+  if (M != N || K != M || !nb_ok(M)) return core_bad(3, fn, "M = N = K in {64, 128, 256} only");
+  if (IB == 0 || IB > K) return core_bad(6, fn, "IB outside 1..K");
+  if (M < 128 && IB > HQ_IBMAX) return core_bad(6, fn, "nb = 64 takes IB <= 32");
+  const int nb = M;
+  const int rc = packed_call({{(double *)A, nb, nb, LDA, true, false}, {(double *)T, IB, nb, LDT, true, false},
+                              {C, nb, nb, LDC, true, true}},
+                             [&](std::vector<PArr> &a, hipStream_t st) {
+                               return hq_dormqr_dev(nb, IB, a[0].d, a[1].d, a[2].d, st);
+                             });
+  return rc == HQ_OK ? 0 : -1000 - rc;
+}
+
+int CORE_dtsmqr(int side, int trans, int M1, int N1, int M2, int N2, int K, int IB, double *A1, int LDA1,
+                double *A2, int LDA2, const double *V, int LDV, const double *T, int LDT, double *WORK,
+                int LDWORK) {
+  (void)WORK;
+  (void)LDWORK;
+  const char *fn = "CORE_dtsmqr";
+  if (side != HQ_PLASMA_LEFT) return core_bad(1, fn, "side must be PlasmaLeft");
+  if (trans != HQ_PLASMA_TRANS) return core_bad(2, fn, "trans must be PlasmaTrans");
+  if (M1 < 0) return core_bad(3, fn, "M1 < 0");
+  if (N1 < 0) return core_bad(4, fn, "N1 < 0");
+  if (M2 < 0) return core_bad(5, fn, "M2 < 0");
+  if (N2 < 0 || N2 != N1) return core_bad(6, fn, "N2 != N1");
+  if (K < 0) return core_bad(7, fn, "K < 0");
+  if (IB < 0) return core_bad(8, fn, "IB < 0");
+  if (LDA1 < (M1 > 1 ? M1 : 1)) return core_bad(10, fn, "LDA1 < max(1, M1)");
+  if (LDA2 < (M2 > 1 ? M2 : 1)) return core_bad(12, fn, "LDA2 < max(1, M2)");
+  if (LDV < (M2 > 1 ? M2 : 1)) return core_bad(14, fn, "LDV < max(1, M2)");
+  if (LDT < (IB > 1 ? IB : 1)) return core_bad(16, fn, "LDT < max(1, IB)");
+  if (M1 == 0 || N1 == 0 || M2 == 0 || K == 0) return (ok(), 0);
+  if (M1 != N1 || M2 != M1 || K != M1 || !nb_ok(M1)) return core_bad(3, fn, "M1 = N1 = M2 = N2 = K in {64, 128, 256} only");
+  if (IB == 0 || IB > K) return core_bad(8, fn, "IB outside 1..K");
+  if (M1 < 128 && IB > HQ_IBMAX) return core_bad(8, fn, "nb = 64 takes IB <= 32");
+  const int nb = M1;
+  const int rc = packed_call({{A1, nb, nb, LDA1, true, true}, {A2, nb, nb, LDA2, true, true},
+                              {(double *)V, nb, nb, LDV, true, false}, {(double *)T, IB, nb, LDT, true, false}},
+                             [&](std::vector<PArr> &a, hipStream_t st) {
+                               return hq_dtsmqr_dev(nb, IB, a[0].d, a[1].d, a[2].d, a[3].d, st);
+                             });
+  return rc == HQ_OK ? 0 : -1000 - rc;
 }
 
 }  // extern "C"

@@ -10,8 +10,14 @@
  * process; query hq_last_error() / hq_last_error_message() (hybrid3_dev.h).
  *
  * Supported geometry: tile_M == tile_N == nb in {64, 128, 256},
- * 1 <= ib <= 32; the matrix is glob_tile_M x glob_tile_N tiles (any
- * glob_tile_M >= 1, glob_tile_N >= 1, tall and wide included).
+ * 1 <= ib <= nb (main.c:85-88 defaults ib to nb); the matrix is
+ * glob_tile_M x glob_tile_N tiles (any glob_tile_M >= 1, glob_tile_N >= 1,
+ * tall and wide included).  As in the reference, mat_init's integer
+ * division drops rows / columns past the last whole tile; hq_tile_qr_mn_dev
+ * (hybrid3_dev.h) factors such matrices with edge tiles.
+ * The host-pointer entry points run one at a time per device (a call waits
+ * for another thread's call to finish); a device fault (a timed-out wait on
+ * the GPU) is reported through hq_last_error().
  */
 #ifndef HQ_HYBRID_TILE_QR_H_
 #define HQ_HYBRID_TILE_QR_H_

@@ -8,6 +8,16 @@
  * or a nonzero hq error code (also kept in hq_last_error()).  `stream` is a
  * hipStream_t passed as void* (NULL = HIP's default stream); work is
  * queued on it and the call returns without synchronising unless stated.
+ * Calls on one device share the library's scratch (claim counter, flags, T /
+ * W scratch, batch tables); a call issued on another stream than the
+ * previous one waits on the device for the previous call's work, so calls
+ * from several streams are ordered rather than racing (prepared batch plans
+ * own their scratch and are not ordered this way).
+ *
+ * ib: any 1 <= ib <= nb.  The panels run with an inner block of at most 32
+ * columns; R, V and TAU do not depend on the blocking (results agree to
+ * rounding across ib), and a T returned for ib > 32 is rebuilt from V and TAU
+ * per ib block (LAPACK dlarft).
  *
  * Tile layout of A / TAU / T (all device memory):
  *   tile (i,j): A   + (i + j*MT)*nb*nb   column-major, ld = nb
@@ -42,6 +52,15 @@ const char *hq_version(void);
  * its own); otherwise it receives every tile's ib x nb T factor. */
 int hq_tile_qr_dev(int MT, int NT, int nb, int ib, double *A, double *TAU,
                    double *T, void *stream);
+
+/* hq_tile_qr_dev for an m x n matrix whose sizes need not be multiples of
+ * nb (edge tiles, which the reference's integer division drops,
+ * mat_prod.c:24-29): A holds ceil(m/nb) x ceil(n/nb) tiles in the layout
+ * above; the entries past row m or column n are set to zero by the call, so
+ * the reflectors are zero in the padded rows and the padded columns' taus
+ * are zero.  R's leading m x n part is that of the unpadded matrix. */
+int hq_tile_qr_mn_dev(int m, int n, int nb, int ib, double *A, double *TAU, double *T,
+                      void *stream);
 
 /* Mixed precision (no reference counterpart): the same flat-tree tile QR
  * with GEQRT / TSQRT (reflectors, T factors, R) in FP64 and the trailing
@@ -86,7 +105,8 @@ int hq_wall_clock_khz(int *khz);
 int hq_schedule_info(int MT, int NT, int nb, int ib, int *ntasks, int *grid);
 
 /* PLASMA-style single-tile kernels on device pointers (nb in {64,128,256},
- * 1 <= ib <= 32, every leading dimension = nb, T leading dimension = ib):
+ * 1 <= ib <= nb (LARFB / SSRFB at nb = 64: ib <= 32), every leading
+ * dimension = nb, T leading dimension = ib):
  *   GEQRT = CORE_dgeqrt(nb,nb,ib,A,nb,T,ib,TAU)        (Hybrid_tile_QR.c:985)
  *   TSQRT = CORE_dtsqrt(nb,nb,ib,A1,nb,A2,nb,T,ib,TAU) (Hybrid_tile_QR.c:1135)
  *   LARFB = CORE_dormqr(Left,Trans,nb,nb,nb,ib,V,nb,T,ib,C,nb) (:1276)
@@ -179,7 +199,16 @@ int hq_cm_to_tile_dev(int count, int nb, const int *m, const int *n, const doubl
 int hq_tile_to_cm_dev(int count, int nb, const int *m, const int *n, const double *const *A,
                       double *const *C, const int *ldc, void *stream);
 
-/* Synchronise `stream`; reports a scheduler timeout raised on the device. */
+/* Bound of the scheduler's dependency waits on this device, in polls of
+ * about a microsecond (0 = the default 2^22, seconds).  A wait that reaches
+ * it raises the device error word; every workgroup then drains without
+ * further work and the next hq_sync (or host-pointer call) reports
+ * HQ_ERR_DEVICE.  A short bound is a watchdog for callers that prefer an
+ * early error to a long stall. */
+int hq_set_wait_limit(unsigned spins);
+
+/* Synchronise `stream`; reports a scheduler timeout or a panel wait that
+ * gave up on the device (and clears both). */
 int hq_sync(void *stream);
 
 /* Release every device buffer the library holds. */

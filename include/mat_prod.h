@@ -35,6 +35,28 @@ typedef struct mstatus {
 void mat_init(mat_stat_t *status, long int mat_m, long int mat_n, int tile_m,
               int tile_n, int my_rank, int proc_num);
 
+/* Replaces mat_make (mat_prod.c:157-182): the reference's input generator,
+ * glibc rand()/RAND_MAX after srand(seed + my_rank), written in the
+ * reference's loop order (tile column, column in tile, tile row, row in
+ * tile), so the same seed gives the reference's matrix bit for bit. */
+void mat_make(mat_stat_t *status, double *matA, unsigned int seed, int my_rank);
+
+/* Replaces mat_remv (mat_prod.c:184-207): subtracts the matrix mat_make
+ * generates with the same seed (A - QR in main.c:253). */
+void mat_remv(mat_stat_t *status, double *matA, unsigned int seed, int my_rank);
+
+/* Replaces mat_comp3 (mat_prod.c:115-154): the 1-norm (largest column sum
+ * of |E|) of a column-major my_mat_M x my_mat_N matrix; NaN if a column sum
+ * is NaN.  One rank: my_rank 0, proc_num 1 (else NaN and hq_last_error). */
+double mat_comp3(mat_stat_t *status, double *matE, int my_rank, int proc_num);
+
+/* Replaces mat_dprint (mat_prod.c:260-281): tile layout -> column-major. */
+void mat_dprint(mat_stat_t *status, double *out, double *mat);
+
+/* Replaces mat_print (mat_prod.c:209-258): the matrix as text, one matrix
+ * row per line, after a "[    0] head" line. */
+void mat_print(mat_stat_t *status, FILE *out, double *mat, const char *head_string);
+
 #define mat_mysize(st) ((unsigned long int)((st)->my_mat_M * (st)->my_mat_N))
 #define mat_myTAU(st) ((unsigned long int)(((st)->my_mat_M / 2) * ((st)->my_mat_N / 2)))
 #define mat_TileSize(st) ((unsigned int)((st)->tile_M * (st)->tile_N))

@@ -73,7 +73,7 @@ def test_rank_and_geometry_errors_leave_arrays_untouched(hq):
     assert hq.last_error()[0] == 2
     L.Hybrid_tile_QR(C.byref(st), _p(A), _p(TAU), 0, 1, 0)    # ib <= 0 (main.c:75-84)
     assert hq.last_error()[0] == 1 and np.array_equal(A, A0)
-    L.Hybrid_tile_QR(C.byref(st), _p(A), _p(TAU), 0, 1, 33)   # ib above the supported 32
+    L.Hybrid_tile_QR(C.byref(st), _p(A), _p(TAU), 0, 1, 65)   # ib above nb = 64
     assert hq.last_error()[0] == 1
     st2 = hq.mat_init(300, 300, 100)                           # nb = 100 unsupported
     L.Hybrid_tile_QR(C.byref(st2), _p(A), _p(TAU), 0, 1, 16)
@@ -100,3 +100,116 @@ def test_tile_init_allocates_three_tiles_of_zeros(hq):
     arr = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_double)), shape=(3 * 16 * 8,))
     assert np.all(arr == 0.0)
     C.CDLL("libc.so.6").free(C.c_void_p(p))
+
+
+# ---------------------------------------------------------------- mat_prod helpers
+def glibc_rand(seed, count):
+    """glibc random_r TYPE_3 (the generator behind rand(), published algorithm):
+    r[0] = seed, r[i] = 16807 r[i-1] mod (2^31 - 1) for i < 31, r[31..33] =
+    r[0..2], r[i] = r[i-3] + r[i-31] mod 2^32; output k is r[344 + k] >> 1."""
+    r = [seed & 0xFFFFFFFF]
+    for i in range(1, 31):
+        x = (16807 * r[i - 1]) % 2147483647
+        r.append(x)
+    for i in range(31, 34):
+        r.append(r[i - 31])
+    out = []
+    i = 34
+    while len(out) < count:
+        r.append((r[i - 3] + r[i - 31]) & 0xFFFFFFFF)
+        if i >= 344:
+            out.append(r[i] >> 1)
+        i += 1
+    return out
+
+
+def test_glibc_model_matches_known_rand_values():
+    # srand(1): glibc's first rand() values
+    assert glibc_rand(1, 3) == [1804289383, 846930886, 1681692777]
+
+
+def test_mat_make_is_the_reference_generator(hq, orc):
+    L = hq.lib()
+    nb, MT = 4, 3
+    st = hq.mat_init(MT * nb, MT * nb, nb)
+    A = np.zeros(MT * MT * nb * nb)
+    L.mat_make(C.byref(st), _p(A), 20151130, 0)
+    vals = np.array(glibc_rand(20151130, A.size), dtype=np.float64) / 2147483647.0
+    # loop order of mat_prod.c:165-180: tile column, column, tile row, row
+    order = [((i + j * MT) * nb * nb + ii + jj * nb) for j in range(MT) for jj in range(nb)
+             for i in range(MT) for ii in range(nb)]
+    expect = np.zeros_like(A)
+    expect[order] = vals
+    assert np.array_equal(A, expect)
+    assert np.array_equal(A, orc.mat_make_glibc(MT, MT, nb, 20151130))
+    L.mat_make(C.byref(st), _p(A), 20151129, 1)            # seed + my_rank
+    assert np.array_equal(A, expect)
+    L.mat_remv(C.byref(st), _p(A), 20151130, 0)
+    assert np.all(A == 0.0)
+
+
+def test_mat_comp3_is_the_one_norm(hq):
+    L = hq.lib()
+    nb = 8
+    st = hq.mat_init(24, 24, nb)
+    E = np.random.default_rng(5).uniform(-1, 1, (24, 24))
+    flat = np.ascontiguousarray(E.T).ravel()            # column-major storage
+    assert abs(L.mat_comp3(C.byref(st), _p(flat), 0, 1) - np.linalg.norm(E, 1)) < 1e-14
+    flat[100] = np.nan
+    assert np.isnan(L.mat_comp3(C.byref(st), _p(flat), 0, 1))
+    assert np.isnan(L.mat_comp3(C.byref(st), _p(flat), 0, 2))
+
+
+def test_mat_dprint_is_tile_to_column_major(hq, orc):
+    L = hq.lib()
+    MT, nb = 3, 8
+    st = hq.mat_init(MT * nb, MT * nb, nb)
+    t = np.random.default_rng(6).uniform(-1, 1, MT * MT * nb * nb)
+    out = np.zeros_like(t)
+    L.mat_dprint(C.byref(st), _p(out), _p(t))
+    cm = orc.tile_to_cm(t, MT, MT, nb)
+    assert np.array_equal(out, cm.ravel(order="F"))
+
+
+def test_mat_print_format(hq, tmp_path):
+    L = hq.lib()
+    libc = C.CDLL("libc.so.6")
+    libc.fopen.restype = C.c_void_p
+    libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+    libc.fclose.argtypes = [C.c_void_p]
+    st = hq.mat_init(4, 4, 2)
+    t = np.arange(16, dtype=np.float64)
+    path = str(tmp_path / "m.txt")
+    f = libc.fopen(path.encode(), b"w")
+    L.mat_print(C.byref(st), f, _p(t), b"A")
+    libc.fclose(f)
+    lines = open(path).read().split("\n")
+    assert lines[0] == "[    0] A"
+    # row 0: tiles (0,0) and (0,1), columns 0..1 each: entries 0, 2 | 8, 10
+    assert lines[1].split("\t")[:4] == ["0.000000", "2.000000", "8.000000", "10.000000"]
+
+
+def test_core_shim_validates_without_device(hq):
+    L = hq.lib()
+    A = np.zeros(64 * 64)
+    assert L.CORE_dgeqrt(-1, 64, 32, _p(A), 64, _p(A), 64, _p(A), None) == -1
+    assert L.CORE_dgeqrt(64, 64, 32, _p(A), 10, _p(A), 64, _p(A), None) == -5
+    assert L.CORE_dgeqrt(96, 96, 32, _p(A), 96, _p(A), 96, _p(A), None) == -1   # unsupported tile
+    assert L.CORE_dgeqrt(0, 64, 32, _p(A), 1, _p(A), 32, _p(A), None) == 0      # quick return
+    assert L.CORE_dtsqrt(64, 64, 32, _p(A), 64, _p(A), 64, _p(A), 8, _p(A), None) == -9
+    assert L.CORE_dormqr(142, 112, 64, 64, 64, 32, _p(A), 64, _p(A), 64, _p(A), 64, None, 64) == -1
+    assert L.CORE_dormqr(141, 111, 64, 64, 64, 32, _p(A), 64, _p(A), 64, _p(A), 64, None, 64) == -2
+    assert L.CORE_dtsmqr(141, 112, 64, 64, 64, 32, 64, 32, _p(A), 64, _p(A), 64, _p(A), 64, _p(A), 64,
+                         None, 32) == -6
+    assert L.CORE_dtsmqr(141, 112, 64, 64, 64, 64, 64, 48, _p(A), 64, _p(A), 64, _p(A), 64, _p(A), 64,
+                         None, 32) == -8    # nb = 64 takes ib <= 32 in the updates
+    code, msg = hq.last_error()
+    assert code == 1 and "argument 8" in msg
+
+
+def test_driver_program_is_built(hq):
+    import subprocess
+    exe = os.path.join(ROOT, "hybrid3_amd", "hybrid3")
+    assert os.access(exe, os.X_OK)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)   # no arguments: no GPU use
+    assert r.returncode == 0 and "This program need two arguments." in r.stderr
