@@ -99,6 +99,8 @@ SIGNATURES = {
     "CORE_dtsmqr": (C.c_int, [C.c_int] * 8 + [_dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp,
                                              C.c_int]),
     "hq_set_wait_limit": (C.c_int, [C.c_uint]),
+    "hq_tile_qr_noalloc_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp]),
+    "hq_tile_qr_noalloc_check": (C.c_int, [_vp]),
     "hq_sync": (C.c_int, [_vp]),
     "hq_finalize": (None, []),
 }

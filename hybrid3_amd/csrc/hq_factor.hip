@@ -2190,6 +2190,13 @@ __global__ void reset_counter_kernel(int *counter, int *epoch_dev) {
   }
 }
 
+// p[0 .. n) <- value (clears counters and flags with a kernel, which a
+// captured graph replays correctly, unlike a memset node).
+__global__ void fill_int_kernel(int *p, size_t n, int value) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+    p[e] = value;
+}
+
 // SplitMix64 of (seed, global column-major index) -> uniform [-1, 1), written
 // in tile layout.  Bit-identical to the CPU generator used by the tests.
 __device__ __forceinline__ double uniform_pm1(uint64_t seed, uint64_t index) {
@@ -2364,6 +2371,11 @@ hipError_t hq_launch_tbuild_wide_tile(int nb, int ib, const double *V, const dou
     hipLaunchKernelGGL((hq::tbuild_wide_tile_kernel<NB>), dim3((NB + ib - 1) / ib), dim3(HQ_THREADS), 0, st, V,
                        tau, T, ib, diag);
   });
+}
+
+hipError_t hq_launch_fill_int(int *p, size_t n, int value, hipStream_t st) {
+  hipLaunchKernelGGL(hq::fill_int_kernel, dim3(grid_for(n)), dim3(256), 0, st, p, n, value);
+  return hipGetLastError();
 }
 
 hipError_t hq_launch_pad_zero(double *A, int MT, int NT, int nb, int m, int n, hipStream_t st) {

@@ -69,3 +69,6 @@ struct hq_bmat {
 #define HQ_THREADS 256
 #define HQ_SLAB 64
 #define HQ_IBMAX 32
+// bytes of the device array behind the allocation-free tile QR (claim counter,
+// flags and task list of one geometry)
+#define HQ_NOALLOC_SCRATCH (32 << 20)

@@ -43,6 +43,7 @@ hipError_t hq_launch_tbuild_wide_tile(int nb, int ib, const double *V, const dou
                                       hipStream_t st);
 hipError_t hq_launch_pad_zero(double *A, int MT, int NT, int nb, int m, int n, hipStream_t st);
 hipError_t hq_fault_take(int *value);
+hipError_t hq_launch_fill_int(int *p, size_t n, int value, hipStream_t st);
 
 // homegrown tile_kernel.h API and verification helpers (hq_homegrown.hip)
 hipError_t hqh_launch_dgeqrt(int row, int col, int block, double *A, int lda, double *TAU,
@@ -852,6 +853,118 @@ void mat_init(mat_stat_t *status, long int mat_m, long int mat_n, int tile_m, in
 int hq_tile_qr_dev(int MT, int NT, int nb, int ib, double *A, double *TAU, double *T,
                    void *stream) {
   return tile_qr_dev_impl(MT, NT, nb, ib, A, TAU, T, stream, false);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free tile QR (hq_tile_qr_noalloc_dev): the scheduler's claim
+// counter, device epoch, task flags and task list live in `scratch`, a
+// file-scope device array of HQ_NOALLOC_SCRATCH bytes that each GPU holds
+// from library load to process end.  Host code never uses the array's name
+// as an address: the device address comes from hipGetSymbolAddress, taken
+// once per device with that device current, and is kept per device.
+// Layout: [0, 256) counter block (int 0: claim counter, 1: time-out word,
+// 2: device epoch), then the flags (one int per task, 256-byte aligned), then
+// the task list.  Every call shares the array of its GPU, which keeps the
+// last call's contents: the epoch only grows, so flags left by earlier calls
+// of the same geometry never read as complete; when the geometry changes,
+// the task list is copied again and the flag range is cleared (it may cover
+// the previous task list).  Calls that overlap on one GPU race.
+static __device__ __align__(256) unsigned char scratch[HQ_NOALLOC_SCRATCH];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+struct NoAllocDev {
+  bool ready = false;
+  unsigned char *ptr = nullptr;  // device address of `scratch` on this device
+  int num_cu = 0;
+  int MT = 0, NT = 0, nb = 0, ib = 0, ntasks = 0, grid = 0;  // resident task list
+};
+NoAllocDev g_noalloc[kMaxDevices];
+std::mutex g_noalloc_mu;
+
+size_t noalloc_flags_off() { return 256; }
+size_t noalloc_tasks_off(int ntasks) { return 256 + (((size_t)ntasks * sizeof(int) + 255) & ~(size_t)255); }
+
+int noalloc_impl(int MT, int NT, int nb, int ib, double *A, double *TAU, double *T, void *stream) {
+  if (!geometry_ok(MT, NT, nb, ib)) return g_err;
+  if (ib > HQ_IBMAX)
+    return set_error(HQ_ERR_ARG, "the allocation-free tile QR takes ib <= %d (ib=%d)", HQ_IBMAX, ib);
+  if (!A || !TAU || !T) return set_error(HQ_ERR_ARG, "A, TAU and T must be device pointers");
+  int dev = 0;
+  HQ_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return set_error(HQ_ERR_ARG, "device %d beyond %d", dev, kMaxDevices);
+  hipStream_t st = (hipStream_t)stream;
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  NoAllocDev &D = g_noalloc[dev];
+  if (!D.ready) {  // set-up on this device, with it current
+    void *p = nullptr;
+    HQ_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    D.ptr = (unsigned char *)p;
+    HQ_HIP(hipDeviceGetAttribute(&D.num_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    // counter block and every flag start at zero
+    HQ_HIP(hq_launch_fill_int((int *)D.ptr, HQ_NOALLOC_SCRATCH / sizeof(int), 0, st));
+    D.ready = true;
+  }
+  int *counter = (int *)D.ptr;
+  int *flags = (int *)(D.ptr + noalloc_flags_off());
+  if (D.MT != MT || D.NT != NT || D.nb != nb || D.ib != ib) {
+    // the task list of this geometry in the claim order of hq_tile_qr_dev
+    // (host memory for the list, this call only), copied into the array
+    DagBuilder b(MT, NT, nb, ib);
+    b.build();
+    const int n = (int)b.t.size();
+    const size_t need = noalloc_tasks_off(n) + sizeof(hq_task) * (size_t)n;
+    if (need > HQ_NOALLOC_SCRATCH)
+      return set_error(HQ_ERR_ARG,
+                       "the allocation-free tile QR holds at most %d MiB of schedule; %d x %d tiles of nb=%d, "
+                       "ib=%d need %zu bytes (%d tasks)",
+                       HQ_NOALLOC_SCRATCH >> 20, MT, NT, nb, ib, need, n);
+    const int grid = D.num_cu > 0 ? D.num_cu : 256;
+    std::vector<int> ord = b.order(grid);
+    std::vector<int> pos(ord.size());
+    for (size_t i = 0; i < ord.size(); i++) pos[ord[i]] = (int)i;
+    std::vector<hq_task> tl(ord.size());
+    for (size_t i = 0; i < ord.size(); i++) {
+      hq_task x = b.t[ord[i]];
+      for (int d = 0; d < x.ndep; d++) x.dep[d] = pos[x.dep[d]];
+      tl[i] = x;
+    }
+    // the new flag range may cover bytes of the previous geometry's task
+    // list, which could equal a later epoch: clear it (epochs start at 1)
+    HQ_HIP(hq_launch_fill_int(flags, (size_t)n, 0, st));
+    HQ_HIP(hipMemcpyAsync(D.ptr + noalloc_tasks_off(n), tl.data(), sizeof(hq_task) * tl.size(),
+                          hipMemcpyHostToDevice, st));
+    HQ_HIP(hipStreamSynchronize(st));  // the host list is released on return
+    D.MT = MT;
+    D.NT = NT;
+    D.nb = nb;
+    D.ib = ib;
+    D.ntasks = n;
+    D.grid = std::min(grid, n);
+  }
+  // LARFB multiplies by T itself (no W = V T16 scratch)
+  hq_geom G{A, TAU, T, MT, NT, nb, ib, A, NT, 0};
+  HQ_HIP(hq_launch_sched(G, nullptr, (const hq_task *)(D.ptr + noalloc_tasks_off(D.ntasks)), D.ntasks, flags, 0,
+                         counter, D.grid, nullptr, st, counter + 2));
+  return ok();
+}
+
+}  // namespace
+
+extern "C" int hq_tile_qr_noalloc_dev(int MT, int NT, int nb, int ib, double *A, double *TAU, double *T,
+                                      void *stream) {
+  return noalloc_impl(MT, NT, nb, ib, A, TAU, T, stream);
+}
+
+extern "C" int hq_tile_qr_noalloc_check(void *stream) {
+  int dev = 0;
+  HQ_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices || !g_noalloc[dev].ready)
+    return set_error(HQ_ERR_ARG, "no allocation-free call was made on device %d", dev);
+  HQ_HIP(hipStreamSynchronize((hipStream_t)stream));
+  if (take_device_faults((int *)g_noalloc[dev].ptr) != HQ_OK) return g_err;
+  return ok();
 }
 
 int hq_tile_qr_mn_dev(int m, int n, int nb, int ib, double *A, double *TAU, double *T, void *stream) {

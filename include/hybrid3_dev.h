@@ -71,6 +71,25 @@ int hq_tile_qr_mn_dev(int m, int n, int nb, int ib, double *A, double *TAU, doub
 int hq_tile_qr_mixed_dev(int MT, int NT, int nb, int ib, double *A, double *TAU,
                          double *T, void *stream);
 
+/* hq_tile_qr_dev without any device allocation (no counterpart in the
+ * reference; for callers that cannot allocate: code under hipGraph capture,
+ * serving loops, harnesses that forbid hipMalloc).  The scheduler's claim
+ * counter, device epoch, task flags and task list live in a device array of
+ * the library of 32 MiB per GPU (held from library load to process end); no
+ * hipMalloc, hipFree or torch allocation happens.  T is required (ib x nb per
+ * tile, ib <= 32) and LARFB multiplies by T itself, so results agree with
+ * hq_tile_qr_dev to rounding (not bit for bit).  Limit: the geometry's task
+ * list must fit, e.g. 16384 x 16384 at nb = 256, ib = 32 (399360 tasks, 22 MB);
+ * larger ones return HQ_ERR_ARG.  The first call for a geometry copies its task
+ * list into the array and synchronises `stream`; later calls with the same
+ * geometry only enqueue two kernels (capturable in a hipGraph, the run's
+ * epoch being advanced on the device).  All calls on one GPU share the array:
+ * calls that overlap on one GPU race.  hq_tile_qr_noalloc_check synchronises
+ * `stream` and reports a scheduler time-out of these calls. */
+int hq_tile_qr_noalloc_dev(int MT, int NT, int nb, int ib, double *A, double *TAU, double *T,
+                           void *stream);
+int hq_tile_qr_noalloc_check(void *stream);
+
 /* Same factorisation, one kernel launch per task in the scheduler's order
  * (stream order replaces the device dependency flags). */
 int hq_tile_qr_dev_serial(int MT, int NT, int nb, int ib, double *A,
