@@ -64,6 +64,7 @@ SIGNATURES = {
     "hq_dtsmqr_dev": (C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp]),
     "hq_fill_uniform_dev": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_uint64, _vp, _vp]),
     "hq_tile_qr_dev_traced": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp, _vp]),
+    "hq_tile_qr_traced_prec_dev": (C.c_int, [C.c_int] * 5 + [_vp, _vp, _vp, _vp, _vp]),
     "hq_schedule_tasks": (C.c_int, [C.c_int] * 4 + [C.POINTER(C.c_int), C.c_int]),
     "hq_wall_clock_khz": (C.c_int, [C.POINTER(C.c_int)]),
     "hq_tile_build_t_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp]),
@@ -85,6 +86,7 @@ SIGNATURES = {
     "hq_tile_to_cm_dev": (C.c_int, [C.c_int] * 2 + [C.POINTER(C.c_int)] * 2
                           + [C.POINTER(_vp), C.POINTER(_vp), C.POINTER(C.c_int), _vp]),
     "hq_tile_qr_mixed_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp]),
+    "hq_tile_qr_split_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp]),
     "hq_tile_qr_mn_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp]),
     # mat_prod.h harness helpers
     "mat_make": (None, [C.POINTER(mat_stat_t), _dp, C.c_uint, C.c_int]),
@@ -190,6 +192,11 @@ def tile_qr_mixed_dev(MT, NT, nb, ib, A, TAU, T=None, stream=None):
 def tile_qr_mn_dev(m, n, nb, ib, A, TAU, T=None, stream=None):
     """Tile QR of an m x n matrix with edge tiles (padding zeroed by the call)."""
     check(lib().hq_tile_qr_mn_dev(m, n, nb, ib, _ptr(A), _ptr(TAU), _ptr(T), _stream(stream)))
+
+
+def tile_qr_split_dev(MT, NT, nb, ib, A, TAU, T=None, stream=None):
+    """FP64 panels, LARFB / SSRFB on FP16 hi+lo splits with FP32 accumulation."""
+    check(lib().hq_tile_qr_split_dev(MT, NT, nb, ib, _ptr(A), _ptr(TAU), _ptr(T), _stream(stream)))
 
 
 def fill_uniform_dev(MT, NT, nb, seed, A, stream=None):

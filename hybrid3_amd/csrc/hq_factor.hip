@@ -664,7 +664,8 @@ __device__ __forceinline__ void store_vt(Smem<NB> &S, double *Wt, int ii, int sb
 // S.TT.
 template <int NB, int WB>
 __device__ __forceinline__ void geqrt_panel(Smem<NB> &S, double *A, double *T, double *TAU, int ib, int ii,
-                                            double *Wt = nullptr, double *Vd = nullptr) {
+                                            double *Wt = nullptr, double *Vd = nullptr,
+                                            unsigned short *VHt = nullptr) {
   constexpr int LDV = Smem<NB>::LDV;
   const int tid = threadIdx.x, wave = tid >> 6;
   const int sb = min(ib, NB - ii);
@@ -686,6 +687,7 @@ __device__ __forceinline__ void geqrt_panel(Smem<NB> &S, double *A, double *T, d
   __syncthreads();
   if (wave == 0) build_t<NB>(S, sb, T + (size_t)ii * ib, ib, TAU + ii);
   __syncthreads();
+  if (VHt) store_vh<NB>(S, VHt, ib, ii, sb);
   if (Wt) {
     store_vt<NB>(S, Wt, ii, sb);
     for (int e = tid; e < sb * NB; e += HQ_THREADS) {
@@ -700,7 +702,7 @@ __device__ __forceinline__ void geqrt_panel(Smem<NB> &S, double *A, double *T, d
 // S.TT.
 template <int NB, int WB>
 __device__ __forceinline__ void tsqrt_panel(Smem<NB> &S, double *A1, double *A2, double *T, double *TAU, int ib,
-                                            int ii) {
+                                            int ii, unsigned short *VHt = nullptr) {
   constexpr int LDV = Smem<NB>::LDV, LDT = Smem<NB>::LDT;
   constexpr int IBP = 16 * WB;
   const int tid = threadIdx.x, wave = tid >> 6;
@@ -732,6 +734,7 @@ __device__ __forceinline__ void tsqrt_panel(Smem<NB> &S, double *A1, double *A2,
   if (wave == 0) build_t<NB>(S, sb, T + (size_t)ii * ib, ib, TAU + ii);
   __syncthreads();
   HQ_PHASE(4);
+  if (VHt) store_vh<NB>(S, VHt, ib, ii, sb);
 }
 
 // For a split trailing task: V of the block at column ii of tile Vt (GEQRT:
@@ -803,9 +806,10 @@ __device__ __forceinline__ void panel_trailing(Smem<NB> &S, double *X, double *A
 // GEQRT = CORE_dgeqrt(nb, nb, ib, A, nb, T, ib, TAU)
 template <int NB, int WB>
 __device__ __forceinline__ void task_geqrt(Smem<NB> &S, double *A, double *T, double *TAU, int ib,
-                                           double *Wt = nullptr, double *Vd = nullptr) {
+                                           double *Wt = nullptr, double *Vd = nullptr,
+                                           unsigned short *VHt = nullptr) {
   for (int ii = 0; ii < NB; ii += ib) {
-    geqrt_panel<NB, WB>(S, A, T, TAU, ib, ii, Wt, Vd);
+    geqrt_panel<NB, WB>(S, A, T, TAU, ib, ii, Wt, Vd, VHt);
     panel_trailing<NB, WB, false>(S, A, nullptr, ib, ii, ii + min(ib, NB - ii), NB);
     __syncthreads();
   }
@@ -813,9 +817,10 @@ __device__ __forceinline__ void task_geqrt(Smem<NB> &S, double *A, double *T, do
 
 // TSQRT = CORE_dtsqrt(nb, nb, ib, A1, nb, A2, nb, T, ib, TAU)
 template <int NB, int WB>
-__device__ __forceinline__ void task_tsqrt(Smem<NB> &S, double *A1, double *A2, double *T, double *TAU, int ib) {
+__device__ __forceinline__ void task_tsqrt(Smem<NB> &S, double *A1, double *A2, double *T, double *TAU, int ib,
+                                           unsigned short *VHt = nullptr) {
   for (int ii = 0; ii < NB; ii += ib) {
-    tsqrt_panel<NB, WB>(S, A1, A2, T, TAU, ib, ii);
+    tsqrt_panel<NB, WB>(S, A1, A2, T, TAU, ib, ii, VHt);
     panel_trailing<NB, WB, true>(S, A2, A1, ib, ii, ii + min(ib, NB - ii), NB);
     __syncthreads();
     HQ_PHASE(5);
@@ -1782,14 +1787,376 @@ __device__ __forceinline__ void task_update_f32(Smem<NB> &S, const double *V, co
   }
 }
 
-template <int NB, int WB>
+// --------------------------------------------------------------------------
+// Split-precision update path (prec 2): LARFB / SSRFB of the same slab tasks
+// on FP16 MFMA with FP32 accumulation, every product taken as
+// hi*hi + hi*lo + lo*hi of FP16 hi + lo splits (x = hi + lo, hi = fp16(x),
+// lo = fp16(x - hi): 22 significant bits, the lo*lo term of 2^-22 relative
+// size dropped).  The panels that produce V and T stay FP64.
+//  * Scaling by powers of two (exact): V by 2^10 (Householder vectors have
+//    |v_i| <= 1, so every entry of V and of its lo part stays inside FP16's
+//    range); the strip by 2^e per column with e chosen from the column's
+//    largest entry over [A1; A2] so that its 2-norm stays below 2^12 -- the
+//    transform is orthogonal, so no intermediate column grows past it.  The lo
+//    parts of small entries go subnormal at an absolute level of 2^-25, i.e.
+//    2^-37 of the column norm.
+//  * The strip lives in FP32 in MFMA accumulator layout; W = V^T X (K = 32
+//    rows per v_mfma_f32_16x16x32_f16: the two row blocks a lane group holds
+//    are its 8 K slots, so the strip's registers split straight into the B
+//    operand), W2 = -T^T W on v_mfma_f32_16x16x4f32 (T in FP32), and
+//    X += V (-W2) with K = 16 reflectors on v_mfma_f32_16x16x16f16 (W2's
+//    accumulator layout is that instruction's B operand).
+//  * v_mfma_f32_16x16x{16,32}_f16 lane layouts (gfx950, measured): A(i,k) at
+//    lane i + 16*(k/E), element k%E, B(k,j) at lane j + 16*(k/E), element k%E
+//    (E = K/4), D(i,j) at lane j + 16*(i/4), register i%4.
+//  * LDS ring slot of a 16-reflector sub-block: V^T (reflector-major, rows
+//    contiguous, the A phase's operands) and V (row-major, 16 reflectors
+//    contiguous, the update phase's), hi and lo of each, as FP16.
+// --------------------------------------------------------------------------
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ f4 mfma_h16(h4 a, h4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0); }
+__device__ __forceinline__ f4 mfma_h32(h8 a, h8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
+constexpr float HV_SCALE = 1024.0f;          // V is staged as 2^10 V
+constexpr float HV_UNSCALE = 1.0f / 1024.0f;
+__device__ __forceinline__ void hsplit(float x, _Float16 &h, _Float16 &l) {
+  h = (_Float16)x;
+  l = (_Float16)(x - (float)h);
+}
+// A sub-block image (and LDS ring slot): 4 arrays of 16 x NB halves,
+//   AH / AL: V^T hi / lo, reflector-major; element (i, row r) at
+//            i*NB + 4*((r >> 2) ^ i) + (r & 3) (4-row chunks swizzled by
+//            the reflector index, so the 16 reflectors of an accumulate
+//            operand read fall on different LDS banks);
+//   UH / UL: V hi / lo, row-major, element (r, i) at r*16 + i.
+// 128 NB bytes, a whole number of 1 KiB direct-to-LDS wave copies.
+template <int NB>
+struct HSlot {
+  static constexpr int BYTES = 128 * NB;
+  static constexpr int AH = 0, AL = 16 * NB, UH = 32 * NB, UL = 48 * NB;  // offsets in halves
+  static constexpr int NOPS = BYTES / 1024 / (HQ_THREADS / 64);            // copies per wave
+};
+__device__ __forceinline__ int hswz(int i, int r) { return 4 * ((r >> 2) ^ i) + (r & 3); }
+// Ring slot i (0..3): two slots in S.V, two in S.V1.
+template <int NB>
+__device__ __forceinline__ _Float16 *hvbuf(Smem<NB> &S, int i) {
+  static_assert(2 * HSlot<NB>::BYTES <= (int)sizeof(S.V), "two split slots per V buffer");
+  return reinterpret_cast<_Float16 *>(reinterpret_cast<unsigned char *>(i < 2 ? S.V : S.V1) +
+                                      (i & 1) * HSlot<NB>::BYTES);
+}
+
+// Panel side: the images of the sub-blocks of the ib block at column ii
+// (V in S.V, column-major, unit lower for GEQRT, V2 for TSQRT, zero past the
+// block) into the tile's image area VHt (sub-block s = b*nsb + h).
+template <int NB>
+__device__ __forceinline__ void store_vh(Smem<NB> &S, unsigned short *VHt, int ib, int ii, int sb) {
+  using L = HSlot<NB>;
+  constexpr int LDV = Smem<NB>::LDV;
+  const int nsb = (ib + 15) >> 4;
+  const int s0 = (ii / ib) * nsb;
+  const int nh = (sb + 15) >> 4;
+  typedef _Float16 h4t __attribute__((ext_vector_type(4)));
+  for (int h = 0; h < nh; h++) {
+    const int kb = min(16, sb - 16 * h);
+    _Float16 *img = reinterpret_cast<_Float16 *>(VHt) + (size_t)(s0 + h) * 64 * NB;
+    // V^T arrays: chunk (i, c) = rows 4c..4c+3 of reflector i
+    for (int e = threadIdx.x; e < 16 * (NB / 4); e += HQ_THREADS) {
+      const int i = e / (NB / 4), c = e - i * (NB / 4);
+      h4t hv, lv;
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        const float f = i < kb ? (float)S.V[(16 * h + i) * LDV + 4 * c + t] * HV_SCALE : 0.f;
+        _Float16 a, b;
+        hsplit(f, a, b);
+        hv[t] = a;
+        lv[t] = b;
+      }
+      *reinterpret_cast<h4t *>(img + L::AH + i * NB + hswz(i, 4 * c)) = hv;
+      *reinterpret_cast<h4t *>(img + L::AL + i * NB + hswz(i, 4 * c)) = lv;
+    }
+    // V arrays: chunk (r, q) = reflectors 4q..4q+3 of row r
+    for (int e = threadIdx.x; e < NB * 4; e += HQ_THREADS) {
+      const int r = e >> 2, q = e & 3;
+      h4t hv, lv;
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        const int i = 4 * q + t;
+        const float f = i < kb ? (float)S.V[(16 * h + i) * LDV + r] * HV_SCALE : 0.f;
+        _Float16 a, b;
+        hsplit(f, a, b);
+        hv[t] = a;
+        lv[t] = b;
+      }
+      *reinterpret_cast<h4t *>(img + L::UH + r * 16 + 4 * q) = hv;
+      *reinterpret_cast<h4t *>(img + L::UL + r * 16 + 4 * q) = lv;
+    }
+  }
+}
+
+// Copy q (< NOPS) of this wave of a sub-block image into ring slot Vb.
+template <int NB>
+__device__ __forceinline__ void dmah_op(int q, _Float16 *Vb, const _Float16 *img) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int Q = wave + (HQ_THREADS / 64) * q;
+  const unsigned char *src = reinterpret_cast<const unsigned char *>(img) + 1024 * Q + 16 * lane;
+  lds_char *dst = (lds_char *)(reinterpret_cast<unsigned char *>(Vb) + 1024 * Q);
+  __builtin_amdgcn_global_load_lds((const void *)src, (void __attribute__((address_space(3))) *)dst, 16, 0, 0);
+}
+
+// Operands of one update phase (row blocks 2G, 2G+1): V(16rb + i, 4g..4g+3)
+// hi / lo for the lane (i, g).
+struct HOpU {
+  h4 h[2], l[2];
+};
+template <int NB>
+__device__ __forceinline__ void ldh_u(HOpU &o, const _Float16 *Vb, int G) {
+  using L = HSlot<NB>;
+  const int lane = threadIdx.x & 63, i = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int t = 0; t < 2; t++) {
+    const int row = 16 * (2 * G + t) + i;
+    o.h[t] = *reinterpret_cast<const h4 *>(Vb + L::UH + row * 16 + 4 * g);
+    o.l[t] = *reinterpret_cast<const h4 *>(Vb + L::UL + row * 16 + 4 * g);
+  }
+}
+// Operands of one accumulate phase: V^T(i, rows 32G + 4g.. and 32G + 16 + 4g..)
+struct HOpA {
+  h8 h, l;
+};
+template <int NB>
+__device__ __forceinline__ void ldh_a(HOpA &o, const _Float16 *Vb, int G) {
+  using L = HSlot<NB>;
+  const int lane = threadIdx.x & 63, i = lane & 15, g = lane >> 4;
+  const int r0 = 32 * G + 4 * g;
+  const h4 h0 = *reinterpret_cast<const h4 *>(Vb + L::AH + i * NB + hswz(i, r0));
+  const h4 h1 = *reinterpret_cast<const h4 *>(Vb + L::AH + i * NB + hswz(i, r0 + 16));
+  const h4 l0 = *reinterpret_cast<const h4 *>(Vb + L::AL + i * NB + hswz(i, r0));
+  const h4 l1 = *reinterpret_cast<const h4 *>(Vb + L::AL + i * NB + hswz(i, r0 + 16));
+  o.h = h8{h0[0], h0[1], h0[2], h0[3], h1[0], h1[1], h1[2], h1[3]};
+  o.l = h8{l0[0], l0[1], l0[2], l0[3], l1[0], l1[1], l1[2], l1[3]};
+}
+
+// One pass of the split pipeline: update phases X += V_u (-W2) and
+// accumulate phases W += V_w^T X interleaved as U0 U1 A0 U2 A1 ... A(NG-1)
+// (or only U / only A); the copies of the next sub-block image, one per
+// phase.
+template <int NB, bool DO_U, bool DO_A, bool DO_DMA>
+__device__ __forceinline__ void fpassh(f4 (&acc)[NB / 16], const h4 &W2h, const h4 &W2l, f4 (&Wn)[2], float (&tto)[4],
+                                       const _Float16 *Vu, const _Float16 *Vw, const float *TTw, _Float16 *Vd,
+                                       const _Float16 *img) {
+  constexpr int NG = NB / 32;
+  constexpr int NP = (DO_U ? NG : 0) + (DO_A ? NG : 0);
+  static_assert(!DO_DMA || NP >= HSlot<NB>::NOPS, "one image copy per phase");
+  auto kind = [](int p) -> int {
+    if (DO_U && !DO_A) return 1;
+    if (!DO_U) return 0;
+    if (p == 0) return 1;
+    if (p == 2 * NG - 1) return 0;
+    return (p & 1) ? 1 : 0;
+  };
+  auto grp = [](int p) -> int {
+    if (!(DO_U && DO_A)) return p;
+    if (p == 0) return 0;
+    if (p == 2 * NG - 1) return NG - 1;
+    return (p & 1) ? (p + 1) / 2 : p / 2 - 1;
+  };
+  const int lane = threadIdx.x & 63, g = lane >> 4, m = lane & 15;
+  // operands one phase ahead (the first phase's are read here); no
+  // instruction crosses a phase boundary, which bounds the live registers
+  HOpU ou[2];
+  HOpA oa[2];
+  if (kind(0))
+    ldh_u<NB>(ou[0], Vu, grp(0));
+  else
+    ldh_a<NB>(oa[0], Vw, grp(0));
+#pragma unroll
+  for (int p = 0; p < NP; p++) {
+    if (DO_DMA && p < HSlot<NB>::NOPS) dmah_op<NB>(p, Vd, img);
+    if (p + 1 < NP) {
+      if (kind(p + 1))
+        ldh_u<NB>(ou[(p + 1) & 1], Vu, grp(p + 1));
+      else
+        ldh_a<NB>(oa[(p + 1) & 1], Vw, grp(p + 1));
+    } else if (DO_A) {
+#pragma unroll
+      for (int sp = 0; sp < 4; sp++) tto[sp] = TTw[m * LDT16 + 4 * g + sp];
+    }
+    const int G = grp(p);
+    if (kind(p)) {
+      const HOpU &o = ou[p & 1];
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        f4 a = acc[2 * G + t];
+        a = mfma_h16(o.h[t], W2h, a);
+        a = mfma_h16(o.h[t], W2l, a);
+        a = mfma_h16(o.l[t], W2h, a);
+        acc[2 * G + t] = a;
+      }
+    } else {
+      const HOpA &o = oa[p & 1];
+      const f4 x0 = acc[2 * G], x1 = acc[2 * G + 1];
+      h8 xh, xl;
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        _Float16 h, l;
+        hsplit(x0[e], h, l);
+        xh[e] = h;
+        xl[e] = l;
+        hsplit(x1[e], h, l);
+        xh[4 + e] = h;
+        xl[4 + e] = l;
+      }
+      Wn[0] = mfma_h32(o.h, xh, Wn[0]);
+      Wn[1] = mfma_h32(o.h, xl, Wn[1]);
+      Wn[1] = mfma_h32(o.l, xh, Wn[1]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// LARFB (UNIT: the images of the diagonal tile's unit lower V) / SSRFB on
+// one 64-column slab in split precision.  VHt: the reflector tile's images.
+template <int NB, bool UNIT>
+__device__ __forceinline__ void task_update_h(Smem<NB> &S, const unsigned short *VHt, const double *T, double *X,
+                                              double *A1, int ib, int slab) {
+  static_assert(NB % HQ_SLAB == 0, "slabs cover the tile exactly");
+  const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, g = lane >> 4, wave = tid >> 6;
+  const int col = slab * HQ_SLAB + 16 * wave + m;
+  double *xcol = X + (size_t)col * NB;
+  double *a1col = UNIT ? nullptr : A1 + (size_t)col * NB;
+  const int nsub = sub_count<NB>(ib);
+  const _Float16 *img0 = reinterpret_cast<const _Float16 *>(VHt);
+  auto sbk = [&](int s) __attribute__((always_inline)) { return sub_blk<NB>(min(s, nsub - 1), ib); };
+  auto img = [&](int s) __attribute__((always_inline)) { return img0 + (size_t)min(s, nsub - 1) * 64 * NB; };
+  f4 acc[NB / 16];
+  f4 Wn[2], a1v = f4{0.f, 0.f, 0.f, 0.f}, W2n = a1v;
+  h4 W2h = h4{0, 0, 0, 0}, W2l = W2h;
+  float tto[4];
+  double treg = 0.0;
+  float xs, xu;  // the column's scale 2^e and 2^-e
+  // prologue: the images of sub-blocks 0 and 1, T^T of both, the strip
+  // (two sweeps over the column, the second from cache: its largest entry
+  // over [A1; A2], then the scaled FP32 copy)
+  {
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+#pragma unroll
+      for (int q = 0; q < HSlot<NB>::NOPS; q++) dmah_op<NB>(q, hvbuf(S, j), img(j));
+      f32tbuf(S, j)[(tid >> 4) * LDT16 + (tid & 15)] = (float)(tt_raw(T, ib, sbk(j)) * tt_mask(sbk(j)));
+    }
+    double mx = 0.0;
+#pragma unroll 4
+    for (int rb = 0; rb < NB / 16; rb++) {
+      const d4 x = *reinterpret_cast<const d4 *>(xcol + 16 * rb + 4 * g);
+#pragma unroll
+      for (int v = 0; v < 4; v++) mx = fmax(mx, fabs(x[v]));
+    }
+    if (!UNIT) {
+#pragma unroll 4
+      for (int rb = 0; rb < NB / 16; rb++) {
+        const d4 a = *reinterpret_cast<const d4 *>(a1col + 16 * rb + 4 * g);
+#pragma unroll
+        for (int v = 0; v < 4; v++) mx = fmax(mx, fabs(a[v]));
+      }
+    }
+    mx = fmax(mx, __shfl_xor(mx, 16, 64));
+    mx = fmax(mx, __shfl_xor(mx, 32, 64));
+    // the column's largest entry is < 2^e, so its 2-norm over the 2 NB rows
+    // of [A1; A2] (NB <= 256) is < 2^(e+5); the scaled norm is < 2^12
+    int e = 0;
+    if (mx > 0.0 && mx <= 1.7e308) (void)frexp(mx, &e);
+    const int sh = mx > 0.0 ? 7 - e : 0;
+    const double dsc = ldexp(1.0, sh);
+    xs = (float)dsc;
+    xu = (float)ldexp(1.0, -sh);
+#pragma unroll
+    for (int rb = 0; rb < NB / 16; rb++) {
+      const d4 x = *reinterpret_cast<const d4 *>(xcol + 16 * rb + 4 * g);
+      acc[rb] = f4{(float)(x[0] * dsc), (float)(x[1] * dsc), (float)(x[2] * dsc), (float)(x[3] * dsc)};
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+  }
+  const double dxu = (double)xu;
+  // before pass s: A1 rows of the accumulate sub-block s+1, T of s+2
+  auto pre = [&](int s) __attribute__((always_inline)) {
+    if (!UNIT) {
+      // rows past a ragged sub-block load a clamped neighbour: their W rows
+      // meet only zero rows and columns of T and are never stored
+      const SubBlk w1 = sbk(s + 1);
+#pragma unroll
+      for (int v = 0; v < 4; v++) a1v[v] = (float)a1col[min(w1.i0 + 4 * g + v, NB - 1)] * xs;
+    }
+    treg = tt_raw(T, ib, sbk(s + 2));
+    Wn[0] = f4{0.f, 0.f, 0.f, 0.f};
+    Wn[1] = Wn[0];
+  };
+  // after pass s: W2 of sub-block s+1 (FP32 T chain), its FP16 split for the
+  // next update phases, T^T of s+2 into the ring, A1 rows of s+1 after the
+  // barrier (every copy of the pass has landed before it)
+  auto post = [&](int s) __attribute__((always_inline)) {
+    f4 W = (Wn[0] + Wn[1]) * HV_UNSCALE;
+    if (!UNIT) W += a1v;
+    f4 t = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int sp = 0; sp < 4; sp++) t = mfma32(tto[sp], W[sp], t);
+    W2n = t;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      _Float16 h, l;
+      hsplit(-t[v] * HV_UNSCALE, h, l);
+      W2h[v] = h;
+      W2l[v] = l;
+    }
+    f32tbuf(S, (s + 2) & 3)[(tid >> 4) * LDT16 + (tid & 15)] = (float)(treg * tt_mask(sbk(s + 2)));
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (!UNIT) {
+      const SubBlk w1 = sbk(s + 1);
+      const f4 a1o = a1v - W2n;
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        if (4 * g + v < w1.kb) a1col[w1.i0 + 4 * g + v] = (double)a1o[v] * dxu;
+    }
+  };
+  // pass -1: W of sub-block 0; copies sub-block 2 into slot 2
+  pre(-1);
+  fpassh<NB, false, true, true>(acc, W2h, W2l, Wn, tto, hvbuf(S, 3), hvbuf(S, 0), f32tbuf(S, 0), hvbuf(S, 2), img(2));
+  post(-1);
+  // pass s: update with s (slot s&3), accumulate with s+1, copy s+3 into
+  // slot (s+3)&3 (read last by pass s-1)
+  for (int s = 0; s + 1 < nsub; s++) {
+    pre(s);
+    fpassh<NB, true, true, true>(acc, W2h, W2l, Wn, tto, hvbuf(S, s & 3), hvbuf(S, (s + 1) & 3),
+                                 f32tbuf(S, (s + 1) & 3), hvbuf(S, (s + 3) & 3), img(s + 3));
+    post(s);
+  }
+  fpassh<NB, true, false, false>(acc, W2h, W2l, Wn, tto, hvbuf(S, (nsub - 1) & 3), hvbuf(S, 0), f32tbuf(S, 0),
+                                 hvbuf(S, 0), img(0));
+#pragma unroll
+  for (int rb = 0; rb < NB / 16; rb++) {
+    const f4 x = acc[rb];
+    *reinterpret_cast<d4 *>(xcol + 16 * rb + 4 * g) =
+        d4{(double)x[0] * dxu, (double)x[1] * dxu, (double)x[2] * dxu, (double)x[3] * dxu};
+  }
+  // no copy may still be in flight when the next task reuses LDS
+  __builtin_amdgcn_s_waitcnt(0);
+}
+
+// PREC (hq_geom::prec) is a template parameter: each precision of the
+// updates is its own kernel instantiation with its own register allocation.
+template <int NB, int WB, int PREC = 0>
 __device__ __forceinline__ int2 task_larfb(Smem<NB> &S, const double *V, const double *T, double *C,
-                                           int ib, int slab, int prec = 0, const double *Wt = nullptr,
-                                           const double *Vd = nullptr, Ahead ah = Ahead{}) {
-  if constexpr (NB >= 128) {
-    if (prec)
-      task_update_f32<NB, true>(S, V, T, C, nullptr, ib, slab);
-    else if (Wt)
+                                           int ib, int slab, const double *Wt = nullptr,
+                                           const double *Vd = nullptr, Ahead ah = Ahead{},
+                                           const unsigned short *VHt = nullptr) {
+  if constexpr (NB >= 128 && PREC == 2) {
+    task_update_h<NB, true>(S, VHt, T, C, nullptr, ib, slab);
+  } else if constexpr (NB >= 128 && PREC == 1) {
+    task_update_f32<NB, true>(S, V, T, C, nullptr, ib, slab);
+  } else if constexpr (NB >= 128) {
+    if (Wt)
       task_larfb_w<NB>(S, Vd, Wt, C, slab);
     else
       return task_update_f<NB, true>(S, V, T, C, nullptr, ib, slab, ah);
@@ -1799,15 +2166,16 @@ __device__ __forceinline__ int2 task_larfb(Smem<NB> &S, const double *V, const d
   return make_int2(-1, 0);
 }
 
-template <int NB, int WB>
+template <int NB, int WB, int PREC = 0>
 __device__ __forceinline__ int2 task_ssrfb(Smem<NB> &S, double *A1, double *A2, const double *V,
-                                           const double *T, int ib, int slab, int prec = 0,
-                                           Ahead ah = Ahead{}) {
-  if constexpr (NB >= 128) {
-    if (prec)
-      task_update_f32<NB, false>(S, V, T, A2, A1, ib, slab);
-    else
-      return task_update_f<NB, false>(S, V, T, A2, A1, ib, slab, ah);
+                                           const double *T, int ib, int slab, Ahead ah = Ahead{},
+                                           const unsigned short *VHt = nullptr) {
+  if constexpr (NB >= 128 && PREC == 2) {
+    task_update_h<NB, false>(S, VHt, T, A2, A1, ib, slab);
+  } else if constexpr (NB >= 128 && PREC == 1) {
+    task_update_f32<NB, false>(S, V, T, A2, A1, ib, slab);
+  } else if constexpr (NB >= 128) {
+    return task_update_f<NB, false>(S, V, T, A2, A1, ib, slab, ah);
   } else {
     task_update<NB, WB, false>(S, V, T, A2, A1, ib, slab);
   }
@@ -1829,29 +2197,35 @@ __device__ __forceinline__ double *wd_ptr(const hq_geom &G, int k) {
 __device__ __forceinline__ double *vd_ptr(const hq_geom &G, int k) {
   return G.W ? G.VD + (size_t)k * G.nb * G.nb : nullptr;
 }
+// Images per tile: one per 16-reflector sub-block, ceil(nb/ib) ib blocks of
+// ceil(ib/16) sub-blocks (nb/16 when ib is a multiple of 16).
+__device__ __forceinline__ unsigned short *vh_ptr(const hq_geom &G, int i, int j) {
+  const size_t per_tile = (size_t)((G.nb + G.ib - 1) / G.ib) * ((G.ib + 15) / 16) * 64 * G.nb;
+  return G.VH ? G.VH + ((size_t)i + (size_t)j * G.MT) * per_tile : nullptr;
+}
 __device__ __forceinline__ double *tau_ptr(const hq_geom &G, int i, int j) {
   return G.TAU + ((size_t)i + (size_t)j * G.MT) * (size_t)G.nb;
 }
 
 // Returns the look-ahead of a slab task (see task_update_f), else (-1, 0).
-template <int NB, int WB>
+template <int NB, int WB, int PREC = 0>
 __device__ __forceinline__ int2 run_task(Smem<NB> &S, const hq_geom &G, const hq_task &t, Ahead ah = Ahead{}) {
   switch (t.kind) {
     case HQ_GEQRT:  // j = ib block of the panel, or -1: the whole tile
       if (t.j < 0)
         task_geqrt<NB, WB>(S, tile_ptr(G, t.k, t.k), t_ptr(G, t.k, t.k), tau_ptr(G, t.k, t.k), G.ib,
-                           wd_ptr(G, t.k), vd_ptr(G, t.k));
+                           wd_ptr(G, t.k), vd_ptr(G, t.k), PREC == 2 ? vh_ptr(G, t.k, t.k) : nullptr);
       else
         geqrt_panel<NB, WB>(S, tile_ptr(G, t.k, t.k), t_ptr(G, t.k, t.k), tau_ptr(G, t.k, t.k), G.ib,
-                            t.j * G.ib, wd_ptr(G, t.k), vd_ptr(G, t.k));
+                            t.j * G.ib, wd_ptr(G, t.k), vd_ptr(G, t.k), PREC == 2 ? vh_ptr(G, t.k, t.k) : nullptr);
       break;
     case HQ_TSQRT:
       if (t.j < 0)
         task_tsqrt<NB, WB>(S, tile_ptr(G, t.k, t.k), tile_ptr(G, t.r, t.k), t_ptr(G, t.r, t.k),
-                           tau_ptr(G, t.r, t.k), G.ib);
+                           tau_ptr(G, t.r, t.k), G.ib, PREC == 2 ? vh_ptr(G, t.r, t.k) : nullptr);
       else
         tsqrt_panel<NB, WB>(S, tile_ptr(G, t.k, t.k), tile_ptr(G, t.r, t.k), t_ptr(G, t.r, t.k),
-                            tau_ptr(G, t.r, t.k), G.ib, t.j * G.ib);
+                            tau_ptr(G, t.r, t.k), G.ib, t.j * G.ib, PREC == 2 ? vh_ptr(G, t.r, t.k) : nullptr);
       break;
     case HQ_GEQRT_U:
       task_geqrt_update<NB, WB>(S, tile_ptr(G, t.k, t.k), t_ptr(G, t.k, t.k), G.ib, t.j, t.slab);
@@ -1861,11 +2235,11 @@ __device__ __forceinline__ int2 run_task(Smem<NB> &S, const hq_geom &G, const hq
                                 t.slab);
       break;
     case HQ_LARFB:
-      return task_larfb<NB, WB>(S, tile_ptr(G, t.k, t.k), t_ptr(G, t.k, t.k), xtile_ptr(G, t.k, t.j), G.ib,
-                                t.slab, G.prec, wd_ptr(G, t.k), vd_ptr(G, t.k), ah);
+      return task_larfb<NB, WB, PREC>(S, tile_ptr(G, t.k, t.k), t_ptr(G, t.k, t.k), xtile_ptr(G, t.k, t.j), G.ib,
+                                      t.slab, wd_ptr(G, t.k), vd_ptr(G, t.k), ah, vh_ptr(G, t.k, t.k));
     case HQ_SSRFB:
-      return task_ssrfb<NB, WB>(S, xtile_ptr(G, t.k, t.j), xtile_ptr(G, t.r, t.j), tile_ptr(G, t.r, t.k),
-                                t_ptr(G, t.r, t.k), G.ib, t.slab, G.prec, ah);
+      return task_ssrfb<NB, WB, PREC>(S, xtile_ptr(G, t.k, t.j), xtile_ptr(G, t.r, t.j), tile_ptr(G, t.r, t.k),
+                                      t_ptr(G, t.r, t.k), G.ib, t.slab, ah, vh_ptr(G, t.r, t.k));
     default:
       break;
   }
@@ -1879,7 +2253,7 @@ __device__ __forceinline__ int2 run_task(Smem<NB> &S, const hq_geom &G, const hq
 // claimed by running workgroups, so every wait ends.
 // A batch passes one geometry per matrix (geoms[task.mat]); a single
 // factorisation passes geoms = nullptr and its geometry by value.
-template <int NB, int WB>
+template <int NB, int WB, int PREC>
 __global__ __launch_bounds__(HQ_THREADS) void sched_kernel(hq_geom G, const hq_geom *__restrict__ geoms,
                                                           const hq_task *__restrict__ tasks,
                                                           int ntasks, int *flags, int epoch_arg,
@@ -1966,9 +2340,9 @@ __global__ __launch_bounds__(HQ_THREADS) void sched_kernel(hq_geom G, const hq_g
       trace[4 * t + 3] = blockIdx.x;
     }
     if (geoms)
-      la = run_task<NB, WB>(S, geoms[tk.mat], tk, ah);
+      la = run_task<NB, WB, PREC>(S, geoms[tk.mat], tk, ah);
     else
-      la = run_task<NB, WB>(S, G, tk, ah);
+      la = run_task<NB, WB, PREC>(S, G, tk, ah);
     // each wave's stores have completed (a barrier waits for LDS only), so
     // none of them can land after the task's flag is published
     __builtin_amdgcn_s_waitcnt(0);
@@ -2283,7 +2657,19 @@ hipError_t hq_launch_sched(const hq_geom &G, const hq_geom *geoms, const hq_task
   hipLaunchKernelGGL(hq::reset_counter_kernel, dim3(1), dim3(64), 0, st, counter, epoch_dev);
   return hq_dispatch(G.nb, G.ib, [&](auto nbc, auto wbc) {
     HQ_NBWB
-    hipLaunchKernelGGL((hq::sched_kernel<NB, WB>), dim3(grid), dim3(HQ_THREADS), 0, st, G, geoms, tasks,
+    if constexpr (NB >= 128) {
+      if (G.prec == 2) {
+        hipLaunchKernelGGL((hq::sched_kernel<NB, WB, 2>), dim3(grid), dim3(HQ_THREADS), 0, st, G, geoms, tasks,
+                           ntasks, flags, epoch, counter, trace, (const int *)epoch_dev, wait_limit);
+        return;
+      }
+      if (G.prec == 1) {
+        hipLaunchKernelGGL((hq::sched_kernel<NB, WB, 1>), dim3(grid), dim3(HQ_THREADS), 0, st, G, geoms, tasks,
+                           ntasks, flags, epoch, counter, trace, (const int *)epoch_dev, wait_limit);
+        return;
+      }
+    }
+    hipLaunchKernelGGL((hq::sched_kernel<NB, WB, 0>), dim3(grid), dim3(HQ_THREADS), 0, st, G, geoms, tasks,
                        ntasks, flags, epoch, counter, trace, (const int *)epoch_dev, wait_limit);
   });
 }

@@ -47,7 +47,8 @@ struct hq_geom {
   double *X;    // matrix updated by LARFB / SSRFB: A itself while factorising,
                 // the Q / C array (MT x XNT tiles) while applying Q^T
   int XNT;
-  int prec;     // 0: FP64 everywhere; 1: LARFB / SSRFB in FP32 (panels FP64)
+  int prec;     // 0: FP64 everywhere; 1: LARFB / SSRFB in FP32; 2: in FP16 hi+lo splits
+                // with FP32 accumulation (panels FP64 in both)
   // Optional (FP64 factorisation, nb >= 128, ib % 16 == 0): for each
   // diagonal tile k, W = V T16 at W + k*nb*nb (T16 = the 16x16 diagonal
   // blocks of T) and its unit lower trapezoidal V at VD + k*nb*nb, written by
@@ -55,7 +56,17 @@ struct hq_geom {
   // itself.
   double *W;
   double *VD;
+  // Split-precision factorisation (prec 2): for each reflector tile (i,j),
+  // the FP16 hi / lo images of its 16-reflector sub-blocks as the update
+  // kernels stage them in LDS, written by the panels; tile (i,j) at
+  // VH + (i + j*MT) * hq_vh_tile(nb, ib) halves, sub-block s at + s*64*nb.
+  // nullptr otherwise.
+  unsigned short *VH;
 };
+
+// halves of split images per tile: one image of 64 nb halves per 16-reflector
+// sub-block, ceil(nb/ib) blocks of ceil(ib/16) sub-blocks
+inline size_t hq_vh_tile(int nb, int ib) { return (size_t)((nb + ib - 1) / ib) * ((ib + 15) / 16) * 64 * nb; }
 
 // One matrix of a batch for the layout kernels: m x n entries in an MT x NT
 // grid of nb x nb tiles (tile layout as hq_geom; entries past m or n are 0).

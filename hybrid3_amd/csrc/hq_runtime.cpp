@@ -356,6 +356,7 @@ struct Context {
   hipStream_t stream = nullptr;
   DevBuf flags, counter, T, A, TAU, X, T1;
   DevBuf W, VD;  // W = V T16 and unit lower V of the diagonal tiles (FP64 factorisation)
+  DevBuf VH;     // FP16 hi / lo images of the reflector sub-blocks (split factorisation)
   int epoch = 0;
   // Device-pointer calls share this context's buffers (claim counter,
   // flags, T / W / VD scratch, batch tables).  Each call waits for `busy`
@@ -514,7 +515,7 @@ int tile_list(Context *c, int MT, int NT, const int2 **out, int *n);
 int tile_qr_dev_impl(int MT, int NT, int nb, int ib, double *A, double *TAU, double *T,
                      void *stream, bool serial, long long *trace = nullptr, int prec = 0) {
   if (!geometry_ok(MT, NT, nb, ib)) return g_err;
-  if (prec && nb < 128) return set_error(HQ_ERR_ARG, "FP32 updates need nb >= 128 (nb=%d)", nb);
+  if (prec && nb < 128) return set_error(HQ_ERR_ARG, "reduced-precision updates need nb >= 128 (nb=%d)", nb);
   if (!A || !TAU) return set_error(HQ_ERR_ARG, "A and TAU must be device pointers");
   Context *c;
   if (get_ctx(&c) != HQ_OK) return g_err;
@@ -534,6 +535,10 @@ int tile_qr_dev_impl(int MT, int NT, int nb, int ib, double *A, double *TAU, dou
     if (c->W.reserve(tiles) != HQ_OK || c->VD.reserve(tiles) != HQ_OK) return g_err;
     G.W = (double *)c->W.p;
     G.VD = (double *)c->VD.p;
+  }
+  if (prec == 2) {
+    if (c->VH.reserve(2 * hq_vh_tile(nb, ik) * MT * NT) != HQ_OK) return g_err;
+    G.VH = (unsigned short *)c->VH.p;
   }
   if (serial) {
     std::vector<hq_task> tl;
@@ -986,6 +991,11 @@ int hq_tile_qr_mixed_dev(int MT, int NT, int nb, int ib, double *A, double *TAU,
   return tile_qr_dev_impl(MT, NT, nb, ib, A, TAU, T, stream, false, nullptr, 1);
 }
 
+int hq_tile_qr_split_dev(int MT, int NT, int nb, int ib, double *A, double *TAU, double *T,
+                         void *stream) {
+  return tile_qr_dev_impl(MT, NT, nb, ib, A, TAU, T, stream, false, nullptr, 2);
+}
+
 int hq_tile_qr_dev_serial(int MT, int NT, int nb, int ib, double *A, double *TAU, double *T,
                           void *stream) {
   return tile_qr_dev_impl(MT, NT, nb, ib, A, TAU, T, stream, true);
@@ -995,6 +1005,13 @@ int hq_tile_qr_dev_traced(int MT, int NT, int nb, int ib, double *A, double *TAU
                           long long *trace, void *stream) {
   if (!trace) return set_error(HQ_ERR_ARG, "trace must be a device buffer of 4*ntasks int64");
   return tile_qr_dev_impl(MT, NT, nb, ib, A, TAU, T, stream, false, trace);
+}
+
+int hq_tile_qr_traced_prec_dev(int MT, int NT, int nb, int ib, int prec, double *A, double *TAU, double *T,
+                               long long *trace, void *stream) {
+  if (!trace) return set_error(HQ_ERR_ARG, "trace must be a device buffer of 4*ntasks int64");
+  if (prec < 0 || prec > 2) return set_error(HQ_ERR_ARG, "prec=%d (0 FP64, 1 FP32, 2 split)", prec);
+  return tile_qr_dev_impl(MT, NT, nb, ib, A, TAU, T, stream, false, trace, prec);
 }
 
 int hq_schedule_tasks(int MT, int NT, int nb, int ib, int *out, int capacity) {

@@ -19,8 +19,9 @@ import numpy as np
 KINDS = ["GEQRT", "TSQRT", "LARFB", "SSRFB", "GEQRT_U", "TSQRT_U"]
 
 
-def run_traced(MT, NT, nb, ib, A, TAU, T=None, stream=None):
-    """Factor A (device tensor) with timestamps; returns (tasks[n,5], trace[n,4] in us)."""
+def run_traced(MT, NT, nb, ib, A, TAU, T=None, stream=None, prec=0):
+    """Factor A (device tensor) with timestamps; returns (tasks[n,5], trace[n,4] in us).
+    prec: trailing updates in FP64 (0), FP32 (1) or FP16 hi+lo splits (2)."""
     import torch
 
     import hybrid3_amd as hq
@@ -30,8 +31,8 @@ def run_traced(MT, NT, nb, ib, A, TAU, T=None, stream=None):
     hq.check(L.hq_schedule_tasks(MT, NT, nb, ib, tasks.ctypes.data_as(C.POINTER(C.c_int)),
                                  5 * ntasks))
     tr = torch.zeros(4 * ntasks, dtype=torch.int64, device=A.device)
-    hq.check(L.hq_tile_qr_dev_traced(MT, NT, nb, ib, hq._ptr(A), hq._ptr(TAU), hq._ptr(T),
-                                     hq._ptr(tr), hq._stream(stream)))
+    hq.check(L.hq_tile_qr_traced_prec_dev(MT, NT, nb, ib, prec, hq._ptr(A), hq._ptr(TAU), hq._ptr(T),
+                                          hq._ptr(tr), hq._stream(stream)))
     hq.sync(stream)
     khz = C.c_int(0)
     hq.check(L.hq_wall_clock_khz(C.byref(khz)))
@@ -114,6 +115,8 @@ def main():
     ap.add_argument("--critical-path", action="store_true",
                     help="also report the DAG's critical path with the measured durations")
     ap.add_argument("--raw", default="", help="write the task list and stamps (us) to this .npz")
+    ap.add_argument("--prec", type=int, default=0,
+                    help="trailing updates in FP64 (0), FP32 (1) or FP16 hi+lo splits (2)")
     args = ap.parse_args()
     import torch
 
@@ -123,9 +126,9 @@ def main():
     A = torch.empty(MT * NT * args.nb * args.nb, dtype=torch.float64, device="cuda")
     TAU = torch.zeros(MT * NT * args.nb, dtype=torch.float64, device="cuda")
     hq.fill_uniform_dev(MT, NT, args.nb, args.seed, A)
-    hq.tile_qr_dev(MT, NT, args.nb, args.ib, A, TAU)   # warm-up (schedule build, code load)
+    run_traced(MT, NT, args.nb, args.ib, A, TAU, prec=args.prec)   # warm-up (schedule build, code load)
     hq.fill_uniform_dev(MT, NT, args.nb, args.seed, A)
-    tasks, us = run_traced(MT, NT, args.nb, args.ib, A, TAU)
+    tasks, us = run_traced(MT, NT, args.nb, args.ib, A, TAU, prec=args.prec)
     _, grid = hq.schedule_info(MT, NT, args.nb, args.ib)
     s = summarize(tasks, us, grid)
     flops = 4.0 / 3.0 * args.n ** 3 if m == args.n else 2.0 * m * args.n ** 2 - 2.0 * args.n ** 3 / 3

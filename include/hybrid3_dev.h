@@ -90,6 +90,16 @@ int hq_tile_qr_noalloc_dev(int MT, int NT, int nb, int ib, double *A, double *TA
                            void *stream);
 int hq_tile_qr_noalloc_check(void *stream);
 
+/* Split precision (no reference counterpart): as hq_tile_qr_mixed_dev, with
+ * the trailing updates LARFB / SSRFB on FP16 MFMA with FP32 accumulation,
+ * every product formed as hi*hi + hi*lo + lo*hi of FP16 hi + lo splits (22
+ * significant bits), V scaled by 2^10 and each updated column by a power of
+ * two chosen from its 2-norm so that every split stays inside FP16's range.
+ * Panels (reflectors, T, R) stay FP64.  R carries FP32-level errors.
+ * nb in {128, 256}. */
+int hq_tile_qr_split_dev(int MT, int NT, int nb, int ib, double *A, double *TAU, double *T,
+                         void *stream);
+
 /* Same factorisation, one kernel launch per task in the scheduler's order
  * (stream order replaces the device dependency flags). */
 int hq_tile_qr_dev_serial(int MT, int NT, int nb, int ib, double *A,
@@ -102,6 +112,12 @@ int hq_tile_qr_dev_serial(int MT, int NT, int nb, int ib, double *A,
 int hq_tile_qr_dev_traced(int MT, int NT, int nb, int ib, double *A,
                           double *TAU, double *T, long long *trace,
                           void *stream);
+
+/* The same timeline for any precision of the trailing updates: prec 0
+ * (FP64, = hq_tile_qr_dev_traced), 1 (hq_tile_qr_mixed_dev), 2
+ * (hq_tile_qr_split_dev). */
+int hq_tile_qr_traced_prec_dev(int MT, int NT, int nb, int ib, int prec, double *A, double *TAU,
+                               double *T, long long *trace, void *stream);
 
 /* Task list in claim order: out[5*i..5*i+4] = kind (0 GEQRT, 1 TSQRT,
  * 2 LARFB, 3 SSRFB), step k, row r, column j, slab.  capacity >= 5*ntasks. */
