@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven-QR measurement")
     ap.add_argument("--batch-count", type=int, default=4000)
     ap.add_argument("--no-lsq", action="store_true", help="skip the mixed-precision least-squares part")
+    ap.add_argument("--no-batched-lsq", action="store_true", help="skip the batched least-squares part")
+    ap.add_argument("--batched-lsq-count", type=int, default=512)
     ap.add_argument("--no-tall", action="store_true", help="skip the 65536 x 4096 QR / dorgqr / dormqr part")
     ap.add_argument("--traffic-file",
                     default=os.path.join(ROOT, "profiles", "sched_kernel_traffic.txt"))
@@ -158,9 +160,11 @@ def tall_bench(m=65536, n=4096, nrhs=64, nb=256, ib=32, seed=20151130, reps=3):
 
 def lsq_bench(m=32768, n=8192, kappas=(1e2, 1e6, 1e10), nb=256, ib=32):
     """Least squares on A = U diag(sigma) V^T (U, V from seeded Gaussian QR,
-    sigma log-spaced to 1/kappa), b = A x_true + 1e-8 N(0,1): plain FP64
-    tile QR + refinement against mixed tile QR (FP32 updates) + refinement,
-    both refined with FP64 residuals until the step stagnates."""
+    sigma log-spaced to 1/kappa), b = A x_true + 1e-8 N(0,1): the plain FP64
+    tile QR against the mixed ones (FP32 updates; FP16 hi+lo split updates),
+    each refined with FP64 residuals by R-preconditioned CGLS.  A speed-up
+    over FP64 is reported only where the mixed solve converged to the FP64
+    solution's backward error."""
     import torch
     from hybrid3_amd import lsq
     out = []
@@ -170,26 +174,98 @@ def lsq_bench(m=32768, n=8192, kappas=(1e2, 1e6, 1e10), nb=256, ib=32):
         xt = torch.randn(n, 1, generator=g, dtype=torch.float64, device="cuda")
         b = A @ xt + 1e-8 * torch.randn(m, 1, generator=g, dtype=torch.float64, device="cuda")
         res = {"kappa": kappa}
-        sols = {}
-        for name, mixed in (("fp64", False), ("mixed", True)):
-            lsq.lstsq(A[:2048, :512].contiguous().t().contiguous().t(), b[:2048], nb=nb, ib=ib, mixed=mixed,
+        for prec in ("fp64", "fp32", "split"):
+            lsq.lstsq(A[:2048, :512].contiguous().t().contiguous().t(), b[:2048], nb=nb, ib=ib, prec=prec,
                       max_iter=2)  # warm the schedule caches of this path
-            x, info = lsq.lstsq(A, b, nb=nb, ib=ib, mixed=mixed, max_iter=30)
-            sols[name] = x
-            res[name] = {"factor_ms": round(info["factor_seconds"] * 1e3, 2),
+            x, info = lsq.lstsq(A, b, nb=nb, ib=ib, prec=prec, max_iter=300)
+            res[prec] = {"factor_ms": round(info["factor_seconds"] * 1e3, 2),
                          "solve_ms": round(info["total_seconds"] * 1e3, 2), "steps": info["steps"],
-                         "backward": lsq.backward_indicator(A, b, x),
+                         "converged": info["converged"], "backward": info["backward"],
                          "err_vs_x_true": (torch.linalg.norm(x - xt) / torch.linalg.norm(xt)).item()}
-        res["mixed_vs_fp64_rel_diff"] = (torch.linalg.norm(sols["mixed"] - sols["fp64"])
-                                         / torch.linalg.norm(sols["fp64"])).item()
-        res["speedup_mixed_vs_fp64"] = round(res["fp64"]["solve_ms"] / res["mixed"]["solve_ms"], 3)
+        for prec in ("fp32", "split"):
+            ok = res[prec]["converged"] and res[prec]["backward"] <= 10 * res["fp64"]["backward"]
+            res[f"speedup_{prec}_vs_fp64"] = (round(res["fp64"]["solve_ms"] / res[prec]["solve_ms"], 3)
+                                              if ok else None)
         out.append(res)
-        note(f"least squares kappa={kappa:g}: fp64 {res['fp64']['solve_ms']} ms, mixed {res['mixed']['solve_ms']} ms")
-        del A, b, sols
+        note(f"least squares kappa={kappa:g}: fp64 {res['fp64']['solve_ms']} ms, fp32 {res['fp32']['solve_ms']} "
+             f"ms ({res['fp32']['converged']}), split {res['split']['solve_ms']} ms ({res['split']['converged']})")
+        del A, b
         torch.cuda.empty_cache()
     return {"workload": f"least squares {m}x{n}, A = U diag(sigma) V^T, b = A x_true + 1e-8 N(0,1); "
-                        f"tile QR nb={nb} ib={ib} + FP64-residual refinement",
+                        f"tile QR nb={nb} ib={ib} + R-preconditioned CGLS with FP64 residuals",
             "qr_flops": 2.0 * m * n * n - 2.0 * n ** 3 / 3.0, "cases": out}
+
+
+def batched_lsq_problems(count=512, seed=5000, k=4):
+    """count problems m x n, m uniform over the multiples of 256 in [1024,
+    8192], n = m/4, kappa log-uniform in [1e1, 1e6], A = U diag(sigma) V^T
+    (U, V from seeded Gaussian QR), b = A x_true + 1e-8 N(0,1), k right-hand
+    sides; grouped by shape as hybrid3_amd.lsq.batched_lstsq takes them."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(seed)
+    ms = rng.integers(4, 33, count) * 256
+    kap = 10.0 ** rng.uniform(1, 6, count)
+    groups, xtrue, kappas = {}, {}, {}
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    for m in sorted(set(ms.tolist())):
+        idx = np.nonzero(ms == m)[0]
+        n = m // 4
+        B = len(idx)
+        U = torch.linalg.qr(torch.randn(B, m, n, generator=g, dtype=torch.float64, device="cuda"))[0]
+        V = torch.linalg.qr(torch.randn(B, n, n, generator=g, dtype=torch.float64, device="cuda"))[0]
+        t = torch.linspace(0, 1, n, dtype=torch.float64, device="cuda")
+        lk = torch.from_numpy(np.log10(kap[idx])).cuda()
+        sig = 10.0 ** (-lk[:, None] * t[None, :])
+        A = torch.bmm(U * sig[:, None, :], V.transpose(1, 2).contiguous())   # (B, m, n)
+        del U, V
+        xt = torch.randn(B, n, k, generator=g, dtype=torch.float64, device="cuda")
+        b = torch.bmm(A, xt) + 1e-8 * torch.randn(B, m, k, generator=g, dtype=torch.float64, device="cuda")
+        groups[(m, n)] = (A.transpose(1, 2).contiguous(), b)   # column-major storage
+        xtrue[(m, n)] = xt
+        kappas[(m, n)] = kap[idx]
+        del A
+    return groups, xtrue, kappas
+
+
+def batched_lsq_bench(count=512, nb=256, ib=32):
+    """Batched least squares: every problem factored in ONE launch (FP64, or
+    FP16 hi+lo split / FP32 trailing updates), refined by R-preconditioned
+    CGLS; time to solution and the worst per-problem backward error and
+    error against x_true."""
+    import torch
+    from hybrid3_amd import lsq
+    groups, xtrue, _ = batched_lsq_problems(count)
+    flops = float(sum(A.shape[0] * (2.0 * m * n * n - 2.0 * n ** 3 / 3.0) for (m, n), (A, _) in groups.items()))
+    res = {"workload": f"{count} least-squares problems, m uniform over multiples of 256 in [1024, 8192], n = m/4, "
+                       "kappa log-uniform in [1e1, 1e6], 4 right-hand sides, A = U diag(sigma) V^T; one batched "
+                       f"tile QR launch (nb={nb}, ib={ib}) + R-preconditioned CGLS with FP64 residuals",
+           "qr_flops": flops, "shapes": len(groups)}
+    small = {k: (A[:1], b[:1]) for k, (A, b) in list(groups.items())[:2]}
+    for prec in ("fp64", "fp32", "split"):
+        lsq.batched_lstsq(small, nb=nb, ib=ib, prec=prec, max_iter=2)   # warm-up
+        out, info = lsq.batched_lstsq(groups, nb=nb, ib=ib, prec=prec)
+        worst_be, worst_err = 0.0, 0.0
+        for sh, (A, b) in groups.items():
+            x = out[sh]
+            r = b - torch.bmm(A.transpose(1, 2), x)
+            atr = torch.linalg.vector_norm(torch.bmm(A, r), dim=1)
+            be = atr / (torch.linalg.matrix_norm(A)[:, None] * torch.linalg.vector_norm(r, dim=1))
+            worst_be = max(worst_be, be.max().item())
+            e = torch.linalg.vector_norm(x - xtrue[sh], dim=1) / torch.linalg.vector_norm(xtrue[sh], dim=1)
+            worst_err = max(worst_err, e.max().item())
+        res[prec] = {"factor_ms": round(info["factor_seconds"] * 1e3, 2),
+                     "solve_ms": round(info["total_seconds"] * 1e3, 2),
+                     "factor_tflops": round(flops / info["factor_seconds"] / 1e12, 3),
+                     "max_steps": max(info["steps"].values()), "converged": info["converged"],
+                     "max_backward": worst_be, "max_err_vs_x_true": worst_err}
+        note(f"batched least squares {prec}: {res[prec]['solve_ms']} ms (factor {res[prec]['factor_ms']} ms)")
+    for prec in ("fp32", "split"):
+        ok = res[prec]["converged"] and res[prec]["max_backward"] <= 10 * res["fp64"]["max_backward"]
+        res[f"speedup_{prec}_vs_fp64"] = round(res["fp64"]["solve_ms"] / res[prec]["solve_ms"], 3) if ok else None
+    del groups
+    torch.cuda.empty_cache()
+    return res
 
 
 def main():
@@ -298,6 +374,13 @@ def main():
             lsq_res = {"error": f"{type(e).__name__}: {e}"}
         note("mixed least squares done")
 
+    blsq = None
+    if not args.no_batched_lsq:
+        try:
+            blsq = batched_lsq_bench(args.batched_lsq_count)
+        except Exception as e:  # the headline line must still print
+            blsq = {"error": f"{type(e).__name__}: {e}"}
+
     cpu = None if args.no_cpu_baseline else cpu_baseline(nb, ib, args.seed)
     if cpu is not None:
         note(f"cpu baseline: {cpu['value']} TFLOP/s")
@@ -327,6 +410,7 @@ def main():
         "batched": batched,
         "tall": tall,
         "mixed_lsq": lsq_res,
+        "batched_lsq": blsq,
         "step_ms": [round(x, 3) for x in fact_ms],
     }
     print(json.dumps(out))

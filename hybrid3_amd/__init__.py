@@ -86,6 +86,12 @@ SIGNATURES = {
     "hq_tile_to_cm_dev": (C.c_int, [C.c_int] * 2 + [C.POINTER(C.c_int)] * 2
                           + [C.POINTER(_vp), C.POINTER(_vp), C.POINTER(C.c_int), _vp]),
     "hq_tile_qr_mixed_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp]),
+    "hq_batched_tile_qr_prec_dev": (C.c_int, [C.c_int] * 4 + [C.POINTER(C.c_int)] * 2 + [C.POINTER(_vp)] * 3
+                                    + [_vp]),
+    "hq_cm32_to_tile_dev": (C.c_int, [C.c_int] * 2 + [C.POINTER(C.c_int)] * 2
+                            + [C.POINTER(_vp), C.POINTER(C.c_int), C.POINTER(_vp), _vp]),
+    "hq_tile_to_cm32_dev": (C.c_int, [C.c_int] * 2 + [C.POINTER(C.c_int)] * 2
+                            + [C.POINTER(_vp), C.POINTER(_vp), C.POINTER(C.c_int), _vp]),
     "hq_tile_qr_split_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp]),
     "hq_tile_qr_mn_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp]),
     # mat_prod.h harness helpers
@@ -218,6 +224,24 @@ def batched_tile_qr_dev(nb, ib, MT, NT, A, TAU, T=None, stream=None):
     Tp = None if T is None else _parr(T)
     check(lib().hq_batched_tile_qr_dev(n, nb, ib, _iarr(MT), _iarr(NT), _parr(A), _parr(TAU), Tp,
                                        _stream(stream)))
+
+
+def batched_tile_qr_prec_dev(prec, nb, ib, MT, NT, A, TAU, T=None, stream=None):
+    """batched_tile_qr_dev with updates in FP64 (0), FP32 (1) or FP16 hi+lo splits (2)."""
+    n = len(MT)
+    Tp = None if T is None else _parr(T)
+    check(lib().hq_batched_tile_qr_prec_dev(prec, n, nb, ib, _iarr(MT), _iarr(NT), _parr(A), _parr(TAU), Tp,
+                                            _stream(stream)))
+
+
+def cm32_to_tile_dev(nb, m, n, Cs, ldc, A, stream=None):
+    check(lib().hq_cm32_to_tile_dev(len(m), nb, _iarr(m), _iarr(n), _parr(Cs), _iarr(ldc), _parr(A),
+                                    _stream(stream)))
+
+
+def tile_to_cm32_dev(nb, m, n, A, Cs, ldc, stream=None):
+    check(lib().hq_tile_to_cm32_dev(len(m), nb, _iarr(m), _iarr(n), _parr(A), _parr(Cs), _iarr(ldc),
+                                    _stream(stream)))
 
 
 def batched_plan_dev(nb, ib, MT, NT, A, TAU, T=None):

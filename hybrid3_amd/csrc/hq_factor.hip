@@ -2617,8 +2617,12 @@ __global__ __launch_bounds__(256) void batched_layout_kernel(const hq_bmat *__re
       b.A[e] = in ? uniform_pm1(b.seed, col * b.m + row) : 0.0;
     } else if (OP == 1) {
       b.A[e] = in ? b.C[row + col * (size_t)b.ld] : 0.0;
-    } else {
+    } else if (OP == 2) {
       if (in) b.C[row + col * (size_t)b.ld] = b.A[e];
+    } else if (OP == 3) {  // FP32 column-major -> FP64 tile layout
+      b.A[e] = in ? (double)b.Cf[row + col * (size_t)b.ld] : 0.0;
+    } else {               // FP64 tile layout -> FP32 column-major
+      if (in) b.Cf[row + col * (size_t)b.ld] = (float)b.A[e];
     }
   }
 }
@@ -2684,8 +2688,12 @@ hipError_t hq_launch_batched_layout(int op, const hq_bmat *mats, int count, int 
     hipLaunchKernelGGL(hq::batched_layout_kernel<0>, grid, dim3(256), 0, st, mats, nb);
   else if (op == 1)
     hipLaunchKernelGGL(hq::batched_layout_kernel<1>, grid, dim3(256), 0, st, mats, nb);
-  else
+  else if (op == 2)
     hipLaunchKernelGGL(hq::batched_layout_kernel<2>, grid, dim3(256), 0, st, mats, nb);
+  else if (op == 3)
+    hipLaunchKernelGGL(hq::batched_layout_kernel<3>, grid, dim3(256), 0, st, mats, nb);
+  else
+    hipLaunchKernelGGL(hq::batched_layout_kernel<4>, grid, dim3(256), 0, st, mats, nb);
   return hipGetLastError();
 }
 

@@ -73,6 +73,7 @@ inline size_t hq_vh_tile(int nb, int ib) { return (size_t)((nb + ib - 1) / ib) *
 struct hq_bmat {
   double *A;      // tile layout
   double *C;      // column-major m x n, leading dimension ld (conversions)
+  float *Cf;      // the same in FP32 (FP32 conversions)
   int m, n, MT, NT, ld;
   uint64_t seed;  // uniform fill: value(row, col) from (seed, col*m + row)
 };

@@ -374,7 +374,7 @@ struct Context {
   std::map<std::pair<int, int>, std::unique_ptr<DevBuf>> tilelists;  // (MT, NT) -> int2 list
   // batches: key = (nb, ib, MT_0, NT_0, MT_1, NT_1, ...)
   std::map<std::vector<int>, std::unique_ptr<CachedSched>> batch_scheds;
-  DevBuf geoms, bmats, Tbatch, Wbatch, VDbatch;
+  DevBuf geoms, bmats, Tbatch, Wbatch, VDbatch, VHbatch;
   std::map<int, std::unique_ptr<struct BatchPlan>> plans;  // prepared batches
   int next_plan = 1;
 };
@@ -692,12 +692,17 @@ int batch_args_ok(int count, int nb, int ib, const int *MT, const int *NT, doubl
 // are carved from Tb / Wb / VDb, reserved here.
 int batch_geoms(int count, int nb, int ib, const int *MT, const int *NT, double *const *A,
                 double *const *TAU, double *const *T, DevBuf &Tb, DevBuf &Wb, DevBuf &VDb,
-                std::vector<hq_geom> &g) {
+                std::vector<hq_geom> &g, int prec = 0, DevBuf *VHb = nullptr) {
   size_t tdoubles = 0;
   for (int i = 0; i < count; i++)
     if (!T || !T[i]) tdoubles += (size_t)MT[i] * NT[i] * ib * nb;
   if (tdoubles && Tb.reserve(sizeof(double) * tdoubles) != HQ_OK) return g_err;
-  const bool wpath = nb >= 128 && ib % 16 == 0;
+  const bool wpath = !prec && nb >= 128 && ib % 16 == 0;
+  size_t vhalves = 0;
+  if (prec == 2) {
+    for (int i = 0; i < count; i++) vhalves += hq_vh_tile(nb, ib) * MT[i] * NT[i];
+    if (VHb->reserve(2 * vhalves) != HQ_OK) return g_err;
+  }
   size_t wdoubles = 0;
   if (wpath) {
     for (int i = 0; i < count; i++) wdoubles += (size_t)std::min(MT[i], NT[i]) * nb * nb;
@@ -705,14 +710,18 @@ int batch_geoms(int count, int nb, int ib, const int *MT, const int *NT, double 
       return g_err;
   }
   g.resize(count);
-  size_t toff = 0, woff = 0;
+  size_t toff = 0, woff = 0, vhoff = 0;
   for (int i = 0; i < count; i++) {
     double *Ti = T ? T[i] : nullptr;
     if (!Ti) {
       Ti = (double *)Tb.p + toff;
       toff += (size_t)MT[i] * NT[i] * ib * nb;
     }
-    g[i] = hq_geom{A[i], TAU[i], Ti, MT[i], NT[i], nb, ib, A[i], NT[i]};
+    g[i] = hq_geom{A[i], TAU[i], Ti, MT[i], NT[i], nb, ib, A[i], NT[i], prec};
+    if (prec == 2) {
+      g[i].VH = (unsigned short *)VHb->p + vhoff;
+      vhoff += hq_vh_tile(nb, ib) * MT[i] * NT[i];
+    }
     if (wpath) {
       g[i].W = (double *)Wb.p + woff;
       g[i].VD = (double *)VDb.p + woff;
@@ -723,14 +732,16 @@ int batch_geoms(int count, int nb, int ib, const int *MT, const int *NT, double 
 }
 
 int batched_qr_impl(int count, int nb, int ib, const int *MT, const int *NT, double *const *A,
-                    double *const *TAU, double *const *T, void *stream) {
+                    double *const *TAU, double *const *T, void *stream, int prec = 0) {
   if (batch_args_ok(count, nb, ib, MT, NT, A, TAU) != HQ_OK) return g_err;
+  if (prec < 0 || prec > 2) return set_error(HQ_ERR_ARG, "prec=%d (0 FP64, 1 FP32, 2 split)", prec);
+  if (prec && nb < 128) return set_error(HQ_ERR_ARG, "reduced-precision updates need nb >= 128 (nb=%d)", nb);
   Context *c;
   if (get_ctx(&c) != HQ_OK) return g_err;
   hipStream_t st = pick_stream(c, stream);
   if (order_in(c, st) != HQ_OK) return g_err;
   std::vector<hq_geom> g;
-  if (batch_geoms(count, nb, ib, MT, NT, A, TAU, T, c->Tbatch, c->Wbatch, c->VDbatch, g) != HQ_OK)
+  if (batch_geoms(count, nb, ib, MT, NT, A, TAU, T, c->Tbatch, c->Wbatch, c->VDbatch, g, prec, &c->VHbatch) != HQ_OK)
     return g_err;
   CachedSched *cs;
   if (get_batch_schedule(c, count, nb, ib, MT, NT, &cs) != HQ_OK) return g_err;
@@ -810,7 +821,10 @@ int batched_layout_impl(int op, int count, int nb, const int *m, const int *n, c
     } else {
       if (!C || !C[i] || !ldc || ldc[i] < m[i])
         return set_error(HQ_ERR_ARG, "matrix %d: column-major pointer and ld >= m required", i);
-      b.C = (double *)C[i];
+      if (op >= 3)
+        b.Cf = (float *)C[i];
+      else
+        b.C = (double *)C[i];
       b.ld = ldc[i];
     }
     max_tiles = std::max(max_tiles, b.MT * b.NT);
@@ -1188,6 +1202,11 @@ int hq_batched_tile_qr_dev(int count, int nb, int ib, const int *MT, const int *
   return batched_qr_impl(count, nb, ib, MT, NT, A, TAU, T, stream);
 }
 
+int hq_batched_tile_qr_prec_dev(int prec, int count, int nb, int ib, const int *MT, const int *NT,
+                                double *const *A, double *const *TAU, double *const *T, void *stream) {
+  return batched_qr_impl(count, nb, ib, MT, NT, A, TAU, T, stream, prec);
+}
+
 int hq_batched_plan_dev(int count, int nb, int ib, const int *MT, const int *NT, double *const *A,
                         double *const *TAU, double *const *T, int *plan) {
   return batched_plan_impl(count, nb, ib, MT, NT, A, TAU, T, plan);
@@ -1231,6 +1250,17 @@ int hq_batched_fill_uniform_dev(int count, int nb, const int *m, const int *n, c
 int hq_cm_to_tile_dev(int count, int nb, const int *m, const int *n, const double *const *C,
                       const int *ldc, double *const *A, void *stream) {
   return batched_layout_impl(1, count, nb, m, n, nullptr, C, ldc, A, stream);
+}
+
+int hq_cm32_to_tile_dev(int count, int nb, const int *m, const int *n, const float *const *C, const int *ldc,
+                        double *const *A, void *stream) {
+  return batched_layout_impl(3, count, nb, m, n, nullptr, (const double *const *)C, ldc, A, stream);
+}
+
+int hq_tile_to_cm32_dev(int count, int nb, const int *m, const int *n, const double *const *A, float *const *C,
+                        const int *ldc, void *stream) {
+  return batched_layout_impl(4, count, nb, m, n, nullptr, (const double *const *)C, ldc, (double *const *)A,
+                             stream);
 }
 
 int hq_tile_to_cm_dev(int count, int nb, const int *m, const int *n, const double *const *A, double *const *C,

@@ -1,41 +1,63 @@
-"""Least squares min ||A x - b|| on one GPU with the tile QR engine, and
-iterative refinement to FP64 accuracy (no reference counterpart).
+"""Least squares min ||A x - b|| on one GPU with the tile QR engine, refined
+to FP64 accuracy (no reference counterpart).
 
-The factorisation is either the plain FP64 tile QR or the mixed one (FP64
-panels, FP32 LARFB / SSRFB updates).  The solution is refined on the
-augmented system [I A; A^T 0] [r; x] = [b; 0] with FP64 residuals
-(Bjorck's refinement): per step
+The factorisation A = QR is the plain FP64 tile QR, or a mixed one: FP64
+panels with the trailing updates in FP32 ("fp32") or in FP16 hi + lo splits
+("split").  Two refinements with FP64 residuals are offered:
 
-    f = b - r - A x,  g = -A^T r                      (FP64, original A)
-    d = Q^T f,  h = R^-T g,  dx = R^-1 (d[:n] - h)    (the computed factors)
-    dr = f - A dx;  x += dx;  r += dr
+* ``method="cgls"`` (default): CGLS on min ||A R^-1 y - b||, x = R^-1 y,
+  started from the semi-normal solution x0 = R^-1 R^-T A^T b.  Every step
+  takes two products with A (the residual is recomputed from x, not
+  updated), one with A^T and two with R^-1 (formed once by a triangular
+  solve); Q is not needed.  With R from a factorisation of relative accuracy u_f,
+  A R^-1 has condition number about 1 + kappa(A) u_f, so the iteration
+  converges in a few steps while kappa u_f << 1, and still converges (more
+  slowly) beyond it, where classical refinement stalls.
+* ``method="refine"``: Bjorck's refinement of the augmented system
+  [I A; A^T 0][r; x] = [b; 0]: f = b - r - A x, g = -A^T r,
+  dx = R^-1 (Q^T f - R^-T g), dr = f - A dx.  Needs Q^T (the library's tile
+  ormqr) per step; converges while kappa u_f < 1.
 
-The fixed point satisfies the FP64 normal equations A^T (b - A x) = 0 up to
-FP64 rounding whatever the accuracy of Q and R; the factors only set the
-convergence rate, about kappa * u of the factorisation, so the mixed
-factorisation converges while kappa * u_fp32 < 1.
-
-A is given column-major (m x n, m >= n, FP64, on the device).  Q^T is applied
-with the library's tile ormqr (the reflectors and T of the factorisation);
-the triangular solves with R and the products with A use torch (rocBLAS):
-they are the refinement's residual arithmetic, not the factorisation.
+Both stop when the iteration makes no further progress or after
+``max_iter`` steps.  ``converged`` says whether the result reached the
+rounding floor of its FP64 residual: a normwise backward-error indicator
+||A^T r|| / (||A||_F ||r||) within eps (||b|| + ||A||_F ||x||) / ||r|| (for
+the refinement, also a relative update at FP64 rounding).  A result that did
+not converge carries the error it reports.
+A is column-major (m x n, m >= n, FP64, on the device).  The triangular
+solves and the products with A are torch (rocBLAS): they are the
+refinement's residual arithmetic, not the factorisation.
 """
 from __future__ import annotations
 
 import time
 
-from . import cm_to_tile_dev, sync, tile_ormqr_dev, tile_qr_dev, tile_qr_mixed_dev, tile_to_cm_dev
+from . import (batched_tile_qr_prec_dev, cm_to_tile_dev, sync, tile_ormqr_dev, tile_qr_dev, tile_qr_mixed_dev,
+               tile_qr_split_dev, tile_to_cm_dev)
 
-__all__ = ["TileQR", "lstsq", "backward_indicator", "gaussian_orthonormal", "svd_matrix"]
+__all__ = ["TileQR", "lstsq", "batched_lstsq", "backward_indicator", "gaussian_orthonormal", "svd_matrix",
+           "PRECISIONS"]
+
+PRECISIONS = {"fp64": 0, "fp32": 1, "split": 2}
+EPS = 2.0 ** -52
+
+
+def _prec(prec, mixed):
+    if prec is None:
+        prec = "fp32" if mixed else "fp64"
+    if prec not in PRECISIONS:
+        raise ValueError(f"prec must be one of {sorted(PRECISIONS)}")
+    return prec
 
 
 class TileQR:
     """Tile QR of a column-major m x n device matrix (zero-padded to tiles)."""
 
-    def __init__(self, A, nb=256, ib=32, mixed=False):
+    def __init__(self, A, nb=256, ib=32, mixed=False, prec=None):
         import torch
         m, n = A.shape
-        self.m, self.n, self.nb, self.ib, self.mixed = m, n, nb, ib, mixed
+        self.prec = _prec(prec, mixed)
+        self.m, self.n, self.nb, self.ib = m, n, nb, ib
         self.MT, self.NT = (m + nb - 1) // nb, (n + nb - 1) // nb
         MT, NT = self.MT, self.NT
         self.RY = torch.empty(MT * NT * nb * nb, dtype=torch.float64, device=A.device)
@@ -43,8 +65,10 @@ class TileQR:
         self.T = torch.zeros(MT * NT * ib * nb, dtype=torch.float64, device=A.device)
         Ac = A if _is_colmajor(A) else A.t().contiguous().t()
         cm_to_tile_dev(nb, [m], [n], [_addr(Ac)], [m], [self.RY])
+        fn = {"fp64": tile_qr_dev, "fp32": tile_qr_mixed_dev, "split": tile_qr_split_dev}[self.prec]
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
-        (tile_qr_mixed_dev if mixed else tile_qr_dev)(MT, NT, nb, ib, self.RY, self.TAU, self.T)
+        fn(MT, NT, nb, ib, self.RY, self.TAU, self.T)
         sync()
         torch.cuda.synchronize()
         self.factor_seconds = time.perf_counter() - t0
@@ -76,18 +100,87 @@ def _addr(t):
     return t.data_ptr()
 
 
-def lstsq(A, b, nb=256, ib=32, mixed=False, max_iter=12, tol=None, factor=None):
+def _cgls(Amul, ATmul, rsolve, rtsolve, b, max_iter, normA, patience=5):
+    """R-preconditioned CGLS on every column of b (device tensors with the
+    right-hand sides in the last dimension; normA: ||A||_F, a scalar or one
+    per problem broadcasting against (.., 1, k)).
+
+    Each step measures every column's normwise backward-error indicator
+    eta = ||A^T r|| / (||A||_F ||r||) on its true residual and keeps the
+    iterate with the smallest eta.  It stops after `patience` steps without
+    progress (no new minimum of eta, and no relative update below half the
+    smallest so far, in any column), or after max_iter steps.  Converged
+    means every column's best eta is within the rounding floor of its FP64
+    residual, eps (||b|| + ||A||_F ||x||) / ||r||.
+    Returns (x, steps, history, converged)."""
+    import torch
+
+    def norms(v):
+        return torch.linalg.vector_norm(v, dim=-2, keepdim=True)
+
+    x = rsolve(rtsolve(ATmul(b)))              # semi-normal start
+    r = b - Amul(x)
+    atr = ATmul(r)
+    s = rtsolve(atr)
+    p = s
+    gam = (s * s).sum(dim=-2, keepdim=True)
+    eta = norms(atr) / (normA * norms(r))
+    best_eta, best_x = eta, x
+    best_dx = None
+    hist, steps, since = [], 0, 0
+    for _ in range(max_iter):
+        t = rsolve(p)
+        q = Amul(t)
+        qq = (q * q).sum(dim=-2, keepdim=True)
+        alpha = torch.where(qq > 0, gam / torch.where(qq > 0, qq, torch.ones_like(qq)), torch.zeros_like(qq))
+        dx = alpha * t
+        x = x + dx
+        # the true residual, not the recursive r - alpha q: the recursion
+        # drifts by ~eps ||A|| ||x|| per step, which for a small residual
+        # would bound the attainable accuracy of x at kappa times that drift
+        r = b - Amul(x)
+        atr = ATmul(r)
+        s = rtsolve(atr)
+        gn = (s * s).sum(dim=-2, keepdim=True)
+        beta = torch.where(gam > 0, gn / torch.where(gam > 0, gam, torch.ones_like(gam)), torch.zeros_like(gam))
+        p = s + beta * p
+        gam = gn
+        steps += 1
+        eta = norms(atr) / (normA * norms(r))
+        better = eta < best_eta
+        # progress: a smaller eta in some column, or the update still
+        # shrinking geometrically (eta need not fall monotonically while x
+        # converges)
+        rel = norms(dx) / torch.where(norms(x) > 0, norms(x), torch.ones_like(eta))
+        shrink = rel < 0.5 * best_dx if best_dx is not None else torch.ones_like(better)
+        best_dx = rel if best_dx is None else torch.where(shrink, rel, best_dx)
+        since = 0 if bool(better.any()) or bool(shrink.any()) else since + 1
+        best_x = torch.where(better, x, best_x)
+        best_eta = torch.where(better, eta, best_eta)
+        hist.append({"eta": eta.max().item(), "dx_rel": rel.max().item()})
+        if since >= patience:
+            break
+    r = b - Amul(best_x)
+    floor = EPS * (norms(b) + normA * norms(best_x)) / norms(r)
+    conv = since >= patience and bool((best_eta <= floor).all())
+    return best_x, steps, hist, conv
+
+
+def lstsq(A, b, nb=256, ib=32, mixed=False, max_iter=None, tol=None, factor=None, prec=None, method="cgls"):
     """x minimising ||A x - b||_2 (A m x n, b m x k, FP64 torch device tensors).
 
     Returns (x, info): info holds the factorisation and total times, the
-    refinement steps taken, and per step ||dx|| / ||x|| and the normwise
+    refinement steps taken, per step ||dx|| / ||x||, whether the iteration
+    reached FP64 rounding (``converged``), and the final normwise
     backward-error indicator ||A^T r|| / (||A||_F ||r||)."""
     import torch
     m, n = A.shape
     if b.dim() == 1:
         b = b[:, None]
+    tol = 4.0 * EPS if tol is None else tol
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    F = factor if factor is not None else TileQR(A, nb, ib, mixed)
+    F = factor if factor is not None else TileQR(A, nb, ib, mixed, prec)
     R = F.R
 
     def rsolve(c):
@@ -96,26 +189,49 @@ def lstsq(A, b, nb=256, ib=32, mixed=False, max_iter=12, tol=None, factor=None):
     def rtsolve(c):
         return torch.linalg.solve_triangular(R.t(), c, upper=False)
 
-    x = rsolve(F.qt(b)[:n])
-    r = b - A @ x
-    steps, hist = 0, []
-    tol = 2.0 ** -52 if tol is None else tol
-    for it in range(max_iter):
-        f = b - r - A @ x
-        g = -(A.t() @ r)
-        dx = rsolve(F.qt(f)[:n] - rtsolve(g))
-        dr = f - A @ dx
-        x = x + dx
-        r = r + dr
-        steps += 1
-        rel = (torch.linalg.norm(dx) / torch.linalg.norm(x)).item()
-        hist.append({"dx_rel": rel, "backward": backward_indicator(A, b, x)})
-        # converged, or stagnated at the rounding level of the problem
-        if rel <= tol or (it > 0 and rel > 0.5 * hist[-2]["dx_rel"]):
-            break
+    if method == "cgls":
+        # the preconditioner as an explicit inverse (one BLAS-3 triangular
+        # solve): every step is then matrix products only.  It need not be
+        # exact -- x is updated directly and its residual formed with A.
+        Ri = torch.linalg.solve_triangular(R, torch.eye(n, dtype=R.dtype, device=R.device), upper=True)
+        RiT = Ri.t().contiguous()
+        x, steps, hist, conv = _cgls(lambda v: A @ v, lambda v: A.t() @ v, lambda c: Ri @ c, lambda c: RiT @ c, b,
+                                     100 if max_iter is None else max_iter, torch.linalg.norm(A))
+    elif method == "refine":
+        max_iter = 30 if max_iter is None else max_iter
+        x = rsolve(F.qt(b)[:n])
+        r = b - A @ x
+        steps, hist, conv = 0, [], False
+        stall = 0
+        for it in range(max_iter):
+            f = b - r - A @ x
+            g = -(A.t() @ r)
+            dx = rsolve(F.qt(f)[:n] - rtsolve(g))
+            dr = f - A @ dx
+            x = x + dx
+            r = r + dr
+            steps += 1
+            rel = (torch.linalg.norm(dx) / torch.linalg.norm(x)).item()
+            hist.append({"dx_rel": rel})
+            if rel <= tol:
+                conv = True
+                break
+            # no progress for three steps: at the rounding floor, or stalled
+            # (kappa u_f >= 1), which `converged` tells apart
+            stall = stall + 1 if it > 0 and rel >= 0.9 * hist[-2]["dx_rel"] else 0
+            if stall >= 3:
+                # stalled: at the rounding floor if the backward indicator
+                # is within the floor of the FP64 residual
+                rr = torch.linalg.norm(b - A @ x)
+                floor = EPS * (torch.linalg.norm(b) + torch.linalg.norm(A) * torch.linalg.norm(x)) / rr
+                conv = backward_indicator(A, b, x) <= floor.item()
+                break
+    else:
+        raise ValueError("method must be 'cgls' or 'refine'")
     torch.cuda.synchronize()
     info = {"factor_seconds": F.factor_seconds, "total_seconds": time.perf_counter() - t0,
-            "steps": steps, "history": hist, "mixed": mixed}
+            "steps": steps, "history": hist, "prec": F.prec, "mixed": F.prec != "fp64", "method": method,
+            "converged": conv, "backward": backward_indicator(A, b, x)}
     return x, info
 
 
@@ -156,3 +272,67 @@ def svd_matrix(m, n, kappa, seed, nb=256, ib=32):
     sigma = torch.logspace(0, -float(torch.log10(torch.tensor(kappa))), n, dtype=torch.float64, device="cuda")
     A = (U * sigma[None, :]) @ V.t().contiguous()
     return A.t().contiguous().t()  # column-major
+
+
+# ---------------------------------------------------------------- batches
+def batched_lstsq(groups, nb=256, ib=32, prec="fp64", max_iter=100):
+    """Many least-squares problems: groups maps a shape (m, n) to (A, b) with
+    A of shape (B, n, m) (each A[i] the column-major storage of an m x n
+    matrix) and b of shape (B, m, k).  Every problem of every group is
+    factored in ONE batched launch, then refined by R-preconditioned CGLS
+    group by group (batched products and triangular solves).
+
+    Returns ({shape: x (B, n, k)}, info)."""
+    import torch
+    pv = PRECISIONS[prec]
+    shapes = list(groups)
+    MT, NT, RY, TAU, cols, owners = [], [], [], [], [], []
+    for sh in shapes:
+        m, n = sh
+        A, _ = groups[sh]
+        B = A.shape[0]
+        mt, nt = (m + nb - 1) // nb, (n + nb - 1) // nb
+        ry = torch.empty(B, mt * nt * nb * nb, dtype=torch.float64, device="cuda")
+        ta = torch.zeros(B, mt * nt * nb, dtype=torch.float64, device="cuda")
+        cm_to_tile_dev(nb, [m] * B, [n] * B, [A[i].data_ptr() for i in range(B)], [m] * B,
+                       [ry[i].data_ptr() for i in range(B)])
+        RY.append(ry)
+        TAU.append(ta)
+        for i in range(B):
+            MT.append(mt)
+            NT.append(nt)
+            cols.append(ry[i].data_ptr())
+            owners.append(ta[i].data_ptr())
+    sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batched_tile_qr_prec_dev(pv, nb, ib, MT, NT, cols, owners)
+    sync()
+    torch.cuda.synchronize()
+    t_fact = time.perf_counter() - t0
+    out, steps, conv = {}, {}, True
+    for g, sh in enumerate(shapes):
+        m, n = sh
+        A, b = groups[sh]
+        B = A.shape[0]
+        mt, nt = (m + nb - 1) // nb, (n + nb - 1) // nb
+        full = torch.empty(B, nt * nb, mt * nb, dtype=torch.float64, device="cuda")
+        tile_to_cm_dev(nb, [mt * nb] * B, [nt * nb] * B, [RY[g][i].data_ptr() for i in range(B)],
+                       [full[i].data_ptr() for i in range(B)], [mt * nb] * B)
+        sync()
+        R = torch.triu(full[:, :n, :n].transpose(1, 2)).contiguous()
+        del full
+        At = A.transpose(1, 2)      # (B, m, n) view of the column-major storage
+        eye = torch.eye(n, dtype=R.dtype, device=R.device).expand(B, n, n)
+        Ri = torch.linalg.solve_triangular(R, eye, upper=True)   # preconditioner, explicit
+        RiT = Ri.transpose(1, 2).contiguous()
+        normA = torch.linalg.matrix_norm(A).view(B, 1, 1)
+        x, st, _, cv = _cgls(lambda v: torch.bmm(At, v), lambda v: torch.bmm(A, v), lambda c: torch.bmm(Ri, c),
+                             lambda c: torch.bmm(RiT, c), b, max_iter, normA)
+        out[sh] = x
+        steps[sh] = st
+        conv = conv and cv
+    torch.cuda.synchronize()
+    info = {"factor_seconds": t_fact, "total_seconds": time.perf_counter() - t0, "steps": steps,
+            "converged": conv, "prec": prec}
+    return out, info

@@ -201,6 +201,12 @@ int hq_batched_tile_qr_dev(int count, int nb, int ib, const int *MT, const int *
                            double *const *A, double *const *TAU, double *const *T,
                            void *stream);
 
+/* hq_batched_tile_qr_dev with the trailing updates of every matrix in FP64
+ * (prec 0), FP32 (1, as hq_tile_qr_mixed_dev) or FP16 hi+lo splits (2, as
+ * hq_tile_qr_split_dev); nb >= 128 for prec 1 and 2. */
+int hq_batched_tile_qr_prec_dev(int prec, int count, int nb, int ib, const int *MT, const int *NT,
+                                double *const *A, double *const *TAU, double *const *T, void *stream);
+
 /* The same factorisation as a plan prepared once and run many times (no
  * counterpart in the reference; for callers that capture the launch in a
  * hipGraph and replay it, e.g. a serving loop over fixed buffers).
@@ -233,6 +239,12 @@ int hq_cm_to_tile_dev(int count, int nb, const int *m, const int *n, const doubl
                       const int *ldc, double *const *A, void *stream);
 int hq_tile_to_cm_dev(int count, int nb, const int *m, const int *n, const double *const *A,
                       double *const *C, const int *ldc, void *stream);
+/* The same for FP32 matrices (a batch of FP32 problems stored as FP32): the
+ * tile layout the kernels factor is FP64, converted on the way in and out. */
+int hq_cm32_to_tile_dev(int count, int nb, const int *m, const int *n, const float *const *C,
+                        const int *ldc, double *const *A, void *stream);
+int hq_tile_to_cm32_dev(int count, int nb, const int *m, const int *n, const double *const *A,
+                        float *const *C, const int *ldc, void *stream);
 
 /* Bound of the scheduler's dependency waits on this device, in polls of
  * about a microsecond (0 = the default 2^22, seconds).  A wait that reaches
