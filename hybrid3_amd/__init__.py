@@ -58,6 +58,7 @@ SIGNATURES = {
     "hq_tile_qr_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp]),
     "hq_tile_qr_dev_serial": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp]),
     "hq_schedule_info": (C.c_int, [C.c_int] * 4 + [C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "hq_schedule_info_prec": (C.c_int, [C.c_int] * 5 + [C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "hq_dgeqrt_dev": (C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
     "hq_dtsqrt_dev": (C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp]),
     "hq_dormqr_dev": (C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
@@ -66,6 +67,7 @@ SIGNATURES = {
     "hq_tile_qr_dev_traced": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp, _vp]),
     "hq_tile_qr_traced_prec_dev": (C.c_int, [C.c_int] * 5 + [_vp, _vp, _vp, _vp, _vp]),
     "hq_schedule_tasks": (C.c_int, [C.c_int] * 4 + [C.POINTER(C.c_int), C.c_int]),
+    "hq_schedule_tasks_prec": (C.c_int, [C.c_int] * 5 + [C.POINTER(C.c_int), C.c_int]),
     "hq_wall_clock_khz": (C.c_int, [C.POINTER(C.c_int)]),
     "hq_tile_build_t_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp]),
     "hq_tile_ormqr_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp, C.c_int, _vp]),
@@ -73,6 +75,8 @@ SIGNATURES = {
     "hq_tile_make_q_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp]),
     "hq_dag_host": (C.c_int, [C.c_int] * 6 + [C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int,
                                              C.POINTER(C.c_int)]),
+    "hq_dag_host_prec": (C.c_int, [C.c_int] * 7 + [C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int,
+                                                  C.POINTER(C.c_int)]),
     "hq_batched_tile_qr_dev": (C.c_int, [C.c_int] * 3 + [C.POINTER(C.c_int)] * 2 + [C.POINTER(_vp)] * 3 + [_vp]),
     "hq_batched_plan_dev": (C.c_int, [C.c_int] * 3 + [C.POINTER(C.c_int)] * 2 + [C.POINTER(_vp)] * 3
                             + [C.POINTER(C.c_int)]),
@@ -282,9 +286,10 @@ def tile_to_cm_dev(nb, m, n, A, Cs, ldc, stream=None):
                                   _stream(stream)))
 
 
-def schedule_info(MT, NT, nb, ib):
+def schedule_info(MT, NT, nb, ib, prec=0):
+    """(tasks, workgroups) of the factorisation DAG with updates in precision prec."""
     n, g = C.c_int(0), C.c_int(0)
-    check(lib().hq_schedule_info(MT, NT, nb, ib, C.byref(n), C.byref(g)))
+    check(lib().hq_schedule_info_prec(MT, NT, nb, ib, prec, C.byref(n), C.byref(g)))
     return n.value, g.value
 
 
@@ -329,14 +334,15 @@ def tile_build_t_dev(MT, NT, nb, ib, RY, TAU, T, stream=None):
     check(lib().hq_tile_build_t_dev(MT, NT, nb, ib, _ptr(RY), _ptr(TAU), _ptr(T), _stream(stream)))
 
 
-def dag_host(MT, NT, nb, ib, XNT=0, P=256):
-    """Task DAG in claim order (no GPU): (tasks[n,5], deps[n,6] with -1 padding)."""
+def dag_host(MT, NT, nb, ib, XNT=0, P=256, prec=0):
+    """Task DAG in claim order (no GPU): (tasks[n,5], deps[n,6] with -1 padding);
+    prec: precision of the factorisation's updates (their slab width)."""
     L = lib()
     n = C.c_int(0)
-    check(L.hq_dag_host(MT, NT, nb, ib, XNT, P, None, None, 0, C.byref(n)))
+    check(L.hq_dag_host_prec(MT, NT, nb, ib, prec, XNT, P, None, None, 0, C.byref(n)))
     tasks = np.zeros(5 * n.value, dtype=np.int32)
     deps = np.zeros(6 * n.value, dtype=np.int32)
     ip = C.POINTER(C.c_int)
-    check(L.hq_dag_host(MT, NT, nb, ib, XNT, P, tasks.ctypes.data_as(ip), deps.ctypes.data_as(ip),
-                        n.value, C.byref(n)))
+    check(L.hq_dag_host_prec(MT, NT, nb, ib, prec, XNT, P, tasks.ctypes.data_as(ip), deps.ctypes.data_as(ip),
+                             n.value, C.byref(n)))
     return tasks.reshape(-1, 5), deps.reshape(-1, 6)

@@ -1652,10 +1652,10 @@ __device__ __forceinline__ void stage32_store(const Stage32<NB> &R, float *Vb, f
   TTb[(threadIdx.x >> 4) * LDT16 + (threadIdx.x & 15)] = (float)(R.t * tt_mask(sb));
 }
 
-template <int NB, bool DO_U, bool DO_A, bool DO_LD>
-__device__ __forceinline__ void fpass32(f4 (&acc)[NB / 16], const f4 &W2, f4 (&Wn)[2], float (&o)[2][8],
-                                        float (&tto)[4], const float *Vu, const float *Vw, const float *TTw,
-                                        Stage32<NB> &R, const double *Vg, const SubBlk &sd) {
+template <int NB, int NS, bool DO_U, bool DO_A, bool DO_LD>
+__device__ __forceinline__ void fpass32(f4 (&acc)[NS][NB / 16], const f4 (&W2)[NS], f4 (&Wn)[NS][2],
+                                        float (&o)[2][8], float (&tto)[4], const float *Vu, const float *Vw,
+                                        const float *TTw, Stage32<NB> &R, const double *Vg, const SubBlk &sd) {
   constexpr int NG = NB / 32;
   constexpr int NP = (DO_U ? NG : 0) + (DO_A ? NG : 0);
   static_assert(!DO_LD || NP >= Stage32<NB>::NLD, "one staging load per phase");
@@ -1686,41 +1686,59 @@ __device__ __forceinline__ void fpass32(f4 (&acc)[NB / 16], const f4 &W2, f4 (&W
       for (int sp = 0; sp < 4; sp++) tto[sp] = TTw[m * LDT16 + 4 * g + sp];
     }
     const int G = grp(p);
+    // the operands of a phase serve every strip of the wave: 2 NS
+    // independent accumulator chains per phase
     if (kind(p)) {
 #pragma unroll
       for (int sp = 0; sp < 4; sp++)
 #pragma unroll
-        for (int i = 0; i < 2; i++) acc[2 * G + i] = mfma32(o[p & 1][2 * sp + i], W2[sp], acc[2 * G + i]);
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+          for (int q = 0; q < NS; q++) acc[q][2 * G + i] = mfma32(o[p & 1][2 * sp + i], W2[q][sp], acc[q][2 * G + i]);
     } else {
 #pragma unroll
       for (int i = 0; i < 2; i++)
 #pragma unroll
-        for (int sp = 0; sp < 4; sp++) Wn[sp & 1] = mfma32(o[p & 1][4 * i + sp], acc[2 * G + i][sp], Wn[sp & 1]);
+        for (int sp = 0; sp < 4; sp++)
+#pragma unroll
+          for (int q = 0; q < NS; q++)
+            Wn[q][sp & 1] = mfma32(o[p & 1][4 * i + sp], acc[q][2 * G + i][sp], Wn[q][sp & 1]);
     }
     __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
     __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
     __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
     __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
-    __builtin_amdgcn_sched_group_barrier(0x008, 6, 0);
+    __builtin_amdgcn_sched_group_barrier(0x008, 8 * NS - 2, 0);
     __builtin_amdgcn_sched_barrier(0);
   }
 }
 
-template <int NB, bool UNIT>
+// LARFB (UNIT) / SSRFB of one slab of 64 NS columns: wave w holds the
+// 16-column strips w + 4q (q < NS) of the slab.
+template <int NB, bool UNIT, int NS>
 __device__ __forceinline__ void task_update_f32(Smem<NB> &S, const double *V, const double *T, double *X,
                                                 double *A1, int ib, int slab) {
-  static_assert(NB % HQ_SLAB == 0, "slabs cover the tile exactly");
+  constexpr int UW = 64 * NS;
+  static_assert(NB % UW == 0, "slabs cover the tile exactly");
   const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, g = lane >> 4, wave = tid >> 6;
-  const int col = slab * HQ_SLAB + 16 * wave + m;
-  double *xcol = X + (size_t)col * NB;
-  double *a1col = UNIT ? nullptr : A1 + (size_t)col * NB;
+  // strip q: column col0 + 64 q
+  const int col0 = slab * UW + 16 * wave + m;
+  double *xcol = X + (size_t)col0 * NB;
+  double *a1col = UNIT ? nullptr : A1 + (size_t)col0 * NB;
+  constexpr size_t QS = (size_t)64 * NB;  // strip stride in doubles
   const int nsub = sub_count<NB>(ib);
   SubBlk w0 = sub_blk<NB>(0, ib), w1 = w0, w2 = next_sub<NB>(w0, ib), w3 = next_sub<NB>(w2, ib);
-  f4 acc[NB / 16];
-  f4 W2 = f4{0.f, 0.f, 0.f, 0.f}, Wn[2], a1v = W2, a1o = W2;
+  f4 acc[NS][NB / 16];
+  f4 W2[NS], Wn[NS][2], a1v[NS], a1o[NS];
+#pragma unroll
+  for (int q = 0; q < NS; q++) {
+    W2[q] = f4{0.f, 0.f, 0.f, 0.f};
+    a1v[q] = W2[q];
+    a1o[q] = W2[q];
+  }
   float o[2][8], tto[4];
   Stage32<NB> R;
-  // prologue: sub-blocks 0 and 1 into ring slots 0, 1; the strip in FP32
+  // prologue: sub-blocks 0 and 1 into ring slots 0, 1; the strips in FP32
   {
 #pragma unroll
     for (int j = 0; j < 2; j++) {
@@ -1731,38 +1749,61 @@ __device__ __forceinline__ void task_update_f32(Smem<NB> &S, const double *V, co
       stage32_store<NB, UNIT>(R, f32vbuf(S, j), f32tbuf(S, j), sj);
     }
 #pragma unroll
-    for (int rb = 0; rb < NB / 16; rb++) {
-      const d4 x = *reinterpret_cast<const d4 *>(xcol + 16 * rb + 4 * g);
-      acc[rb] = f4{(float)x[0], (float)x[1], (float)x[2], (float)x[3]};
-    }
+    for (int q = 0; q < NS; q++)
+#pragma unroll
+      for (int rb = 0; rb < NB / 16; rb++) {
+        const d4 x = *reinterpret_cast<const d4 *>(xcol + q * QS + 16 * rb + 4 * g);
+        acc[q][rb] = f4{(float)x[0], (float)x[1], (float)x[2], (float)x[3]};
+      }
     __syncthreads();
     ld_a32<NB>(o[0], f32vbuf(S, 0), 0);
   }
+  // A1 rows of the accumulate sub-block, loaded before a pass as doubles and
+  // converted after it: a conversion next to the load would make the wave
+  // wait for each load in turn at the start of the pass
+  d4 a1raw[NS];
   auto pre = [&](int s) {
     R.t = tt_raw(T, ib, w2);
     if (!UNIT) {
 #pragma unroll
-      for (int v = 0; v < 4; v++) a1v[v] = (float)a1col[min(w1.i0 + 4 * g + v, NB - 1)];
+      for (int q = 0; q < NS; q++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) a1raw[q][v] = a1col[q * QS + min(w1.i0 + 4 * g + v, NB - 1)];
     }
-    Wn[0] = f4{0.f, 0.f, 0.f, 0.f};
-    Wn[1] = Wn[0];
+#pragma unroll
+    for (int q = 0; q < NS; q++) {
+      Wn[q][0] = f4{0.f, 0.f, 0.f, 0.f};
+      Wn[q][1] = Wn[q][0];
+    }
   };
   // after pass s: W2_{s+1}; sub-block s+2 into its ring slot; A1 rows of
   // s+1 stored after the barrier
   auto post = [&](int s) {
-    const f4 W = UNIT ? Wn[0] + Wn[1] : (Wn[0] + Wn[1]) + a1v;
-    f4 W2n = f4{0.f, 0.f, 0.f, 0.f};
+    f4 W[NS], W2n[NS];
 #pragma unroll
-    for (int sp = 0; sp < 4; sp++) W2n = mfma32(tto[sp], W[sp], W2n);
-    a1o = a1v - W2n;
-    W2 = -W2n;
+    for (int q = 0; q < NS; q++) {
+      if (!UNIT) a1v[q] = f4{(float)a1raw[q][0], (float)a1raw[q][1], (float)a1raw[q][2], (float)a1raw[q][3]};
+      W[q] = UNIT ? Wn[q][0] + Wn[q][1] : (Wn[q][0] + Wn[q][1]) + a1v[q];
+      W2n[q] = f4{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int sp = 0; sp < 4; sp++)
+#pragma unroll
+      for (int q = 0; q < NS; q++) W2n[q] = mfma32(tto[sp], W[q][sp], W2n[q]);
+#pragma unroll
+    for (int q = 0; q < NS; q++) {
+      a1o[q] = a1v[q] - W2n[q];
+      W2[q] = -W2n[q];
+    }
     ld_u32<NB>(o[0], f32vbuf(S, (s + 1) & 3), 0);
     stage32_store<NB, UNIT>(R, f32vbuf(S, (s + 2) & 3), f32tbuf(S, (s + 2) & 3), w2);
     __syncthreads();
     if (!UNIT) {
 #pragma unroll
-      for (int v = 0; v < 4; v++)
-        if (4 * g + v < w1.kb) a1col[w1.i0 + 4 * g + v] = (double)a1o[v];
+      for (int q = 0; q < NS; q++)
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+          if (4 * g + v < w1.kb) a1col[q * QS + w1.i0 + 4 * g + v] = (double)a1o[q][v];
     }
     w0 = w1;
     w1 = w2;
@@ -1770,21 +1811,24 @@ __device__ __forceinline__ void task_update_f32(Smem<NB> &S, const double *V, co
     w3 = next_sub<NB>(w3, ib);
   };
   pre(-1);
-  fpass32<NB, false, true, true>(acc, W2, Wn, o, tto, f32vbuf(S, 3), f32vbuf(S, 0), f32tbuf(S, 0), R, V, w2);
+  fpass32<NB, NS, false, true, true>(acc, W2, Wn, o, tto, f32vbuf(S, 3), f32vbuf(S, 0), f32tbuf(S, 0), R, V, w2);
   post(-1);
   for (int s = 0; s + 1 < nsub; s++) {
     pre(s);
-    fpass32<NB, true, true, true>(acc, W2, Wn, o, tto, f32vbuf(S, s & 3), f32vbuf(S, (s + 1) & 3),
-                                  f32tbuf(S, (s + 1) & 3), R, V, w2);
+    fpass32<NB, NS, true, true, true>(acc, W2, Wn, o, tto, f32vbuf(S, s & 3), f32vbuf(S, (s + 1) & 3),
+                                      f32tbuf(S, (s + 1) & 3), R, V, w2);
     post(s);
   }
-  fpass32<NB, true, false, false>(acc, W2, Wn, o, tto, f32vbuf(S, (nsub - 1) & 3), f32vbuf(S, 0),
-                                  f32tbuf(S, 0), R, V, w2);
+  fpass32<NB, NS, true, false, false>(acc, W2, Wn, o, tto, f32vbuf(S, (nsub - 1) & 3), f32vbuf(S, 0),
+                                      f32tbuf(S, 0), R, V, w2);
 #pragma unroll
-  for (int rb = 0; rb < NB / 16; rb++) {
-    const f4 x = acc[rb];
-    *reinterpret_cast<d4 *>(xcol + 16 * rb + 4 * g) = d4{(double)x[0], (double)x[1], (double)x[2], (double)x[3]};
-  }
+  for (int q = 0; q < NS; q++)
+#pragma unroll
+    for (int rb = 0; rb < NB / 16; rb++) {
+      const f4 x = acc[q][rb];
+      *reinterpret_cast<d4 *>(xcol + q * QS + 16 * rb + 4 * g) =
+          d4{(double)x[0], (double)x[1], (double)x[2], (double)x[3]};
+    }
 }
 
 // --------------------------------------------------------------------------
@@ -1941,10 +1985,10 @@ __device__ __forceinline__ void ldh_a(HOpA &o, const _Float16 *Vb, int G) {
 // accumulate phases W += V_w^T X interleaved as U0 U1 A0 U2 A1 ... A(NG-1)
 // (or only U / only A); the copies of the next sub-block image, one per
 // phase.
-template <int NB, bool DO_U, bool DO_A, bool DO_DMA>
-__device__ __forceinline__ void fpassh(f4 (&acc)[NB / 16], const h4 &W2h, const h4 &W2l, f4 (&Wn)[2], float (&tto)[4],
-                                       const _Float16 *Vu, const _Float16 *Vw, const float *TTw, _Float16 *Vd,
-                                       const _Float16 *img) {
+template <int NB, int NS, bool DO_U, bool DO_A, bool DO_DMA>
+__device__ __forceinline__ void fpassh(f4 (&acc)[NS][NB / 16], const h4 (&W2h)[NS], const h4 (&W2l)[NS],
+                                       f4 (&Wn)[NS][2], float (&tto)[4], const _Float16 *Vu, const _Float16 *Vw,
+                                       const float *TTw, _Float16 *Vd, const _Float16 *img) {
   constexpr int NG = NB / 32;
   constexpr int NP = (DO_U ? NG : 0) + (DO_A ? NG : 0);
   static_assert(!DO_DMA || NP >= HSlot<NB>::NOPS, "one image copy per phase");
@@ -1983,60 +2027,81 @@ __device__ __forceinline__ void fpassh(f4 (&acc)[NB / 16], const h4 &W2h, const 
       for (int sp = 0; sp < 4; sp++) tto[sp] = TTw[m * LDT16 + 4 * g + sp];
     }
     const int G = grp(p);
+    // the operands of a phase serve every strip of the wave; each product
+    // term runs over all 2 NS accumulators before the next (independent
+    // chains back to back)
     if (kind(p)) {
       const HOpU &o = ou[p & 1];
 #pragma unroll
-      for (int t = 0; t < 2; t++) {
-        f4 a = acc[2 * G + t];
-        a = mfma_h16(o.h[t], W2h, a);
-        a = mfma_h16(o.h[t], W2l, a);
-        a = mfma_h16(o.l[t], W2h, a);
-        acc[2 * G + t] = a;
-      }
+      for (int t = 0; t < 2; t++)
+#pragma unroll
+        for (int q = 0; q < NS; q++) acc[q][2 * G + t] = mfma_h16(o.h[t], W2h[q], acc[q][2 * G + t]);
+#pragma unroll
+      for (int t = 0; t < 2; t++)
+#pragma unroll
+        for (int q = 0; q < NS; q++) acc[q][2 * G + t] = mfma_h16(o.h[t], W2l[q], acc[q][2 * G + t]);
+#pragma unroll
+      for (int t = 0; t < 2; t++)
+#pragma unroll
+        for (int q = 0; q < NS; q++) acc[q][2 * G + t] = mfma_h16(o.l[t], W2h[q], acc[q][2 * G + t]);
     } else {
       const HOpA &o = oa[p & 1];
-      const f4 x0 = acc[2 * G], x1 = acc[2 * G + 1];
-      h8 xh, xl;
+      // strip by strip: the split of strip q+1 overlaps strip q's MFMAs
 #pragma unroll
-      for (int e = 0; e < 4; e++) {
-        _Float16 h, l;
-        hsplit(x0[e], h, l);
-        xh[e] = h;
-        xl[e] = l;
-        hsplit(x1[e], h, l);
-        xh[4 + e] = h;
-        xl[4 + e] = l;
+      for (int q = 0; q < NS; q++) {
+        const f4 x0 = acc[q][2 * G], x1 = acc[q][2 * G + 1];
+        h8 xh, xl;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          _Float16 h, l;
+          hsplit(x0[e], h, l);
+          xh[e] = h;
+          xl[e] = l;
+          hsplit(x1[e], h, l);
+          xh[4 + e] = h;
+          xl[4 + e] = l;
+        }
+        Wn[q][0] = mfma_h32(o.h, xh, Wn[q][0]);
+        Wn[q][1] = mfma_h32(o.h, xl, Wn[q][1]);
+        Wn[q][0] = mfma_h32(o.l, xh, Wn[q][0]);
       }
-      Wn[0] = mfma_h32(o.h, xh, Wn[0]);
-      Wn[1] = mfma_h32(o.h, xl, Wn[1]);
-      Wn[1] = mfma_h32(o.l, xh, Wn[1]);
     }
     __builtin_amdgcn_sched_barrier(0);
   }
 }
 
 // LARFB (UNIT: the images of the diagonal tile's unit lower V) / SSRFB on
-// one 64-column slab in split precision.  VHt: the reflector tile's images.
-template <int NB, bool UNIT>
+// one slab of 64 NS columns in split precision (wave w holds the 16-column
+// strips w + 4q, q < NS).  VHt: the reflector tile's images.
+template <int NB, bool UNIT, int NS>
 __device__ __forceinline__ void task_update_h(Smem<NB> &S, const unsigned short *VHt, const double *T, double *X,
                                               double *A1, int ib, int slab) {
-  static_assert(NB % HQ_SLAB == 0, "slabs cover the tile exactly");
+  constexpr int UW = 64 * NS;
+  static_assert(NB % UW == 0, "slabs cover the tile exactly");
   const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, g = lane >> 4, wave = tid >> 6;
-  const int col = slab * HQ_SLAB + 16 * wave + m;
-  double *xcol = X + (size_t)col * NB;
-  double *a1col = UNIT ? nullptr : A1 + (size_t)col * NB;
+  // strip q: column col0 + 64 q
+  const int col0 = slab * UW + 16 * wave + m;
+  double *xcol = X + (size_t)col0 * NB;
+  double *a1col = UNIT ? nullptr : A1 + (size_t)col0 * NB;
+  constexpr size_t QS = (size_t)64 * NB;  // strip stride in doubles
   const int nsub = sub_count<NB>(ib);
   const _Float16 *img0 = reinterpret_cast<const _Float16 *>(VHt);
   auto sbk = [&](int s) __attribute__((always_inline)) { return sub_blk<NB>(min(s, nsub - 1), ib); };
   auto img = [&](int s) __attribute__((always_inline)) { return img0 + (size_t)min(s, nsub - 1) * 64 * NB; };
-  f4 acc[NB / 16];
-  f4 Wn[2], a1v = f4{0.f, 0.f, 0.f, 0.f}, W2n = a1v;
-  h4 W2h = h4{0, 0, 0, 0}, W2l = W2h;
+  f4 acc[NS][NB / 16];
+  f4 Wn[NS][2], a1v[NS];  // a1v: the A1 rows read before a pass, their new values after it
+  h4 W2h[NS], W2l[NS];
+#pragma unroll
+  for (int q = 0; q < NS; q++) {
+    a1v[q] = f4{0.f, 0.f, 0.f, 0.f};
+    W2h[q] = h4{0, 0, 0, 0};
+    W2l[q] = W2h[q];
+  }
   float tto[4];
   double treg = 0.0;
-  float xs, xu;  // the column's scale 2^e and 2^-e
-  // prologue: the images of sub-blocks 0 and 1, T^T of both, the strip
-  // (two sweeps over the column, the second from cache: its largest entry
+  float xs[NS], xu[NS];  // each strip column's scale 2^e and 2^-e
+  // prologue: the images of sub-blocks 0 and 1, T^T of both, the strips
+  // (two sweeps over each column, the second from cache: its largest entry
   // over [A1; A2], then the scaled FP32 copy)
   {
 #pragma unroll
@@ -2045,101 +2110,129 @@ __device__ __forceinline__ void task_update_h(Smem<NB> &S, const unsigned short 
       for (int q = 0; q < HSlot<NB>::NOPS; q++) dmah_op<NB>(q, hvbuf(S, j), img(j));
       f32tbuf(S, j)[(tid >> 4) * LDT16 + (tid & 15)] = (float)(tt_raw(T, ib, sbk(j)) * tt_mask(sbk(j)));
     }
-    double mx = 0.0;
-#pragma unroll 4
-    for (int rb = 0; rb < NB / 16; rb++) {
-      const d4 x = *reinterpret_cast<const d4 *>(xcol + 16 * rb + 4 * g);
 #pragma unroll
-      for (int v = 0; v < 4; v++) mx = fmax(mx, fabs(x[v]));
-    }
-    if (!UNIT) {
+    for (int q = 0; q < NS; q++) {
+      double mx = 0.0;
 #pragma unroll 4
       for (int rb = 0; rb < NB / 16; rb++) {
-        const d4 a = *reinterpret_cast<const d4 *>(a1col + 16 * rb + 4 * g);
+        const d4 x = *reinterpret_cast<const d4 *>(xcol + q * QS + 16 * rb + 4 * g);
 #pragma unroll
-        for (int v = 0; v < 4; v++) mx = fmax(mx, fabs(a[v]));
+        for (int v = 0; v < 4; v++) mx = fmax(mx, fabs(x[v]));
       }
-    }
-    mx = fmax(mx, __shfl_xor(mx, 16, 64));
-    mx = fmax(mx, __shfl_xor(mx, 32, 64));
-    // the column's largest entry is < 2^e, so its 2-norm over the 2 NB rows
-    // of [A1; A2] (NB <= 256) is < 2^(e+5); the scaled norm is < 2^12
-    int e = 0;
-    if (mx > 0.0 && mx <= 1.7e308) (void)frexp(mx, &e);
-    const int sh = mx > 0.0 ? 7 - e : 0;
-    const double dsc = ldexp(1.0, sh);
-    xs = (float)dsc;
-    xu = (float)ldexp(1.0, -sh);
+      if (!UNIT) {
+#pragma unroll 4
+        for (int rb = 0; rb < NB / 16; rb++) {
+          const d4 a = *reinterpret_cast<const d4 *>(a1col + q * QS + 16 * rb + 4 * g);
 #pragma unroll
-    for (int rb = 0; rb < NB / 16; rb++) {
-      const d4 x = *reinterpret_cast<const d4 *>(xcol + 16 * rb + 4 * g);
-      acc[rb] = f4{(float)(x[0] * dsc), (float)(x[1] * dsc), (float)(x[2] * dsc), (float)(x[3] * dsc)};
+          for (int v = 0; v < 4; v++) mx = fmax(mx, fabs(a[v]));
+        }
+      }
+      mx = fmax(mx, __shfl_xor(mx, 16, 64));
+      mx = fmax(mx, __shfl_xor(mx, 32, 64));
+      // the column's largest entry is < 2^e, so its 2-norm over the 2 NB rows
+      // of [A1; A2] (NB <= 256) is < 2^(e+5); the scaled norm is < 2^12
+      int e = 0;
+      if (mx > 0.0 && mx <= 1.7e308) (void)frexp(mx, &e);
+      const int sh = mx > 0.0 ? 7 - e : 0;
+      const double dsc = ldexp(1.0, sh);
+      xs[q] = (float)dsc;
+      xu[q] = (float)ldexp(1.0, -sh);
+#pragma unroll
+      for (int rb = 0; rb < NB / 16; rb++) {
+        const d4 x = *reinterpret_cast<const d4 *>(xcol + q * QS + 16 * rb + 4 * g);
+        acc[q][rb] = f4{(float)(x[0] * dsc), (float)(x[1] * dsc), (float)(x[2] * dsc), (float)(x[3] * dsc)};
+      }
     }
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
   }
-  const double dxu = (double)xu;
-  // before pass s: A1 rows of the accumulate sub-block s+1, T of s+2
+  // before pass s: A1 rows of the accumulate sub-block s+1 (as doubles,
+  // converted after the pass: a conversion next to the load would make the
+  // wave wait for each load in turn), T of s+2
+  d4 a1raw[NS];
   auto pre = [&](int s) __attribute__((always_inline)) {
     if (!UNIT) {
       // rows past a ragged sub-block load a clamped neighbour: their W rows
       // meet only zero rows and columns of T and are never stored
       const SubBlk w1 = sbk(s + 1);
 #pragma unroll
-      for (int v = 0; v < 4; v++) a1v[v] = (float)a1col[min(w1.i0 + 4 * g + v, NB - 1)] * xs;
+      for (int q = 0; q < NS; q++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) a1raw[q][v] = a1col[q * QS + min(w1.i0 + 4 * g + v, NB - 1)];
     }
     treg = tt_raw(T, ib, sbk(s + 2));
-    Wn[0] = f4{0.f, 0.f, 0.f, 0.f};
-    Wn[1] = Wn[0];
+#pragma unroll
+    for (int q = 0; q < NS; q++) {
+      Wn[q][0] = f4{0.f, 0.f, 0.f, 0.f};
+      Wn[q][1] = Wn[q][0];
+    }
   };
   // after pass s: W2 of sub-block s+1 (FP32 T chain), its FP16 split for the
   // next update phases, T^T of s+2 into the ring, A1 rows of s+1 after the
   // barrier (every copy of the pass has landed before it)
   auto post = [&](int s) __attribute__((always_inline)) {
-    f4 W = (Wn[0] + Wn[1]) * HV_UNSCALE;
-    if (!UNIT) W += a1v;
-    f4 t = f4{0.f, 0.f, 0.f, 0.f};
+    f4 W[NS], t[NS];
 #pragma unroll
-    for (int sp = 0; sp < 4; sp++) t = mfma32(tto[sp], W[sp], t);
-    W2n = t;
+    for (int q = 0; q < NS; q++) {
+      W[q] = (Wn[q][0] + Wn[q][1]) * HV_UNSCALE;
+      if (!UNIT) {
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-      _Float16 h, l;
-      hsplit(-t[v] * HV_UNSCALE, h, l);
-      W2h[v] = h;
-      W2l[v] = l;
+        for (int v = 0; v < 4; v++) a1v[q][v] = (float)a1raw[q][v] * xs[q];
+        W[q] += a1v[q];
+      }
+      t[q] = f4{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int sp = 0; sp < 4; sp++)
+#pragma unroll
+      for (int q = 0; q < NS; q++) t[q] = mfma32(tto[sp], W[q][sp], t[q]);
+#pragma unroll
+    for (int q = 0; q < NS; q++) {
+      if (!UNIT) a1v[q] -= t[q];
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        _Float16 h, l;
+        hsplit(-t[q][v] * HV_UNSCALE, h, l);
+        W2h[q][v] = h;
+        W2l[q][v] = l;
+      }
     }
     f32tbuf(S, (s + 2) & 3)[(tid >> 4) * LDT16 + (tid & 15)] = (float)(treg * tt_mask(sbk(s + 2)));
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (!UNIT) {
       const SubBlk w1 = sbk(s + 1);
-      const f4 a1o = a1v - W2n;
 #pragma unroll
-      for (int v = 0; v < 4; v++)
-        if (4 * g + v < w1.kb) a1col[w1.i0 + 4 * g + v] = (double)a1o[v] * dxu;
+      for (int q = 0; q < NS; q++)
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+          if (4 * g + v < w1.kb) a1col[q * QS + w1.i0 + 4 * g + v] = (double)a1v[q][v] * (double)xu[q];
     }
   };
   // pass -1: W of sub-block 0; copies sub-block 2 into slot 2
   pre(-1);
-  fpassh<NB, false, true, true>(acc, W2h, W2l, Wn, tto, hvbuf(S, 3), hvbuf(S, 0), f32tbuf(S, 0), hvbuf(S, 2), img(2));
+  fpassh<NB, NS, false, true, true>(acc, W2h, W2l, Wn, tto, hvbuf(S, 3), hvbuf(S, 0), f32tbuf(S, 0), hvbuf(S, 2),
+                                    img(2));
   post(-1);
   // pass s: update with s (slot s&3), accumulate with s+1, copy s+3 into
   // slot (s+3)&3 (read last by pass s-1)
   for (int s = 0; s + 1 < nsub; s++) {
     pre(s);
-    fpassh<NB, true, true, true>(acc, W2h, W2l, Wn, tto, hvbuf(S, s & 3), hvbuf(S, (s + 1) & 3),
-                                 f32tbuf(S, (s + 1) & 3), hvbuf(S, (s + 3) & 3), img(s + 3));
+    fpassh<NB, NS, true, true, true>(acc, W2h, W2l, Wn, tto, hvbuf(S, s & 3), hvbuf(S, (s + 1) & 3),
+                                     f32tbuf(S, (s + 1) & 3), hvbuf(S, (s + 3) & 3), img(s + 3));
     post(s);
   }
-  fpassh<NB, true, false, false>(acc, W2h, W2l, Wn, tto, hvbuf(S, (nsub - 1) & 3), hvbuf(S, 0), f32tbuf(S, 0),
-                                 hvbuf(S, 0), img(0));
+  fpassh<NB, NS, true, false, false>(acc, W2h, W2l, Wn, tto, hvbuf(S, (nsub - 1) & 3), hvbuf(S, 0), f32tbuf(S, 0),
+                                     hvbuf(S, 0), img(0));
 #pragma unroll
-  for (int rb = 0; rb < NB / 16; rb++) {
-    const f4 x = acc[rb];
-    *reinterpret_cast<d4 *>(xcol + 16 * rb + 4 * g) =
-        d4{(double)x[0] * dxu, (double)x[1] * dxu, (double)x[2] * dxu, (double)x[3] * dxu};
-  }
+  for (int q = 0; q < NS; q++)
+#pragma unroll
+    for (int rb = 0; rb < NB / 16; rb++) {
+      const f4 x = acc[q][rb];
+      const double u = (double)xu[q];
+      *reinterpret_cast<d4 *>(xcol + q * QS + 16 * rb + 4 * g) =
+          d4{(double)x[0] * u, (double)x[1] * u, (double)x[2] * u, (double)x[3] * u};
+    }
   // no copy may still be in flight when the next task reuses LDS
   __builtin_amdgcn_s_waitcnt(0);
 }
@@ -2152,9 +2245,9 @@ __device__ __forceinline__ int2 task_larfb(Smem<NB> &S, const double *V, const d
                                            const double *Vd = nullptr, Ahead ah = Ahead{},
                                            const unsigned short *VHt = nullptr) {
   if constexpr (NB >= 128 && PREC == 2) {
-    task_update_h<NB, true>(S, VHt, T, C, nullptr, ib, slab);
+    task_update_h<NB, true, hq_upd_slab(NB, 2) / 64>(S, VHt, T, C, nullptr, ib, slab);
   } else if constexpr (NB >= 128 && PREC == 1) {
-    task_update_f32<NB, true>(S, V, T, C, nullptr, ib, slab);
+    task_update_f32<NB, true, hq_upd_slab(NB, 1) / 64>(S, V, T, C, nullptr, ib, slab);
   } else if constexpr (NB >= 128) {
     if (Wt)
       task_larfb_w<NB>(S, Vd, Wt, C, slab);
@@ -2171,9 +2264,9 @@ __device__ __forceinline__ int2 task_ssrfb(Smem<NB> &S, double *A1, double *A2, 
                                            const double *T, int ib, int slab, Ahead ah = Ahead{},
                                            const unsigned short *VHt = nullptr) {
   if constexpr (NB >= 128 && PREC == 2) {
-    task_update_h<NB, false>(S, VHt, T, A2, A1, ib, slab);
+    task_update_h<NB, false, hq_upd_slab(NB, 2) / 64>(S, VHt, T, A2, A1, ib, slab);
   } else if constexpr (NB >= 128 && PREC == 1) {
-    task_update_f32<NB, false>(S, V, T, A2, A1, ib, slab);
+    task_update_f32<NB, false, hq_upd_slab(NB, 1) / 64>(S, V, T, A2, A1, ib, slab);
   } else if constexpr (NB >= 128) {
     return task_update_f<NB, false>(S, V, T, A2, A1, ib, slab, ah);
   } else {
@@ -2663,7 +2756,10 @@ hipError_t hq_launch_sched(const hq_geom &G, const hq_geom *geoms, const hq_task
     HQ_NBWB
     if constexpr (NB >= 128) {
       if (G.prec == 2) {
-        hipLaunchKernelGGL((hq::sched_kernel<NB, WB, 2>), dim3(grid), dim3(HQ_THREADS), 0, st, G, geoms, tasks,
+        // one instantiation (its panels take any ib <= 32) for the split
+        // updates: the ib <= 16 one at nb = 256 crashes this compiler's
+        // AGPR-copy rewrite pass
+        hipLaunchKernelGGL((hq::sched_kernel<NB, 2, 2>), dim3(grid), dim3(HQ_THREADS), 0, st, G, geoms, tasks,
                            ntasks, flags, epoch, counter, trace, (const int *)epoch_dev, wait_limit);
         return;
       }

@@ -29,7 +29,8 @@ struct hq_task {
   int32_t k;      // elimination step (panel column)
   int32_t r;      // tile row (TSQRT / SSRFB)
   int32_t j;      // tile column (LARFB / SSRFB)
-  int32_t slab;   // 64-column slab of tile column j (LARFB / SSRFB)
+  int32_t slab;   // column slab: hq_upd_slab(nb, prec) columns of tile column j (LARFB / SSRFB),
+                  // 64 columns of the panel tile (GEQRT_U / TSQRT_U)
   int32_t mat;    // matrix of a batch (index into the geometry table; 0 otherwise)
   int32_t ndep;
   int32_t dep[HQ_MAXDEP];
@@ -80,6 +81,17 @@ struct hq_bmat {
 
 #define HQ_THREADS 256
 #define HQ_SLAB 64
+
+// Column width of one LARFB / SSRFB task.  FP64 and split updates: 64 (one
+// 16-column strip per wave).  FP32 updates: 128 (two strips per wave), so
+// the MFMA work of a pass -- half that of FP64 per column -- still covers the
+// latency of the sub-block loads issued for the next pass, and each V tile is
+// read once per 128 columns.  (Two FP32 strips per wave plus the split
+// pipeline's operands exceed the register file at nb = 256.)  The panels'
+// trailing tasks keep 64-column slabs in every precision.
+__host__ __device__ inline constexpr int hq_upd_slab(int nb, int prec) {
+  return (prec == 1 ? 128 : HQ_SLAB) < nb ? (prec == 1 ? 128 : HQ_SLAB) : nb;
+}
 #define HQ_IBMAX 32
 // bytes of the device array behind the allocation-free tile QR (claim counter,
 // flags and task list of one geometry)

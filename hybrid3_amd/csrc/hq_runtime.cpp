@@ -125,24 +125,33 @@ struct DagBuilder {
   bool apply;
   int XNT;
   int NBLK;  // ib blocks per tile
+  int prec;  // precision of the LARFB / SSRFB updates (their slab width and cost)
+  int UW, SU;  // LARFB / SSRFB slab width (hq_upd_slab) and slabs per tile
   std::vector<hq_task> t;
   std::vector<int> idl, ids;
   // panel tasks of the current step k: (row r, block b); their trailing
   // tasks: (row r, block b, slab s).  Row k is the GEQRT of the step.
   std::vector<int> idp, idu;
   bool split;  // GEQRT / TSQRT as panel + trailing tasks (else one task per tile)
-  DagBuilder(int MT_, int NT_, int nb_, int ib_, bool apply_ = false, int XNT_ = 0, bool split_ = true)
-      : MT(MT_), NT(NT_), nb(nb_), ib(ib_), apply(apply_), XNT(apply_ ? XNT_ : NT_), split(split_) {
+  DagBuilder(int MT_, int NT_, int nb_, int ib_, bool apply_ = false, int XNT_ = 0, bool split_ = true,
+             int prec_ = 0)
+      : MT(MT_), NT(NT_), nb(nb_), ib(ib_), apply(apply_), XNT(apply_ ? XNT_ : NT_), prec(apply_ ? 0 : prec_),
+        split(split_) {
     S = (nb + HQ_SLAB - 1) / HQ_SLAB;
+    UW = hq_upd_slab(nb, prec);
+    SU = (nb + UW - 1) / UW;
     KT = std::min(MT, NT);
     NBLK = (nb + ib - 1) / ib;
-    idl.assign((size_t)KT * XNT * S, -1);
-    ids.assign((size_t)KT * XNT * MT * S, -1);
+    idl.assign((size_t)KT * XNT * SU, -1);
+    ids.assign((size_t)KT * XNT * MT * SU, -1);
     idp.assign((size_t)MT * NBLK, -1);
     idu.assign((size_t)MT * NBLK * S, -1);
   }
-  int &L(int k, int j, int s) { return idl[((size_t)k * XNT + j) * S + s]; }
-  int &Sr(int k, int j, int r, int s) { return ids[(((size_t)k * XNT + j) * MT + r) * S + s]; }
+  // L / Sr: update tasks by update slab u (UW columns); Su: the update task
+  // covering the 64-column slab s of the panel tasks
+  int &L(int k, int j, int u) { return idl[((size_t)k * XNT + j) * SU + u]; }
+  int &Sr(int k, int j, int r, int u) { return ids[(((size_t)k * XNT + j) * MT + r) * SU + u]; }
+  int Su(int k, int j, int r, int s) { return Sr(k, j, r, s * HQ_SLAB / UW); }
   int &P(int r, int b) { return idp[(size_t)r * NBLK + b]; }
   int &U(int r, int b, int s) { return idu[((size_t)r * NBLK + b) * S + s]; }
   int add(int kind, int k, int r, int j, int s) {
@@ -168,13 +177,13 @@ struct DagBuilder {
   void build_apply() {
     for (int k = 0; k < KT; k++) {
       for (int j = 0; j < XNT; j++)
-        for (int s = 0; s < S; s++) {
+        for (int s = 0; s < SU; s++) {
           int l = L(k, j, s) = add(HQ_LARFB, k, k, j, s);
           if (k > 0) dep(l, Sr(k - 1, j, k, s));
         }
       for (int r = k + 1; r < MT; r++)
         for (int j = 0; j < XNT; j++)
-          for (int s = 0; s < S; s++) {
+          for (int s = 0; s < SU; s++) {
             int x = Sr(k, j, r, s) = add(HQ_SSRFB, k, r, j, s);
             dep(x, r == k + 1 ? L(k, j, s) : Sr(k, j, r - 1, s));
             if (k > 0) dep(x, Sr(k - 1, j, r, s));
@@ -200,7 +209,7 @@ struct DagBuilder {
       const int p = add(kind, k, r, -1, 0);
       if (r > k) dep(p, P(r - 1, 0));
       if (k > 0)
-        for (int s = 0; s < S; s++) dep(p, Sr(k - 1, k, r, s));
+        for (int u = 0; u < SU; u++) dep(p, Sr(k - 1, k, r, u));
       for (int b = 0; b < NBLK; b++) P(r, b) = p;
       return;
     }
@@ -209,12 +218,13 @@ struct DagBuilder {
       const int p = P(r, b) = add(kind, k, r, b, 0);
       if (r > k) dep(p, P(r - 1, b));
       for (int s = ii / HQ_SLAB; s <= (j0 - 1) / HQ_SLAB; s++)
-        dep(p, b == 0 ? (k > 0 ? Sr(k - 1, k, r, s) : -1) : U(r, b - 1, s));
+        if (b > 0 || s * HQ_SLAB % UW == 0 || s == ii / HQ_SLAB)  // one dependency per update task
+          dep(p, b == 0 ? (k > 0 ? Su(k - 1, k, r, s) : -1) : U(r, b - 1, s));
       if (j0 >= nb) continue;
       for (int s = j0 / HQ_SLAB; s < S; s++) {
         const int u = U(r, b, s) = add(ukind, k, r, b, s);
         dep(u, p);
-        dep(u, b == 0 ? (k > 0 ? Sr(k - 1, k, r, s) : -1) : U(r, b - 1, s));
+        dep(u, b == 0 ? (k > 0 ? Su(k - 1, k, r, s) : -1) : U(r, b - 1, s));
         if (r > k) dep(u, U(r - 1, b, s));
       }
     }
@@ -230,7 +240,7 @@ struct DagBuilder {
       panel_tasks(k, k);
       const int g = P(k, NBLK - 1);
       for (int j = k + 1; j < NT; j++)
-        for (int s = 0; s < S; s++) {
+        for (int s = 0; s < SU; s++) {
           int l = L(k, j, s) = add(HQ_LARFB, k, k, j, s);
           dep(l, g);
           if (k > 0) dep(l, Sr(k - 1, j, k, s));
@@ -239,7 +249,7 @@ struct DagBuilder {
         panel_tasks(k, r);
         const int q = P(r, NBLK - 1);
         for (int j = k + 1; j < NT; j++)
-          for (int s = 0; s < S; s++) {
+          for (int s = 0; s < SU; s++) {
             int x = Sr(k, j, r, s) = add(HQ_SSRFB, k, r, j, s);
             dep(x, q);
             dep(x, r == k + 1 ? L(k, j, s) : Sr(k, j, r - 1, s));
@@ -253,9 +263,14 @@ struct DagBuilder {
   // (GEQRT panel 58, TSQRT panel 43, trailing tasks 22 per 64-column slab of
   // the full tile height, LARFB 80 and SSRFB 85 per slab), scaled by the
   // work of the kind; a whole-tile panel QR (batches) ≈ 540.
+  // Reduced-precision updates: the per-column cost of an update task
+  // relative to FP64 (measured means per 64 columns at nb = 256: FP32 PF1,
+  // split PF2, against 85 us in FP64).
   double cost(const hq_task &x) const {
+    static constexpr double PF1 = 0.7, PF2 = 1.0;
     const double f = nb / 256.0;
-    const double w = (double)std::min(HQ_SLAB, nb) / 64.0;  // slab width / 64
+    const double w = (double)std::min(HQ_SLAB, nb) / 64.0;  // trailing slab width / 64
+    const double wu = (double)UW / 64.0 * (prec == 1 ? PF1 : prec == 2 ? PF2 : 1.0);
     if ((x.kind == HQ_GEQRT || x.kind == HQ_TSQRT) && x.j < 0) return 540.0 * f * f * f;
     const double c = std::min(ib, nb - x.j * ib) / 32.0;     // panel block width / 32
     switch (x.kind) {
@@ -263,8 +278,8 @@ struct DagBuilder {
       case HQ_TSQRT: return 43.0 * f * c;
       case HQ_GEQRT_U: return 40.0 * f * c * w * (nb - x.j * ib) / nb;
       case HQ_TSQRT_U: return 22.0 * f * c * w;
-      case HQ_LARFB: return 80.0 * f * f * w;
-      default: return 85.0 * f * f * w;
+      case HQ_LARFB: return 80.0 * f * f * wu;
+      default: return 85.0 * f * f * wu;
     }
   }
   // List-scheduling simulation on P workers with bottom-level priorities;
@@ -370,7 +385,7 @@ struct Context {
   std::mutex host_mu;
   std::mutex lane_mu;
   std::vector<std::unique_ptr<Lane>> lanes;  // idle lanes
-  std::map<std::tuple<int, int, int, int, int>, std::unique_ptr<CachedSched>> scheds;
+  std::map<std::tuple<int, int, int, int, int, int>, std::unique_ptr<CachedSched>> scheds;
   std::map<std::pair<int, int>, std::unique_ptr<DevBuf>> tilelists;  // (MT, NT) -> int2 list
   // batches: key = (nb, ib, MT_0, NT_0, MT_1, NT_1, ...)
   std::map<std::vector<int>, std::unique_ptr<CachedSched>> batch_scheds;
@@ -445,8 +460,8 @@ int reserve_flags(Context *c, int n) {
 
 // Claim order of tasks tl (canonical order, dependencies as indices into
 // tl) from the list-scheduling simulation; dependencies remapped.
-std::vector<hq_task> claim_order(std::vector<hq_task> &&tl, int nb, int ib, int P) {
-  DagBuilder b(1, 1, nb, ib);
+std::vector<hq_task> claim_order(std::vector<hq_task> &&tl, int nb, int ib, int P, int prec = 0) {
+  DagBuilder b(1, 1, nb, ib, false, 0, true, prec);
   b.t = std::move(tl);
   std::vector<int> ord = b.order(P);
   const int n = (int)b.t.size();
@@ -461,15 +476,17 @@ std::vector<hq_task> claim_order(std::vector<hq_task> &&tl, int nb, int ib, int 
   return out;
 }
 
-int get_schedule(Context *c, int MT, int NT, int nb, int ib, CachedSched **out, int XNT = -1) {
-  // XNT < 0: factorisation DAG; XNT >= 1: Q^T application to MT x XNT tiles
-  auto key = std::make_tuple(MT, NT, nb, ib, XNT);
+int get_schedule(Context *c, int MT, int NT, int nb, int ib, CachedSched **out, int XNT = -1, int prec = 0) {
+  // XNT < 0: factorisation DAG (prec: precision of its updates); XNT >= 1:
+  // Q^T application to MT x XNT tiles (FP64)
+  if (XNT >= 1) prec = 0;
+  auto key = std::make_tuple(MT, NT, nb, ib, XNT, prec);
   auto it = c->scheds.find(key);
   if (it != c->scheds.end()) {
     *out = it->second.get();
     return HQ_OK;
   }
-  DagBuilder b(MT, NT, nb, ib, XNT >= 1, XNT);
+  DagBuilder b(MT, NT, nb, ib, XNT >= 1, XNT, true, prec);
   b.build();
   int grid = c->num_cu > 0 ? c->num_cu : 256;
   std::vector<int> ord = b.order(grid);
@@ -546,7 +563,7 @@ int tile_qr_dev_impl(int MT, int NT, int nb, int ib, double *A, double *TAU, dou
     for (const hq_task &tk : tl) HQ_HIP(hq_launch_task(G, tk, st));
   } else {
     CachedSched *cs;
-    if (get_schedule(c, MT, NT, nb, ik, &cs) != HQ_OK) return g_err;
+    if (get_schedule(c, MT, NT, nb, ik, &cs, -1, prec) != HQ_OK) return g_err;
     c->epoch++;
     HQ_HIP(hq_launch_sched(G, nullptr, (const hq_task *)cs->tasks.p, cs->ntasks, (int *)c->flags.p,
                            c->epoch, (int *)c->counter.p, cs->grid, trace, st, nullptr, c->wait_limit));
@@ -629,11 +646,12 @@ int apply_qt_impl(int MT, int NT, int nb, int ib, const double *RY, const double
 // its matrix index and the whole batch gets one claim order, so large
 // matrices' critical paths start first and small ones fill the gaps.
 int get_batch_schedule(Context *c, int count, int nb, int ib, const int *MT, const int *NT,
-                       CachedSched **out) {
+                       CachedSched **out, int prec = 0) {
   std::vector<int> key;
-  key.reserve(2 + 2 * (size_t)count);
+  key.reserve(3 + 2 * (size_t)count);
   key.push_back(nb);
   key.push_back(ib);
+  key.push_back(prec);
   for (int i = 0; i < count; i++) {
     key.push_back(MT[i]);
     key.push_back(NT[i]);
@@ -650,7 +668,7 @@ int get_batch_schedule(Context *c, int count, int nb, int ib, const int *MT, con
     if (dag.empty()) {
       // many independent matrices keep the machine busy without pipelining
       // each tile column's panel chain: one task per panel QR
-      DagBuilder b(MT[i], NT[i], nb, ib, false, 0, false);
+      DagBuilder b(MT[i], NT[i], nb, ib, false, 0, false, prec);
       b.build();
       dag = b.t;
     }
@@ -662,7 +680,7 @@ int get_batch_schedule(Context *c, int count, int nb, int ib, const int *MT, con
     }
   }
   const int grid = c->num_cu > 0 ? c->num_cu : 256;
-  std::vector<hq_task> tl = claim_order(std::move(all), nb, ib, grid);
+  std::vector<hq_task> tl = claim_order(std::move(all), nb, ib, grid, prec);
   const int n = (int)tl.size();
   std::unique_ptr<CachedSched> cs(new CachedSched());
   if (cs->tasks.reserve(sizeof(hq_task) * (size_t)n) != HQ_OK) return g_err;
@@ -744,7 +762,7 @@ int batched_qr_impl(int count, int nb, int ib, const int *MT, const int *NT, dou
   if (batch_geoms(count, nb, ib, MT, NT, A, TAU, T, c->Tbatch, c->Wbatch, c->VDbatch, g, prec, &c->VHbatch) != HQ_OK)
     return g_err;
   CachedSched *cs;
-  if (get_batch_schedule(c, count, nb, ib, MT, NT, &cs) != HQ_OK) return g_err;
+  if (get_batch_schedule(c, count, nb, ib, MT, NT, &cs, prec) != HQ_OK) return g_err;
   if (c->geoms.reserve(sizeof(hq_geom) * (size_t)count) != HQ_OK) return g_err;
   HQ_HIP(hipMemcpyAsync(c->geoms.p, g.data(), sizeof(hq_geom) * (size_t)count, hipMemcpyHostToDevice, st));
   c->epoch++;
@@ -1028,12 +1046,18 @@ int hq_tile_qr_traced_prec_dev(int MT, int NT, int nb, int ib, int prec, double 
   return tile_qr_dev_impl(MT, NT, nb, ib, A, TAU, T, stream, false, trace, prec);
 }
 
-int hq_schedule_tasks(int MT, int NT, int nb, int ib, int *out, int capacity) {
-  if (!geometry_ok(MT, NT, nb, ib)) return g_err;
+static int prec_ok(int prec, int nb) {
+  if (prec < 0 || prec > 2) return set_error(HQ_ERR_ARG, "prec=%d (0 FP64, 1 FP32, 2 split)", prec);
+  if (prec && nb < 128) return set_error(HQ_ERR_ARG, "reduced-precision updates need nb >= 128 (nb=%d)", nb);
+  return HQ_OK;
+}
+
+int hq_schedule_tasks_prec(int MT, int NT, int nb, int ib, int prec, int *out, int capacity) {
+  if (!geometry_ok(MT, NT, nb, ib) || prec_ok(prec, nb) != HQ_OK) return g_err;
   Context *c;
   if (get_ctx(&c) != HQ_OK) return g_err;
   CachedSched *cs;
-  if (get_schedule(c, MT, NT, nb, ibi(ib), &cs) != HQ_OK) return g_err;
+  if (get_schedule(c, MT, NT, nb, ibi(ib), &cs, -1, prec) != HQ_OK) return g_err;
   if (capacity < cs->ntasks * 5) return set_error(HQ_ERR_ARG, "need capacity >= 5*ntasks");
   std::vector<hq_task> tl(cs->ntasks);
   HQ_HIP(hipMemcpy(tl.data(), cs->tasks.p, sizeof(hq_task) * tl.size(), hipMemcpyDeviceToHost));
@@ -1047,11 +1071,15 @@ int hq_schedule_tasks(int MT, int NT, int nb, int ib, int *out, int capacity) {
   return ok();
 }
 
-int hq_dag_host(int MT, int NT, int nb, int ib, int XNT, int P, int *tasks, int *deps,
-                int capacity, int *ntasks) {
-  if (!geometry_ok(MT, NT, nb, ib)) return g_err;
+int hq_schedule_tasks(int MT, int NT, int nb, int ib, int *out, int capacity) {
+  return hq_schedule_tasks_prec(MT, NT, nb, ib, 0, out, capacity);
+}
+
+int hq_dag_host_prec(int MT, int NT, int nb, int ib, int prec, int XNT, int P, int *tasks, int *deps,
+                     int capacity, int *ntasks) {
+  if (!geometry_ok(MT, NT, nb, ib) || prec_ok(prec, nb) != HQ_OK) return g_err;
   if (P < 1) return set_error(HQ_ERR_ARG, "P must be >= 1");
-  DagBuilder b(MT, NT, nb, ibi(ib), XNT >= 1, XNT);
+  DagBuilder b(MT, NT, nb, ibi(ib), XNT >= 1, XNT, true, prec);
   b.build();
   const int n = (int)b.t.size();
   if (ntasks) *ntasks = n;
@@ -1071,6 +1099,11 @@ int hq_dag_host(int MT, int NT, int nb, int ib, int XNT, int P, int *tasks, int 
   return ok();
 }
 
+int hq_dag_host(int MT, int NT, int nb, int ib, int XNT, int P, int *tasks, int *deps,
+                int capacity, int *ntasks) {
+  return hq_dag_host_prec(MT, NT, nb, ib, 0, XNT, P, tasks, deps, capacity, ntasks);
+}
+
 int hq_wall_clock_khz(int *khz) {
   int dev = 0;
   HQ_HIP(hipGetDevice(&dev));
@@ -1078,10 +1111,11 @@ int hq_wall_clock_khz(int *khz) {
   return ok();
 }
 
-int hq_schedule_info(int MT, int NT, int nb, int ib, int *ntasks, int *grid) {
-  if (!geometry_ok(MT, NT, nb, ib)) return g_err;
+int hq_schedule_info_prec(int MT, int NT, int nb, int ib, int prec, int *ntasks, int *grid) {
+  if (!geometry_ok(MT, NT, nb, ib) || prec_ok(prec, nb) != HQ_OK) return g_err;
   ib = ibi(ib);
-  int KT = std::min(MT, NT), S = (nb + HQ_SLAB - 1) / HQ_SLAB;
+  const int KT = std::min(MT, NT), S = (nb + HQ_SLAB - 1) / HQ_SLAB;
+  const int UW = hq_upd_slab(nb, prec), SU = (nb + UW - 1) / UW;
   // tasks of one panel QR (GEQRT or TSQRT): a panel per ib block and a
   // trailing task per slab right of the block
   long per_tile = 0;
@@ -1091,7 +1125,7 @@ int hq_schedule_info(int MT, int NT, int nb, int ib, int *ntasks, int *grid) {
   }
   long n = 0;
   for (int k = 0; k < KT; k++)
-    n += (MT - k) * per_tile + (long)(NT - k - 1) * S + (long)(NT - k - 1) * (MT - k - 1) * S;
+    n += (MT - k) * per_tile + (long)(NT - k - 1) * SU + (long)(NT - k - 1) * (MT - k - 1) * SU;
   if (ntasks) *ntasks = (int)n;
   if (grid) {
     int dev = 0, cu = 256;
@@ -1102,6 +1136,10 @@ int hq_schedule_info(int MT, int NT, int nb, int ib, int *ntasks, int *grid) {
       *grid = (int)std::min<long>(256, n);
   }
   return ok();
+}
+
+int hq_schedule_info(int MT, int NT, int nb, int ib, int *ntasks, int *grid) {
+  return hq_schedule_info_prec(MT, NT, nb, ib, 0, ntasks, grid);
 }
 
 // Single-tile kernels: any 1 <= ib <= nb.  The panels (GEQRT / TSQRT) run

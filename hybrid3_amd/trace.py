@@ -26,10 +26,10 @@ def run_traced(MT, NT, nb, ib, A, TAU, T=None, stream=None, prec=0):
 
     import hybrid3_amd as hq
     L = hq.lib()
-    ntasks, _ = hq.schedule_info(MT, NT, nb, ib)
+    ntasks, _ = hq.schedule_info(MT, NT, nb, ib, prec)
     tasks = np.zeros(5 * ntasks, dtype=np.int32)
-    hq.check(L.hq_schedule_tasks(MT, NT, nb, ib, tasks.ctypes.data_as(C.POINTER(C.c_int)),
-                                 5 * ntasks))
+    hq.check(L.hq_schedule_tasks_prec(MT, NT, nb, ib, prec, tasks.ctypes.data_as(C.POINTER(C.c_int)),
+                                      5 * ntasks))
     tr = torch.zeros(4 * ntasks, dtype=torch.int64, device=A.device)
     hq.check(L.hq_tile_qr_traced_prec_dev(MT, NT, nb, ib, prec, hq._ptr(A), hq._ptr(TAU), hq._ptr(T),
                                           hq._ptr(tr), hq._stream(stream)))
@@ -69,14 +69,14 @@ def summarize(tasks, us, grid):
     return out
 
 
-def critical_path(tasks, us, MT, NT, nb, ib, grid):
+def critical_path(tasks, us, MT, NT, nb, ib, grid, prec=0):
     """Longest dependency chain (us) with the measured task durations, and the
     last time a task became ready and then waited more than 5 us to start
     (claim and acquire take 1-3 us; after this time the run is limited by
     the DAG, not by the claim order).  None if the host DAG's order
     differs from the traced one."""
     import hybrid3_amd as hq
-    t2, deps = hq.dag_host(MT, NT, nb, ib, 0, grid)
+    t2, deps = hq.dag_host(MT, NT, nb, ib, 0, grid, prec)
     if t2.shape != tasks.shape or not (t2 == tasks).all():
         return None
     dur = us[:, 2] - us[:, 1]
@@ -129,12 +129,12 @@ def main():
     run_traced(MT, NT, args.nb, args.ib, A, TAU, prec=args.prec)   # warm-up (schedule build, code load)
     hq.fill_uniform_dev(MT, NT, args.nb, args.seed, A)
     tasks, us = run_traced(MT, NT, args.nb, args.ib, A, TAU, prec=args.prec)
-    _, grid = hq.schedule_info(MT, NT, args.nb, args.ib)
+    _, grid = hq.schedule_info(MT, NT, args.nb, args.ib, args.prec)
     s = summarize(tasks, us, grid)
     flops = 4.0 / 3.0 * args.n ** 3 if m == args.n else 2.0 * m * args.n ** 2 - 2.0 * args.n ** 3 / 3
     s["tflops_traced"] = flops / (s["makespan_us"] * 1e-6) / 1e12
     if args.critical_path:
-        s["critical_path"] = critical_path(tasks, us, MT, NT, args.nb, args.ib, grid)
+        s["critical_path"] = critical_path(tasks, us, MT, NT, args.nb, args.ib, grid, args.prec)
     print(json.dumps(s, indent=1))
     if args.chrome:
         chrome(tasks, us, args.chrome)

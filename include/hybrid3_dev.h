@@ -123,6 +123,11 @@ int hq_tile_qr_traced_prec_dev(int MT, int NT, int nb, int ib, int prec, double 
  * 2 LARFB, 3 SSRFB), step k, row r, column j, slab.  capacity >= 5*ntasks. */
 int hq_schedule_tasks(int MT, int NT, int nb, int ib, int *out, int capacity);
 
+/* The same for the factorisation whose trailing updates run in precision
+ * prec (0 FP64, 1 FP32, 2 split): reduced-precision LARFB / SSRFB tasks
+ * cover 128 columns (slab = 128-column slab), FP64 ones 64. */
+int hq_schedule_tasks_prec(int MT, int NT, int nb, int ib, int prec, int *out, int capacity);
+
 /* Host-only: the DAG the device scheduler runs, in claim order (no GPU
  * needed).  XNT < 1: factorisation; XNT >= 1: Q^T application to MT x XNT
  * tiles.  P = number of workers of the list-scheduling simulation that
@@ -131,6 +136,9 @@ int hq_schedule_tasks(int MT, int NT, int nb, int ib, int *out, int capacity);
  * only *ntasks is set. */
 int hq_dag_host(int MT, int NT, int nb, int ib, int XNT, int P, int *tasks,
                 int *deps, int capacity, int *ntasks);
+/* hq_dag_host of the factorisation with updates in precision prec. */
+int hq_dag_host_prec(int MT, int NT, int nb, int ib, int prec, int XNT, int P, int *tasks,
+                     int *deps, int capacity, int *ntasks);
 
 /* Frequency of the device wall clock used by the trace, in kHz. */
 int hq_wall_clock_khz(int *khz);
@@ -138,6 +146,8 @@ int hq_wall_clock_khz(int *khz);
 /* Schedule of the DAG for a geometry: number of tasks and the number of
  * persistent workgroups a launch uses. */
 int hq_schedule_info(int MT, int NT, int nb, int ib, int *ntasks, int *grid);
+/* hq_schedule_info of the factorisation with updates in precision prec. */
+int hq_schedule_info_prec(int MT, int NT, int nb, int ib, int prec, int *ntasks, int *grid);
 
 /* PLASMA-style single-tile kernels on device pointers (nb in {64,128,256},
  * 1 <= ib <= nb (LARFB / SSRFB at nb = 64: ib <= 32), every leading

@@ -47,13 +47,15 @@ def test_claim_order_is_topological_and_complete(hq, MT, NT, nb, XNT):
     assert len(seen) == n
 
 
-def footprint(t, nb, ib, apply_mode):
+def footprint(t, nb, ib, apply_mode, uw=64):
     """(array, ti, tj, part, rows, cols, write) items: rows / cols are
     half-open index ranges inside the tile; part U / L / A = the entries with
-    row <= col / row > col / all of that rectangle."""
+    row <= col / row > col / all of that rectangle.  uw: column width of a
+    LARFB / SSRFB slab (the panels' trailing tasks use 64)."""
     kind, k, r, j, s = t
     full = (0, nb)
-    slab = (64 * s, min(nb, 64 * s + 64))
+    sw = uw if kind in (LARFB, SSRFB) else 64
+    slab = (sw * s, min(nb, sw * s + sw))
     X = "X" if apply_mode else "A"
     if kind in (GEQRT, TSQRT, GEQRT_U, TSQRT_U):
         ii, j0 = j * ib, min(nb, j * ib + ib)
@@ -103,6 +105,34 @@ def test_every_conflicting_pair_is_ordered(hq, MT, NT, nb, XNT):
             if d >= 0:
                 reach[i] |= reach[d] | {d}
     fps = [footprint(t, nb, ib, XNT > 0) for t in tasks]
+    for i in range(n):
+        for j in range(i):
+            if conflicts(fps[i], fps[j]):
+                assert j in reach[i], (tasks[j], tasks[i])
+
+
+@pytest.mark.parametrize("MT,NT,nb,ib,prec", [(3, 3, 128, 24, 1), (4, 3, 256, 32, 2), (2, 4, 256, 16, 1),
+                                               (5, 2, 128, 32, 2)])
+def test_wide_update_slabs_are_ordered(hq, MT, NT, nb, ib, prec):
+    """FP32-update factorisations run LARFB / SSRFB on 128-column slabs
+    beside the panels' 64-column trailing tasks: the task count matches
+    the schedule, the order is topological and every conflicting pair of
+    tasks is ordered."""
+    tasks, deps = hq.dag_host(MT, NT, nb, ib, 0, 8, prec)
+    n = len(tasks)
+    assert hq.schedule_info(MT, NT, nb, ib, prec)[0] == n
+    uw = min(128 if prec == 1 else 64, nb)
+    upd = (tasks[:, 0] == LARFB) | (tasks[:, 0] == SSRFB)
+    assert tasks[upd, 4].max() == nb // uw - 1
+    KT, SU = min(MT, NT), nb // uw
+    assert upd.sum() == sum((NT - k - 1) * SU * (MT - k) for k in range(KT))
+    reach = [set() for _ in range(n)]
+    for i in range(n):
+        for d in deps[i]:
+            assert d < i
+            if d >= 0:
+                reach[i] |= reach[d] | {d}
+    fps = [footprint(t, nb, ib, False, uw) for t in tasks]
     for i in range(n):
         for j in range(i):
             if conflicts(fps[i], fps[j]):
