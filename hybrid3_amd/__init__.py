@@ -56,6 +56,7 @@ SIGNATURES = {
     "hq_last_error_message": (C.c_char_p, []),
     "hq_version": (C.c_char_p, []),
     "hq_tile_qr_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp]),
+    "hq_tile_qr_prepare_dev": (C.c_int, [C.c_int] * 5),
     "hq_tile_qr_dev_serial": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp]),
     "hq_schedule_info": (C.c_int, [C.c_int] * 4 + [C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "hq_schedule_info_prec": (C.c_int, [C.c_int] * 5 + [C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -92,6 +93,7 @@ SIGNATURES = {
     "hq_tile_qr_mixed_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp]),
     "hq_batched_tile_qr_prec_dev": (C.c_int, [C.c_int] * 4 + [C.POINTER(C.c_int)] * 2 + [C.POINTER(_vp)] * 3
                                     + [_vp]),
+    "hq_batched_prepare_dev": (C.c_int, [C.c_int] * 4 + [C.POINTER(C.c_int)] * 2),
     "hq_cm32_to_tile_dev": (C.c_int, [C.c_int] * 2 + [C.POINTER(C.c_int)] * 2
                             + [C.POINTER(_vp), C.POINTER(C.c_int), C.POINTER(_vp), _vp]),
     "hq_tile_to_cm32_dev": (C.c_int, [C.c_int] * 2 + [C.POINTER(C.c_int)] * 2
@@ -238,6 +240,11 @@ def batched_tile_qr_prec_dev(prec, nb, ib, MT, NT, A, TAU, T=None, stream=None):
                                             _stream(stream)))
 
 
+def batched_prepare(prec, nb, ib, MT, NT):
+    """Build the schedule of a batch ahead of its first batched_tile_qr_prec_dev call."""
+    check(lib().hq_batched_prepare_dev(prec, len(MT), nb, ib, _iarr(MT), _iarr(NT)))
+
+
 def cm32_to_tile_dev(nb, m, n, Cs, ldc, A, stream=None):
     check(lib().hq_cm32_to_tile_dev(len(m), nb, _iarr(m), _iarr(n), _parr(Cs), _iarr(ldc), _parr(A),
                                     _stream(stream)))
@@ -284,6 +291,11 @@ def cm_to_tile_dev(nb, m, n, Cs, ldc, A, stream=None):
 def tile_to_cm_dev(nb, m, n, A, Cs, ldc, stream=None):
     check(lib().hq_tile_to_cm_dev(len(m), nb, _iarr(m), _iarr(n), _parr(A), _parr(Cs), _iarr(ldc),
                                   _stream(stream)))
+
+
+def tile_qr_prepare(MT, NT, nb, ib, prec=0):
+    """Build the schedule and reserve the scratch of a factorisation ahead of its first call."""
+    check(lib().hq_tile_qr_prepare_dev(MT, NT, nb, ib, prec))
 
 
 def schedule_info(MT, NT, nb, ib, prec=0):

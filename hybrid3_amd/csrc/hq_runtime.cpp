@@ -529,6 +529,25 @@ int ensure_T(Context *c, int MT, int NT, int nb, int ib, double **T) {
 
 int tile_list(Context *c, int MT, int NT, const int2 **out, int *n);
 
+// Library scratch of a factorisation (G.ib: the inner block width it runs
+// with), reserved and entered into G.
+int reserve_scratch(Context *c, hq_geom &G) {
+  const int MT = G.MT, NT = G.NT, nb = G.nb, ik = G.ib;
+  if (!G.prec && nb >= 128 && ik % 16 == 0) {
+    // the GEQRT panels also write W = V T16 and the unit lower V of the
+    // diagonal tiles; LARFB then needs no T multiplication between passes
+    const size_t tiles = sizeof(double) * (size_t)nb * nb * std::min(MT, NT);
+    if (c->W.reserve(tiles) != HQ_OK || c->VD.reserve(tiles) != HQ_OK) return g_err;
+    G.W = (double *)c->W.p;
+    G.VD = (double *)c->VD.p;
+  }
+  if (G.prec == 2) {
+    if (c->VH.reserve(2 * hq_vh_tile(nb, ik) * MT * NT) != HQ_OK) return g_err;
+    G.VH = (unsigned short *)c->VH.p;
+  }
+  return HQ_OK;
+}
+
 int tile_qr_dev_impl(int MT, int NT, int nb, int ib, double *A, double *TAU, double *T,
                      void *stream, bool serial, long long *trace = nullptr, int prec = 0) {
   if (!geometry_ok(MT, NT, nb, ib)) return g_err;
@@ -545,18 +564,7 @@ int tile_qr_dev_impl(int MT, int NT, int nb, int ib, double *A, double *TAU, dou
   if (ensure_T(c, MT, NT, nb, ik, &T) != HQ_OK) return g_err;
   if (order_in(c, st) != HQ_OK) return g_err;
   hq_geom G{A, TAU, T, MT, NT, nb, ik, A, NT, prec};
-  if (!prec && nb >= 128 && ik % 16 == 0) {
-    // the GEQRT panels also write W = V T16 and the unit lower V of the
-    // diagonal tiles; LARFB then needs no T multiplication between passes
-    const size_t tiles = sizeof(double) * (size_t)nb * nb * std::min(MT, NT);
-    if (c->W.reserve(tiles) != HQ_OK || c->VD.reserve(tiles) != HQ_OK) return g_err;
-    G.W = (double *)c->W.p;
-    G.VD = (double *)c->VD.p;
-  }
-  if (prec == 2) {
-    if (c->VH.reserve(2 * hq_vh_tile(nb, ik) * MT * NT) != HQ_OK) return g_err;
-    G.VH = (unsigned short *)c->VH.p;
-  }
+  if (reserve_scratch(c, G) != HQ_OK) return g_err;
   if (serial) {
     std::vector<hq_task> tl;
     host_schedule(MT, NT, nb, ik, c->num_cu, tl);
@@ -890,6 +898,20 @@ void mat_init(mat_stat_t *status, long int mat_m, long int mat_n, int tile_m, in
 int hq_tile_qr_dev(int MT, int NT, int nb, int ib, double *A, double *TAU, double *T,
                    void *stream) {
   return tile_qr_dev_impl(MT, NT, nb, ib, A, TAU, T, stream, false);
+}
+
+int hq_tile_qr_prepare_dev(int MT, int NT, int nb, int ib, int prec) {
+  if (!geometry_ok(MT, NT, nb, ib)) return g_err;
+  if (prec < 0 || prec > 2) return set_error(HQ_ERR_ARG, "prec=%d (0 FP64, 1 FP32, 2 split)", prec);
+  if (prec && nb < 128) return set_error(HQ_ERR_ARG, "reduced-precision updates need nb >= 128 (nb=%d)", nb);
+  Context *c;
+  if (get_ctx(&c) != HQ_OK) return g_err;
+  const int ik = ibi(ib);
+  CachedSched *cs;
+  if (get_schedule(c, MT, NT, nb, ik, &cs, -1, prec) != HQ_OK) return g_err;
+  hq_geom G{nullptr, nullptr, nullptr, MT, NT, nb, ik, nullptr, NT, prec};
+  if (reserve_scratch(c, G) != HQ_OK) return g_err;
+  return ok();
 }
 
 // ---------------------------------------------------------------------------
@@ -1238,6 +1260,21 @@ int hq_tile_make_q_dev(int MT, int NT, int nb, int ib, double *RY, const double 
 int hq_batched_tile_qr_dev(int count, int nb, int ib, const int *MT, const int *NT, double *const *A,
                            double *const *TAU, double *const *T, void *stream) {
   return batched_qr_impl(count, nb, ib, MT, NT, A, TAU, T, stream);
+}
+
+int hq_batched_prepare_dev(int prec, int count, int nb, int ib, const int *MT, const int *NT) {
+  if (count < 1 || count > 65535) return set_error(HQ_ERR_ARG, "batch of %d matrices (1..65535)", count);
+  if (!MT || !NT) return set_error(HQ_ERR_ARG, "MT and NT arrays are required");
+  if (ib > HQ_IBMAX) return set_error(HQ_ERR_ARG, "batches take ib <= %d (ib=%d)", HQ_IBMAX, ib);
+  if (prec < 0 || prec > 2) return set_error(HQ_ERR_ARG, "prec=%d (0 FP64, 1 FP32, 2 split)", prec);
+  if (prec && nb < 128) return set_error(HQ_ERR_ARG, "reduced-precision updates need nb >= 128 (nb=%d)", nb);
+  for (int i = 0; i < count; i++)
+    if (!geometry_ok(MT[i], NT[i], nb, ib)) return g_err;
+  Context *c;
+  if (get_ctx(&c) != HQ_OK) return g_err;
+  CachedSched *cs;
+  if (get_batch_schedule(c, count, nb, ib, MT, NT, &cs, prec) != HQ_OK) return g_err;
+  return ok();
 }
 
 int hq_batched_tile_qr_prec_dev(int prec, int count, int nb, int ib, const int *MT, const int *NT,

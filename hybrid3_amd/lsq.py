@@ -32,14 +32,15 @@ from __future__ import annotations
 
 import time
 
-from . import (batched_tile_qr_prec_dev, cm_to_tile_dev, sync, tile_ormqr_dev, tile_qr_dev, tile_qr_mixed_dev,
-               tile_qr_split_dev, tile_to_cm_dev)
+from . import (batched_prepare, batched_tile_qr_prec_dev, cm_to_tile_dev, sync, tile_ormqr_dev, tile_qr_dev,
+               tile_qr_mixed_dev, tile_qr_prepare, tile_qr_split_dev, tile_to_cm_dev)
 
 __all__ = ["TileQR", "lstsq", "batched_lstsq", "backward_indicator", "gaussian_orthonormal", "svd_matrix",
            "PRECISIONS"]
 
 PRECISIONS = {"fp64": 0, "fp32": 1, "split": 2}
 EPS = 2.0 ** -52
+PROGRESS = 0.8   # a CGLS step whose eta and update shrink by less than this made no progress
 
 
 def _prec(prec, mixed):
@@ -66,17 +67,17 @@ class TileQR:
         Ac = A if _is_colmajor(A) else A.t().contiguous().t()
         cm_to_tile_dev(nb, [m], [n], [_addr(Ac)], [m], [self.RY])
         fn = {"fp64": tile_qr_dev, "fp32": tile_qr_mixed_dev, "split": tile_qr_split_dev}[self.prec]
+        tile_qr_prepare(MT, NT, nb, ib, PRECISIONS[self.prec])   # schedule and scratch (cached)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         fn(MT, NT, nb, ib, self.RY, self.TAU, self.T)
         sync()
         torch.cuda.synchronize()
         self.factor_seconds = time.perf_counter() - t0
-        # R (n x n upper) column-major for the triangular solves
-        full = torch.empty(MT * nb * NT * nb, dtype=torch.float64, device=A.device)
-        tile_to_cm_dev(nb, [MT * nb], [NT * nb], [self.RY], [full], [MT * nb])
-        sync()
-        self.R = torch.triu(full.view(NT * nb, MT * nb).t()[:n, :n]).contiguous()
+        # R (n x n upper) from the top NT tile rows: tile (i, j) holds
+        # entries [i nb + row, j nb + col] at [j][i][col][row]
+        top = self.RY.view(NT, MT, nb, nb)[:, :NT]
+        self.R = torch.triu(top.permute(1, 3, 0, 2).reshape(NT * nb, NT * nb)[:n, :n]).contiguous()
 
     def qt(self, Y):
         """Q^T Y for a column-major m x k device block (k <= nb)."""
@@ -100,24 +101,32 @@ def _addr(t):
     return t.data_ptr()
 
 
-def _cgls(Amul, ATmul, rsolve, rtsolve, b, max_iter, normA, patience=5):
+def _cgls(Amul, ATmul, rsolve, rtsolve, b, max_iter, normA, patience=2):
     """R-preconditioned CGLS on every column of b (device tensors with the
     right-hand sides in the last dimension; normA: ||A||_F, a scalar or one
     per problem broadcasting against (.., 1, k)).
 
-    Each step measures every column's normwise backward-error indicator
-    eta = ||A^T r|| / (||A||_F ||r||) on its true residual and keeps the
-    iterate with the smallest eta.  It stops after `patience` steps without
-    progress (no new minimum of eta, and no relative update below half the
-    smallest so far, in any column), or after max_iter steps.  Converged
-    means every column's best eta is within the rounding floor of its FP64
-    residual, eps (||b|| + ||A||_F ||x||) / ||r||.
+    One pass over A per step forms both A x (the true residual of the
+    current iterate, so rounding in the recursive residual cannot
+    accumulate over steps) and q = A t (the step direction), a second pass
+    A^T r.  Each step measures every column's normwise backward-error
+    indicator eta = ||A^T r|| / (||A||_F ||r||) and keeps the iterate with
+    the smallest eta.  A step makes progress if some column's eta falls
+    below PROGRESS times its smallest value so far, or its relative update
+    ||dx|| / ||x|| below PROGRESS times the smallest so far (eta need not
+    fall monotonically while x converges); the iteration stops after
+    `patience` steps without progress (as LAPACK's refinement stops once the
+    backward error stops falling by a fixed factor), or after max_iter
+    steps.  Converged means every
+    column's best eta is within the rounding floor of its FP64 residual,
+    eps (||b|| + ||A||_F ||x||) / ||r||.
     Returns (x, steps, history, converged)."""
     import torch
 
     def norms(v):
         return torch.linalg.vector_norm(v, dim=-2, keepdim=True)
 
+    k = b.shape[-1]
     x = rsolve(rtsolve(ATmul(b)))              # semi-normal start
     r = b - Amul(x)
     atr = ATmul(r)
@@ -130,15 +139,14 @@ def _cgls(Amul, ATmul, rsolve, rtsolve, b, max_iter, normA, patience=5):
     hist, steps, since = [], 0, 0
     for _ in range(max_iter):
         t = rsolve(p)
-        q = Amul(t)
+        AXT = Amul(torch.cat([x, t], dim=-1))
+        r = b - AXT[..., :k]
+        q = AXT[..., k:]
         qq = (q * q).sum(dim=-2, keepdim=True)
         alpha = torch.where(qq > 0, gam / torch.where(qq > 0, qq, torch.ones_like(qq)), torch.zeros_like(qq))
         dx = alpha * t
         x = x + dx
-        # the true residual, not the recursive r - alpha q: the recursion
-        # drifts by ~eps ||A|| ||x|| per step, which for a small residual
-        # would bound the attainable accuracy of x at kappa times that drift
-        r = b - Amul(x)
+        r = r - alpha * q
         atr = ATmul(r)
         s = rtsolve(atr)
         gn = (s * s).sum(dim=-2, keepdim=True)
@@ -147,32 +155,56 @@ def _cgls(Amul, ATmul, rsolve, rtsolve, b, max_iter, normA, patience=5):
         gam = gn
         steps += 1
         eta = norms(atr) / (normA * norms(r))
+        nx = norms(x)
+        rel = norms(dx) / torch.where(nx > 0, nx, torch.ones_like(eta))
+        progress = eta < PROGRESS * best_eta
+        if best_dx is not None:
+            progress = progress | (rel < PROGRESS * best_dx)
+        best_dx = rel if best_dx is None else torch.minimum(rel, best_dx)
         better = eta < best_eta
-        # progress: a smaller eta in some column, or the update still
-        # shrinking geometrically (eta need not fall monotonically while x
-        # converges)
-        rel = norms(dx) / torch.where(norms(x) > 0, norms(x), torch.ones_like(eta))
-        shrink = rel < 0.5 * best_dx if best_dx is not None else torch.ones_like(better)
-        best_dx = rel if best_dx is None else torch.where(shrink, rel, best_dx)
-        since = 0 if bool(better.any()) or bool(shrink.any()) else since + 1
         best_x = torch.where(better, x, best_x)
         best_eta = torch.where(better, eta, best_eta)
         hist.append({"eta": eta.max().item(), "dx_rel": rel.max().item()})
+        since = 0 if bool(progress.any()) else since + 1
         if since >= patience:
             break
     r = b - Amul(best_x)
     floor = EPS * (norms(b) + normA * norms(best_x)) / norms(r)
-    conv = since >= patience and bool((best_eta <= floor).all())
+    conv = bool((best_eta <= floor).all())
     return best_x, steps, hist, conv
 
 
-def lstsq(A, b, nb=256, ib=32, mixed=False, max_iter=None, tol=None, factor=None, prec=None, method="cgls"):
+def _tri_inv(R, leaf=1024):
+    """Inverse of an upper triangular R by recursive halving:
+    [A B; 0 C]^-1 = [A^-1, -A^-1 B C^-1; 0, C^-1], the off-diagonal blocks
+    as two GEMMs; leaves by a triangular solve.  (One triangular solve
+    against the identity is a few times slower on rocBLAS.)  Works on
+    batches (..., n, n)."""
+    import torch
+    n = R.shape[-1]
+    if n <= leaf:
+        eye = torch.eye(n, dtype=R.dtype, device=R.device).expand(R.shape)
+        return torch.linalg.solve_triangular(R, eye, upper=True)
+    h = (n // 2 + 255) // 256 * 256 if n > 512 else n // 2
+    Ai, Ci = _tri_inv(R[..., :h, :h], leaf), _tri_inv(R[..., h:, h:], leaf)
+    out = torch.zeros_like(R)
+    out[..., :h, :h] = Ai
+    out[..., h:, h:] = Ci
+    out[..., :h, h:] = -(Ai @ R[..., :h, h:]) @ Ci
+    return out
+
+
+def lstsq(A, b, nb=256, ib=32, mixed=False, max_iter=None, tol=None, factor=None, prec=None, method="cgls",
+          fallback=True):
     """x minimising ||A x - b||_2 (A m x n, b m x k, FP64 torch device tensors).
 
     Returns (x, info): info holds the factorisation and total times, the
     refinement steps taken, per step ||dx|| / ||x||, whether the iteration
     reached FP64 rounding (``converged``), and the final normwise
-    backward-error indicator ||A^T r|| / (||A||_F ||r||)."""
+    backward-error indicator ||A^T r|| / (||A||_F ||r||).  With
+    ``fallback``, a reduced-precision solve that does not converge (kappa
+    beyond about 1/u of its factorisation) is repeated on the FP64
+    factorisation (info["fallback"]; the times include both)."""
     import torch
     m, n = A.shape
     if b.dim() == 1:
@@ -193,7 +225,7 @@ def lstsq(A, b, nb=256, ib=32, mixed=False, max_iter=None, tol=None, factor=None
         # the preconditioner as an explicit inverse (one BLAS-3 triangular
         # solve): every step is then matrix products only.  It need not be
         # exact -- x is updated directly and its residual formed with A.
-        Ri = torch.linalg.solve_triangular(R, torch.eye(n, dtype=R.dtype, device=R.device), upper=True)
+        Ri = _tri_inv(R)
         RiT = Ri.t().contiguous()
         x, steps, hist, conv = _cgls(lambda v: A @ v, lambda v: A.t() @ v, lambda c: Ri @ c, lambda c: RiT @ c, b,
                                      100 if max_iter is None else max_iter, torch.linalg.norm(A))
@@ -231,7 +263,14 @@ def lstsq(A, b, nb=256, ib=32, mixed=False, max_iter=None, tol=None, factor=None
     torch.cuda.synchronize()
     info = {"factor_seconds": F.factor_seconds, "total_seconds": time.perf_counter() - t0,
             "steps": steps, "history": hist, "prec": F.prec, "mixed": F.prec != "fp64", "method": method,
-            "converged": conv, "backward": backward_indicator(A, b, x)}
+            "converged": conv, "fallback": False}
+    if fallback and not conv and F.prec != "fp64" and factor is None:
+        del F, R
+        x, i2 = lstsq(A, b, nb=nb, ib=ib, max_iter=max_iter, tol=tol, prec="fp64", method=method, fallback=False)
+        info.update({"total_seconds": info["total_seconds"] + i2["total_seconds"], "converged": i2["converged"],
+                     "fallback": True, "fallback_steps": i2["steps"],
+                     "factor_seconds": info["factor_seconds"] + i2["factor_seconds"]})
+    info["backward"] = backward_indicator(A, b, x)
     return x, info
 
 
@@ -303,6 +342,7 @@ def batched_lstsq(groups, nb=256, ib=32, prec="fp64", max_iter=100):
             NT.append(nt)
             cols.append(ry[i].data_ptr())
             owners.append(ta[i].data_ptr())
+    batched_prepare(pv, nb, ib, MT, NT)   # schedule (cached)
     sync()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -316,15 +356,10 @@ def batched_lstsq(groups, nb=256, ib=32, prec="fp64", max_iter=100):
         A, b = groups[sh]
         B = A.shape[0]
         mt, nt = (m + nb - 1) // nb, (n + nb - 1) // nb
-        full = torch.empty(B, nt * nb, mt * nb, dtype=torch.float64, device="cuda")
-        tile_to_cm_dev(nb, [mt * nb] * B, [nt * nb] * B, [RY[g][i].data_ptr() for i in range(B)],
-                       [full[i].data_ptr() for i in range(B)], [mt * nb] * B)
-        sync()
-        R = torch.triu(full[:, :n, :n].transpose(1, 2)).contiguous()
-        del full
+        top = RY[g].view(B, nt, mt, nb, nb)[:, :, :nt]     # [b][tj][ti][col][row]
+        R = torch.triu(top.permute(0, 2, 4, 1, 3).reshape(B, nt * nb, nt * nb)[:, :n, :n]).contiguous()
         At = A.transpose(1, 2)      # (B, m, n) view of the column-major storage
-        eye = torch.eye(n, dtype=R.dtype, device=R.device).expand(B, n, n)
-        Ri = torch.linalg.solve_triangular(R, eye, upper=True)   # preconditioner, explicit
+        Ri = _tri_inv(R)            # preconditioner, explicit
         RiT = Ri.transpose(1, 2).contiguous()
         normA = torch.linalg.matrix_norm(A).view(B, 1, 1)
         x, st, _, cv = _cgls(lambda v: torch.bmm(At, v), lambda v: torch.bmm(A, v), lambda c: torch.bmm(Ri, c),

@@ -53,6 +53,13 @@ const char *hq_version(void);
 int hq_tile_qr_dev(int MT, int NT, int nb, int ib, double *A, double *TAU,
                    double *T, void *stream);
 
+/* Builds and caches the schedule of the factorisation of an MT x NT tile
+ * grid with updates in precision prec (0 FP64, 1 FP32, 2 split) and reserves
+ * the library scratch it uses (all but T, which a call either passes or
+ * reserves itself), so that the calls that follow spend no host time on
+ * them.  Optional: the first call of a geometry does the same work. */
+int hq_tile_qr_prepare_dev(int MT, int NT, int nb, int ib, int prec);
+
 /* hq_tile_qr_dev for an m x n matrix whose sizes need not be multiples of
  * nb (edge tiles, which the reference's integer division drops,
  * mat_prod.c:24-29): A holds ceil(m/nb) x ceil(n/nb) tiles in the layout
@@ -216,6 +223,10 @@ int hq_batched_tile_qr_dev(int count, int nb, int ib, const int *MT, const int *
  * hq_tile_qr_split_dev); nb >= 128 for prec 1 and 2. */
 int hq_batched_tile_qr_prec_dev(int prec, int count, int nb, int ib, const int *MT, const int *NT,
                                 double *const *A, double *const *TAU, double *const *T, void *stream);
+
+/* Builds and caches the schedule of a batch (the shapes, nb, ib and update
+ * precision of a later hq_batched_tile_qr_prec_dev call) ahead of it. */
+int hq_batched_prepare_dev(int prec, int count, int nb, int ib, const int *MT, const int *NT);
 
 /* The same factorisation as a plan prepared once and run many times (no
  * counterpart in the reference; for callers that capture the launch in a
