@@ -148,6 +148,7 @@ struct Smem {
       double G[TSZ];   // Gram V^T V, G[p*LDT + c]
     };
   };
+  double XA[16 * HQ_SLAB];  // staging of the pipelined split update (16 rows of a slab)
   double tau[HQ_IBMAX];
   int task;
   int nref;     // reflectors published in the current panel factorisation
@@ -1872,15 +1873,24 @@ __device__ __forceinline__ void hsplit(float x, _Float16 &h, _Float16 &l) {
 //            i*NB + 4*((r >> 2) ^ i) + (r & 3) (4-row chunks swizzled by
 //            the reflector index, so the 16 reflectors of an accumulate
 //            operand read fall on different LDS banks);
-//   UH / UL: V hi / lo, row-major, element (r, i) at r*16 + i.
-// 128 NB bytes, a whole number of 1 KiB direct-to-LDS wave copies.
+//   UH / UL: V hi / lo, row-major, element (r, i) at
+//            r*16 + 4*((i >> 2) ^ ((r >> 2) & 3)) + (i & 3) (the 4-reflector
+//            chunks of a row swizzled by the row, so that the 16 rows of an
+//            update operand read fall on different LDS banks).
+// 128 NB bytes, a whole number of 1 KiB direct-to-LDS wave copies.  (The
+// update operands could also be read from AH / AL with the LDS transpose
+// load, halving the image; but the compiler makes every such read wait for
+// all direct-to-LDS copies in flight, which the pipelined update cannot
+// afford.)
 template <int NB>
 struct HSlot {
   static constexpr int BYTES = 128 * NB;
+  static constexpr int HALVES = 64 * NB;                                   // image size in halves
   static constexpr int AH = 0, AL = 16 * NB, UH = 32 * NB, UL = 48 * NB;  // offsets in halves
   static constexpr int NOPS = BYTES / 1024 / (HQ_THREADS / 64);            // copies per wave
 };
 __device__ __forceinline__ int hswz(int i, int r) { return 4 * ((r >> 2) ^ i) + (r & 3); }
+__device__ __forceinline__ int uswz(int r, int q) { return r * 16 + 4 * (q ^ ((r >> 2) & 3)); }  // chunk q of row r
 // Ring slot i (0..3): two slots in S.V, two in S.V1.
 template <int NB>
 __device__ __forceinline__ _Float16 *hvbuf(Smem<NB> &S, int i) {
@@ -1902,7 +1912,7 @@ __device__ __forceinline__ void store_vh(Smem<NB> &S, unsigned short *VHt, int i
   typedef _Float16 h4t __attribute__((ext_vector_type(4)));
   for (int h = 0; h < nh; h++) {
     const int kb = min(16, sb - 16 * h);
-    _Float16 *img = reinterpret_cast<_Float16 *>(VHt) + (size_t)(s0 + h) * 64 * NB;
+    _Float16 *img = reinterpret_cast<_Float16 *>(VHt) + (size_t)(s0 + h) * L::HALVES;
     // V^T arrays: chunk (i, c) = rows 4c..4c+3 of reflector i
     for (int e = threadIdx.x; e < 16 * (NB / 4); e += HQ_THREADS) {
       const int i = e / (NB / 4), c = e - i * (NB / 4);
@@ -1931,8 +1941,8 @@ __device__ __forceinline__ void store_vh(Smem<NB> &S, unsigned short *VHt, int i
         hv[t] = a;
         lv[t] = b;
       }
-      *reinterpret_cast<h4t *>(img + L::UH + r * 16 + 4 * q) = hv;
-      *reinterpret_cast<h4t *>(img + L::UL + r * 16 + 4 * q) = lv;
+      *reinterpret_cast<h4t *>(img + L::UH + uswz(r, q)) = hv;
+      *reinterpret_cast<h4t *>(img + L::UL + uswz(r, q)) = lv;
     }
   }
 }
@@ -1960,8 +1970,8 @@ __device__ __forceinline__ void ldh_u(HOpU &o, const _Float16 *Vb, int G) {
 #pragma unroll
   for (int t = 0; t < 2; t++) {
     const int row = 16 * (2 * G + t) + i;
-    o.h[t] = *reinterpret_cast<const h4 *>(Vb + L::UH + row * 16 + 4 * g);
-    o.l[t] = *reinterpret_cast<const h4 *>(Vb + L::UL + row * 16 + 4 * g);
+    o.h[t] = *reinterpret_cast<const h4 *>(Vb + L::UH + uswz(row, g));
+    o.l[t] = *reinterpret_cast<const h4 *>(Vb + L::UL + uswz(row, g));
   }
 }
 // Operands of one accumulate phase: V^T(i, rows 32G + 4g.. and 32G + 16 + 4g..)
@@ -2075,7 +2085,7 @@ __device__ __forceinline__ void fpassh(f4 (&acc)[NS][NB / 16], const h4 (&W2h)[N
 // strips w + 4q, q < NS).  VHt: the reflector tile's images.
 template <int NB, bool UNIT, int NS>
 __device__ __forceinline__ void task_update_h(Smem<NB> &S, const unsigned short *VHt, const double *T, double *X,
-                                              double *A1, int ib, int slab) {
+                                              double *A1, int ib, int slab, const int *CE) {
   constexpr int UW = 64 * NS;
   static_assert(NB % UW == 0, "slabs cover the tile exactly");
   const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, g = lane >> 4, wave = tid >> 6;
@@ -2087,7 +2097,7 @@ __device__ __forceinline__ void task_update_h(Smem<NB> &S, const unsigned short 
   const int nsub = sub_count<NB>(ib);
   const _Float16 *img0 = reinterpret_cast<const _Float16 *>(VHt);
   auto sbk = [&](int s) __attribute__((always_inline)) { return sub_blk<NB>(min(s, nsub - 1), ib); };
-  auto img = [&](int s) __attribute__((always_inline)) { return img0 + (size_t)min(s, nsub - 1) * 64 * NB; };
+  auto img = [&](int s) __attribute__((always_inline)) { return img0 + (size_t)min(s, nsub - 1) * HSlot<NB>::HALVES; };
   f4 acc[NS][NB / 16];
   f4 Wn[NS][2], a1v[NS];  // a1v: the A1 rows read before a pass, their new values after it
   h4 W2h[NS], W2l[NS];
@@ -2112,6 +2122,19 @@ __device__ __forceinline__ void task_update_h(Smem<NB> &S, const unsigned short 
     }
 #pragma unroll
     for (int q = 0; q < NS; q++) {
+      if (CE) {
+        // the column's scale from its 2-norm: one sweep over the strip
+        const int sh = CE[col0 + q * 64];
+        const double dsc = ldexp(1.0, sh);
+        xs[q] = (float)dsc;
+        xu[q] = (float)ldexp(1.0, -sh);
+#pragma unroll
+        for (int rb = 0; rb < NB / 16; rb++) {
+          const d4 x = *reinterpret_cast<const d4 *>(xcol + q * QS + 16 * rb + 4 * g);
+          acc[q][rb] = f4{(float)(x[0] * dsc), (float)(x[1] * dsc), (float)(x[2] * dsc), (float)(x[3] * dsc)};
+        }
+        continue;
+      }
       double mx = 0.0;
 #pragma unroll 4
       for (int rb = 0; rb < NB / 16; rb++) {
@@ -2237,15 +2260,270 @@ __device__ __forceinline__ void task_update_h(Smem<NB> &S, const unsigned short 
   __builtin_amdgcn_s_waitcnt(0);
 }
 
+// Split-precision LARFB / SSRFB pipelined two passes deep (ib a multiple of
+// 16, so every sub-block is 16 reflectors at row 16 s; precomputed column
+// scales).  Pass p copies to LDS the image of sub-block p+3, its 16 x 16 T
+// block and the A1 rows of sub-block p+2; the end of pass p waits only for
+// the copies of pass p-1 (s_waitcnt vmcnt(N), N = this wave's copies of pass
+// p: loads retire in issue order, and stores in between only make the wait
+// longer) and synchronises with a barrier that leaves the copies of pass p in
+// flight.  No register load is in flight during the passes: the compiler's
+// own wait for one would wait for every copy.  Staging: the A1 rows of
+// sub-block j in buffer j % 2 (S.TT1, S.XA; each wave copies and reads its
+// own 16 columns), the T block of j in buffer j % 2 (S.TT past the FP32 T
+// ring; wave w copies columns 8 (w / 2) .. +7, its own 4 and its partner's
+// -- the same bytes -- and converts its own after its own wait).
+// Workgroup barrier that leaves vector memory operations in flight: no
+// memory-model fence (__syncthreads()'s fences, even LDS-only ones while a
+// direct-to-LDS copy is in flight, make every wave wait for all of its
+// vector memory operations), only the hardware barrier with this wave's LDS
+// operations complete before it and compiler barriers that keep memory
+// accesses on their side of it.  The caller waits explicitly, before it,
+// for the copies whose LDS data is read after it.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0); vmcnt, expcnt unconstrained
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+template <int NB>
+struct H2Stage {
+  static constexpr int A1B = 16 * HQ_SLAB;   // doubles per A1 buffer (16 rows of the slab)
+  static constexpr int TB = 256;             // doubles per T buffer
+  static constexpr int T0 = 4 * 16 * LDT16 / 2;  // doubles of S.TT before the T buffers (the FP32 ring)
+  static_assert(A1B <= Smem<NB>::TSZ && A1B <= 16 * HQ_SLAB, "A1 buffers fit");
+  static_assert(T0 + 2 * TB <= Smem<NB>::TSZ, "T buffers fit in S.TT");
+};
+template <int NB>
+__device__ __forceinline__ void dmat_op(double *Ts, const double *T, int ib, int s) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int c = 8 * (wave >> 1) + (lane >> 3);      // column of the T block
+  const int i0 = 16 * s, p0 = i0 % ib;
+  const double *src = T + (size_t)(i0 + c) * ib + p0 + 2 * (lane & 7);
+  lds_char *dst = (lds_char *)(Ts + 128 * (wave >> 1));
+  __builtin_amdgcn_global_load_lds((const void *)src, (void __attribute__((address_space(3))) *)dst, 16, 0, 0);
+}
+// copy q (0, 1) of this wave's A1 rows 16 s .. 16 s + 15 of slab columns
+// 16 w + 8 q .. +7 (column c of the slab at 16 c doubles of the buffer)
+template <int NB>
+__device__ __forceinline__ void dmaa_op(int q, double *Ab, const double *A1slab, int s) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int c = 16 * wave + 8 * q + (lane >> 3);
+  const double *src = A1slab + (size_t)c * NB + 16 * s + 2 * (lane & 7);
+  lds_char *dst = (lds_char *)(Ab + 16 * (16 * wave + 8 * q));
+  __builtin_amdgcn_global_load_lds((const void *)src, (void __attribute__((address_space(3))) *)dst, 16, 0, 0);
+}
+
+// One pass of the two-deep split pipeline: fpassh's phases, the copies
+// given by copy(k), k < NC, copy k in phase min(k, NP - 1).
+template <int NB, bool DO_U, bool DO_A, int NC, class Copy>
+__device__ __forceinline__ void fpassh2(f4 (&acc)[NB / 16], const h4 &W2h, const h4 &W2l, f4 (&Wn)[2],
+                                        float (&tto)[4], const _Float16 *Vu, const _Float16 *Vw, const float *TTw,
+                                        Copy &&copy) {
+  constexpr int NG = NB / 32;
+  constexpr int NP = (DO_U ? NG : 0) + (DO_A ? NG : 0);
+  auto kind = [](int p) -> int {
+    if (DO_U && !DO_A) return 1;
+    if (!DO_U) return 0;
+    if (p == 0) return 1;
+    if (p == 2 * NG - 1) return 0;
+    return (p & 1) ? 1 : 0;
+  };
+  auto grp = [](int p) -> int {
+    if (!(DO_U && DO_A)) return p;
+    if (p == 0) return 0;
+    if (p == 2 * NG - 1) return NG - 1;
+    return (p & 1) ? (p + 1) / 2 : p / 2 - 1;
+  };
+  const int lane = threadIdx.x & 63, g = lane >> 4, m = lane & 15;
+  HOpU ou[2];
+  HOpA oa[2];
+  if (kind(0))
+    ldh_u<NB>(ou[0], Vu, grp(0));
+  else
+    ldh_a<NB>(oa[0], Vw, grp(0));
+#pragma unroll
+  for (int p = 0; p < NP; p++) {
+#pragma unroll
+    for (int k = 0; k < NC; k++)
+      if ((k < NP - 1 ? k : NP - 1) == p) copy(k);
+    if (p + 1 < NP) {
+      if (kind(p + 1))
+        ldh_u<NB>(ou[(p + 1) & 1], Vu, grp(p + 1));
+      else
+        ldh_a<NB>(oa[(p + 1) & 1], Vw, grp(p + 1));
+    } else if (DO_A) {
+#pragma unroll
+      for (int sp = 0; sp < 4; sp++) tto[sp] = TTw[m * LDT16 + 4 * g + sp];
+    }
+    const int G = grp(p);
+    if (kind(p)) {
+      const HOpU &o = ou[p & 1];
+#pragma unroll
+      for (int t = 0; t < 2; t++) acc[2 * G + t] = mfma_h16(o.h[t], W2h, acc[2 * G + t]);
+#pragma unroll
+      for (int t = 0; t < 2; t++) acc[2 * G + t] = mfma_h16(o.h[t], W2l, acc[2 * G + t]);
+#pragma unroll
+      for (int t = 0; t < 2; t++) acc[2 * G + t] = mfma_h16(o.l[t], W2h, acc[2 * G + t]);
+    } else {
+      const HOpA &o = oa[p & 1];
+      const f4 x0 = acc[2 * G], x1 = acc[2 * G + 1];
+      h8 xh, xl;
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        _Float16 h, l;
+        hsplit(x0[e], h, l);
+        xh[e] = h;
+        xl[e] = l;
+        hsplit(x1[e], h, l);
+        xh[4 + e] = h;
+        xl[4 + e] = l;
+      }
+      Wn[0] = mfma_h32(o.h, xh, Wn[0]);
+      Wn[1] = mfma_h32(o.h, xl, Wn[1]);
+      Wn[0] = mfma_h32(o.l, xh, Wn[0]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+template <int NB, bool UNIT>
+__device__ __forceinline__ void task_update_h2(Smem<NB> &S, const unsigned short *VHt, const double *T, double *X,
+                                               double *A1, int ib, int slab, const int *CE) {
+  static_assert(NB % HQ_SLAB == 0, "slabs cover the tile exactly");
+  using ST = H2Stage<NB>;
+  constexpr int NSUB = NB / 16;
+  constexpr int NC = HSlot<NB>::NOPS + 1 + (UNIT ? 0 : 2);  // copies per wave and pass
+  const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, g = lane >> 4, wave = tid >> 6;
+  const int col = slab * HQ_SLAB + 16 * wave + m;
+  double *xcol = X + (size_t)col * NB;
+  const double *a1slab = UNIT ? nullptr : A1 + (size_t)slab * HQ_SLAB * NB;
+  double *a1col = UNIT ? nullptr : A1 + (size_t)col * NB;
+  const _Float16 *img0 = reinterpret_cast<const _Float16 *>(VHt);
+  auto img = [&](int s) __attribute__((always_inline)) { return img0 + (size_t)min(s, NSUB - 1) * HSlot<NB>::HALVES; };
+  auto abuf = [&](int s) __attribute__((always_inline)) { return (s & 1) ? S.XA : S.TT1; };
+  auto tbuf = [&](int s) __attribute__((always_inline)) { return S.TT + ST::T0 + ST::TB * (s & 1); };
+  // the copies of pass p: image p+3 into its ring slot, T of p+3, A1 rows of p+2
+  auto copies = [&](int p) __attribute__((always_inline)) {
+    const int si = min(p + 3, NSUB - 1), sa = min(p + 2, NSUB - 1);
+    _Float16 *slot = hvbuf(S, (p + 3) & 3);
+    const _Float16 *src = img(p + 3);
+    double *tb = tbuf(p + 3), *ab = abuf(p + 2);
+    return [=, &S](int k) __attribute__((always_inline)) {
+      if (k < HSlot<NB>::NOPS)
+        dmah_op<NB>(k, slot, src);
+      else if (k == HSlot<NB>::NOPS)
+        dmat_op<NB>(tb, T, ib, si);
+      else if (!UNIT)
+        dmaa_op<NB>(k - HSlot<NB>::NOPS - 1, ab, a1slab, sa);
+    };
+  };
+  f4 acc[NB / 16];
+  f4 Wn[2];
+  h4 W2h = h4{0, 0, 0, 0}, W2l = W2h;
+  float tto[4];
+  const int sh = CE[col];
+  const double dsc = ldexp(1.0, sh);
+  const float xs = (float)dsc, xu = (float)ldexp(1.0, -sh);
+  const int c = tid >> 4, pr = tid & 15;  // this thread's element of a 16 x 16 T block
+  // prologue: images 0 and 1, the A1 rows of 0, T^T of 0 and 1 (register
+  // loads), the strip; all waited for
+  {
+    if (!UNIT) {
+      dmaa_op<NB>(0, abuf(0), a1slab, 0);
+      dmaa_op<NB>(1, abuf(0), a1slab, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+#pragma unroll
+      for (int q = 0; q < HSlot<NB>::NOPS; q++) dmah_op<NB>(q, hvbuf(S, j), img(j));
+      const double tv = *gmem(T + (size_t)(16 * j + c) * ib + (16 * j) % ib + pr);
+      f32tbuf(S, j)[c * LDT16 + pr] = pr <= c ? (float)tv : 0.f;
+    }
+#pragma unroll
+    for (int rb = 0; rb < NB / 16; rb++) {
+      const d4 x = *reinterpret_cast<const d4 *>(xcol + 16 * rb + 4 * g);
+      acc[rb] = f4{(float)(x[0] * dsc), (float)(x[1] * dsc), (float)(x[2] * dsc), (float)(x[3] * dsc)};
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+  }
+  // after pass s: this wave's copies of pass s-1 waited for; W2 of sub-block
+  // s+1 and its FP16 split; T^T of s+2 into the ring; barrier; the new A1 rows
+  // of s+1 stored
+  auto post = [&](int s, bool copied) __attribute__((always_inline)) {
+    if (copied)
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm_lgkm0(NC));
+    else
+      __builtin_amdgcn_s_waitcnt(0);
+    f4 W = (Wn[0] + Wn[1]) * HV_UNSCALE;
+    f4 a1 = f4{0.f, 0.f, 0.f, 0.f};
+    if (!UNIT) {
+      const double *ar = abuf(s + 1) + 16 * (16 * wave + m) + 4 * g;
+#pragma unroll
+      for (int v = 0; v < 4; v++) a1[v] = (float)(ar[v] * dsc);
+      W += a1;
+    }
+    f4 t = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int sp = 0; sp < 4; sp++) t = mfma32(tto[sp], W[sp], t);
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      _Float16 h, l;
+      hsplit(-t[v] * HV_UNSCALE, h, l);
+      W2h[v] = h;
+      W2l[v] = l;
+    }
+    if (s >= 0 && s + 2 < NSUB)
+      f32tbuf(S, (s + 2) & 3)[c * LDT16 + pr] = pr <= c ? (float)tbuf(s + 2)[16 * c + pr] : 0.f;
+    lds_barrier();
+    if (!UNIT) {
+      const f4 a1o = a1 - t;
+      *reinterpret_cast<d4 *>(a1col + 16 * (s + 1) + 4 * g) =
+          d4{(double)a1o[0] * xu, (double)a1o[1] * xu, (double)a1o[2] * xu, (double)a1o[3] * xu};
+    }
+    Wn[0] = f4{0.f, 0.f, 0.f, 0.f};
+    Wn[1] = Wn[0];
+  };
+  Wn[0] = f4{0.f, 0.f, 0.f, 0.f};
+  Wn[1] = Wn[0];
+  // pass -1: W of sub-block 0
+  fpassh2<NB, false, true, NC>(acc, W2h, W2l, Wn, tto, hvbuf(S, 3), hvbuf(S, 0), f32tbuf(S, 0), copies(-1));
+  post(-1, true);
+  // pass s: update with s, accumulate with s+1 (copies clamped to the last
+  // sub-block: every pass makes the same number)
+  for (int s = 0; s + 1 < NSUB; s++) {
+    fpassh2<NB, true, true, NC>(acc, W2h, W2l, Wn, tto, hvbuf(S, s & 3), hvbuf(S, (s + 1) & 3),
+                                f32tbuf(S, (s + 1) & 3), copies(s));
+    post(s, true);
+  }
+  fpassh2<NB, true, false, 0>(acc, W2h, W2l, Wn, tto, hvbuf(S, (NSUB - 1) & 3), hvbuf(S, 0), f32tbuf(S, 0),
+                              [](int) {});
+  const double u = (double)xu;
+#pragma unroll
+  for (int rb = 0; rb < NB / 16; rb++) {
+    const f4 x = acc[rb];
+    *reinterpret_cast<d4 *>(xcol + 16 * rb + 4 * g) = d4{(double)x[0] * u, (double)x[1] * u, (double)x[2] * u, (double)x[3] * u};
+  }
+  // no copy may still be in flight when the next task reuses LDS
+  __builtin_amdgcn_s_waitcnt(0);
+}
+
 // PREC (hq_geom::prec) is a template parameter: each precision of the
 // updates is its own kernel instantiation with its own register allocation.
 template <int NB, int WB, int PREC = 0>
 __device__ __forceinline__ int2 task_larfb(Smem<NB> &S, const double *V, const double *T, double *C,
                                            int ib, int slab, const double *Wt = nullptr,
                                            const double *Vd = nullptr, Ahead ah = Ahead{},
-                                           const unsigned short *VHt = nullptr) {
+                                           const unsigned short *VHt = nullptr, const int *CE = nullptr) {
   if constexpr (NB >= 128 && PREC == 2) {
-    task_update_h<NB, true, hq_upd_slab(NB, 2) / 64>(S, VHt, T, C, nullptr, ib, slab);
+    if (CE && ib % 16 == 0)
+      task_update_h2<NB, true>(S, VHt, T, C, nullptr, ib, slab, CE);
+    else
+      task_update_h<NB, true, hq_upd_slab(NB, 2) / 64>(S, VHt, T, C, nullptr, ib, slab, CE);
   } else if constexpr (NB >= 128 && PREC == 1) {
     task_update_f32<NB, true, hq_upd_slab(NB, 1) / 64>(S, V, T, C, nullptr, ib, slab);
   } else if constexpr (NB >= 128) {
@@ -2262,9 +2540,12 @@ __device__ __forceinline__ int2 task_larfb(Smem<NB> &S, const double *V, const d
 template <int NB, int WB, int PREC = 0>
 __device__ __forceinline__ int2 task_ssrfb(Smem<NB> &S, double *A1, double *A2, const double *V,
                                            const double *T, int ib, int slab, Ahead ah = Ahead{},
-                                           const unsigned short *VHt = nullptr) {
+                                           const unsigned short *VHt = nullptr, const int *CE = nullptr) {
   if constexpr (NB >= 128 && PREC == 2) {
-    task_update_h<NB, false, hq_upd_slab(NB, 2) / 64>(S, VHt, T, A2, A1, ib, slab);
+    if (CE && ib % 16 == 0)
+      task_update_h2<NB, false>(S, VHt, T, A2, A1, ib, slab, CE);
+    else
+      task_update_h<NB, false, hq_upd_slab(NB, 2) / 64>(S, VHt, T, A2, A1, ib, slab, CE);
   } else if constexpr (NB >= 128 && PREC == 1) {
     task_update_f32<NB, false, hq_upd_slab(NB, 1) / 64>(S, V, T, A2, A1, ib, slab);
   } else if constexpr (NB >= 128) {
@@ -2295,6 +2576,10 @@ __device__ __forceinline__ double *vd_ptr(const hq_geom &G, int k) {
 __device__ __forceinline__ unsigned short *vh_ptr(const hq_geom &G, int i, int j) {
   const size_t per_tile = (size_t)((G.nb + G.ib - 1) / G.ib) * ((G.ib + 15) / 16) * 64 * G.nb;
   return G.VH ? G.VH + ((size_t)i + (size_t)j * G.MT) * per_tile : nullptr;
+}
+// column scales of tile column j (split factorisation), or nullptr
+__device__ __forceinline__ const int *ce_ptr(const hq_geom &G, int j) {
+  return G.CE ? G.CE + (size_t)j * G.nb : nullptr;
 }
 __device__ __forceinline__ double *tau_ptr(const hq_geom &G, int i, int j) {
   return G.TAU + ((size_t)i + (size_t)j * G.MT) * (size_t)G.nb;
@@ -2329,10 +2614,11 @@ __device__ __forceinline__ int2 run_task(Smem<NB> &S, const hq_geom &G, const hq
       break;
     case HQ_LARFB:
       return task_larfb<NB, WB, PREC>(S, tile_ptr(G, t.k, t.k), t_ptr(G, t.k, t.k), xtile_ptr(G, t.k, t.j), G.ib,
-                                      t.slab, wd_ptr(G, t.k), vd_ptr(G, t.k), ah, vh_ptr(G, t.k, t.k));
+                                      t.slab, wd_ptr(G, t.k), vd_ptr(G, t.k), ah, vh_ptr(G, t.k, t.k),
+                                      ce_ptr(G, t.j));
     case HQ_SSRFB:
       return task_ssrfb<NB, WB, PREC>(S, xtile_ptr(G, t.k, t.j), xtile_ptr(G, t.r, t.j), tile_ptr(G, t.r, t.k),
-                                      t_ptr(G, t.r, t.k), G.ib, t.slab, ah, vh_ptr(G, t.r, t.k));
+                                      t_ptr(G, t.r, t.k), G.ib, t.slab, ah, vh_ptr(G, t.r, t.k), ce_ptr(G, t.j));
     default:
       break;
   }
@@ -2602,6 +2888,47 @@ __global__ __launch_bounds__(HQ_THREADS) void tbuild_wide_tile_kernel(const doub
 
 // Zero the entries of an MT x NT tile grid past row m / column n (the
 // padding of edge tiles), touching only the last tile row and column.
+// Exponent sh of each matrix column's split scale: one wave per column, the
+// column's 2-norm N over the MT tiles by the scaled sum of squares (LAPACK
+// dlassq: no overflow or underflow for any finite entries), then
+// sh = 12 - e with N < 2^e, so 2^sh N < 2^12 (sh = 0 for a zero column).
+__global__ __launch_bounds__(256) void colscale_kernel(const double *A, int MT, int NT, int nb, int *CE) {
+  const int lane = threadIdx.x & 63;
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (c >= NT * nb) return;
+  const int tj = c / nb, cc = c - tj * nb;
+  double sc = 0.0, sq = 1.0;
+  for (int ti = 0; ti < MT; ti++) {
+    const double *col = A + ((size_t)ti + (size_t)tj * MT) * nb * nb + (size_t)cc * nb;
+    for (int r = lane; r < nb; r += 64) {
+      const double x = fabs(col[r]);
+      if (x > 0.0) {
+        if (x > sc) {
+          sq = 1.0 + sq * (sc / x) * (sc / x);
+          sc = x;
+        } else {
+          sq += (x / sc) * (x / sc);
+        }
+      }
+    }
+  }
+  for (int o = 32; o >= 1; o >>= 1) {
+    const double s2 = __shfl_xor(sc, o, 64), q2 = __shfl_xor(sq, o, 64);
+    const double m = fmax(sc, s2);
+    if (m > 0.0) {
+      const double a = sc / m, b = s2 / m;
+      sq = (sc > 0.0 ? sq * a * a : 0.0) + (s2 > 0.0 ? q2 * b * b : 0.0);
+    }
+    sc = m;
+  }
+  if (lane == 0) {
+    int e = 0;
+    const double N = sc * sqrt(sq);
+    if (N > 0.0 && N <= 1.7e308) (void)frexp(N, &e);
+    CE[c] = N > 0.0 ? 12 - e : 0;
+  }
+}
+
 __global__ void pad_zero_kernel(double *A, int MT, int NT, int nb, int m, int n) {
   const size_t ts = (size_t)nb * nb;
   const size_t total = ts * (size_t)(MT + NT);
@@ -2865,6 +3192,12 @@ hipError_t hq_launch_tbuild_wide_tile(int nb, int ib, const double *V, const dou
 
 hipError_t hq_launch_fill_int(int *p, size_t n, int value, hipStream_t st) {
   hipLaunchKernelGGL(hq::fill_int_kernel, dim3(grid_for(n)), dim3(256), 0, st, p, n, value);
+  return hipGetLastError();
+}
+
+hipError_t hq_launch_colscale(const double *A, int MT, int NT, int nb, int *CE, hipStream_t st) {
+  const int cols = NT * nb;
+  hipLaunchKernelGGL(hq::colscale_kernel, dim3((cols + 3) / 4), dim3(256), 0, st, A, MT, NT, nb, CE);
   return hipGetLastError();
 }
 

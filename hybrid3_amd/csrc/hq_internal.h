@@ -63,6 +63,14 @@ struct hq_geom {
   // VH + (i + j*MT) * hq_vh_tile(nb, ib) halves, sub-block s at + s*64*nb.
   // nullptr otherwise.
   unsigned short *VH;
+  // Split-precision factorisation: per matrix column c (NT nb entries) the
+  // power-of-two exponent sh of the scale 2^sh applied to the column in the
+  // split updates, with 2^sh ||A(:, c)||_2 < 2^12 (the column's 2-norm is
+  // invariant under the orthogonal updates, so it bounds every part of the
+  // column at every step).  Written by a pass over A before the
+  // factorisation; nullptr: each update task finds a scale from its own
+  // columns (two sweeps over them).
+  const int *CE;
 };
 
 // halves of split images per tile: one image of 64 nb halves per 16-reflector

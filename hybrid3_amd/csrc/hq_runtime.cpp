@@ -42,6 +42,7 @@ hipError_t hq_launch_tbuild_wide(const hq_geom &G, const int2 *tiles, int ntiles
 hipError_t hq_launch_tbuild_wide_tile(int nb, int ib, const double *V, const double *tau, double *T, int diag,
                                       hipStream_t st);
 hipError_t hq_launch_pad_zero(double *A, int MT, int NT, int nb, int m, int n, hipStream_t st);
+hipError_t hq_launch_colscale(const double *A, int MT, int NT, int nb, int *CE, hipStream_t st);
 hipError_t hq_fault_take(int *value);
 hipError_t hq_launch_fill_int(int *p, size_t n, int value, hipStream_t st);
 
@@ -372,6 +373,7 @@ struct Context {
   DevBuf flags, counter, T, A, TAU, X, T1;
   DevBuf W, VD;  // W = V T16 and unit lower V of the diagonal tiles (FP64 factorisation)
   DevBuf VH;     // FP16 hi / lo images of the reflector sub-blocks (split factorisation)
+  DevBuf CE;     // per-column scale exponents (split factorisation)
   int epoch = 0;
   // Device-pointer calls share this context's buffers (claim counter,
   // flags, T / W / VD scratch, batch tables).  Each call waits for `busy`
@@ -544,6 +546,8 @@ int reserve_scratch(Context *c, hq_geom &G) {
   if (G.prec == 2) {
     if (c->VH.reserve(2 * hq_vh_tile(nb, ik) * MT * NT) != HQ_OK) return g_err;
     G.VH = (unsigned short *)c->VH.p;
+    if (c->CE.reserve(sizeof(int) * (size_t)NT * nb) != HQ_OK) return g_err;
+    G.CE = (const int *)c->CE.p;
   }
   return HQ_OK;
 }
@@ -565,6 +569,7 @@ int tile_qr_dev_impl(int MT, int NT, int nb, int ib, double *A, double *TAU, dou
   if (order_in(c, st) != HQ_OK) return g_err;
   hq_geom G{A, TAU, T, MT, NT, nb, ik, A, NT, prec};
   if (reserve_scratch(c, G) != HQ_OK) return g_err;
+  if (G.CE) HQ_HIP(hq_launch_colscale(A, MT, NT, nb, (int *)G.CE, st));
   if (serial) {
     std::vector<hq_task> tl;
     host_schedule(MT, NT, nb, ik, c->num_cu, tl);

@@ -40,7 +40,9 @@ __all__ = ["TileQR", "lstsq", "batched_lstsq", "backward_indicator", "gaussian_o
 
 PRECISIONS = {"fp64": 0, "fp32": 1, "split": 2}
 EPS = 2.0 ** -52
-PROGRESS = 0.8   # a CGLS step whose eta and update shrink by less than this made no progress
+PROGRESS = 0.8        # a CGLS step whose eta and update shrink by less than this made no progress
+SMALL_UPDATE = 1e-6   # relative update below which such a step counts toward the stop at once
+TRANSIENT = 8         # steps after which every step without progress counts
 
 
 def _prec(prec, mixed):
@@ -116,7 +118,8 @@ def _cgls(Amul, ATmul, rsolve, rtsolve, b, max_iter, normA, patience=2):
     ||dx|| / ||x|| below PROGRESS times the smallest so far (eta need not
     fall monotonically while x converges); the iteration stops after
     `patience` steps without progress (as LAPACK's refinement stops once the
-    backward error stops falling by a fixed factor), or after max_iter
+    backward error stops falling by a fixed factor) -- counted once the
+    update is below SMALL_UPDATE or after TRANSIENT steps -- or after max_iter
     steps.  Converged means every
     column's best eta is within the rounding floor of its FP64 residual,
     eps (||b|| + ||A||_F ||x||) / ||r||.
@@ -165,7 +168,14 @@ def _cgls(Amul, ATmul, rsolve, rtsolve, b, max_iter, normA, patience=2):
         best_x = torch.where(better, x, best_x)
         best_eta = torch.where(better, eta, best_eta)
         hist.append({"eta": eta.max().item(), "dx_rel": rel.max().item()})
-        since = 0 if bool(progress.any()) else since + 1
+        # a step without progress counts toward the stop once the update is
+        # small (at the rounding floor) or the start-up transient is over: with
+        # factors far less accurate than 1/kappa, eta oscillates over the first
+        # steps while x still moves by O(1)
+        if bool(progress.any()):
+            since = 0
+        elif bool((rel < SMALL_UPDATE).all()) or steps > TRANSIENT:
+            since += 1
         if since >= patience:
             break
     r = b - Amul(best_x)

@@ -78,12 +78,15 @@ def test_split_graded_columns(hq, orc, nb):
     assert colerr.max() < 2e-5, colerr.max()
 
 
-@pytest.mark.parametrize("prec", ["fp32", "split"])
-def test_mixed_large_colnorm(hq, prec):
+@pytest.mark.parametrize("prec,n", [("fp32", 4096), ("split", 4096), ("fp32", 16384), ("split", 16384)])
+def test_mixed_large_colnorm(hq, prec, n):
+    """Whole-chip runs (16384: 85k update tasks in flight on every CU, the
+    pipelined split update's copies two passes deep) keep R's column norms
+    equal to A's to FP32 level."""
     import torch
-    n, nb, ib = 4096, 256, 32
+    nb, ib = 256, 32
     MT = NT = n // nb
-    print(f"mixed {prec} colnorm 4096", flush=True)
+    print(f"mixed {prec} colnorm {n}", flush=True)
     A = torch.empty(MT * NT * nb * nb, dtype=torch.float64, device="cuda")
     hq.fill_uniform_dev(MT, NT, nb, 99, A)
     A0 = A.clone()
