@@ -44,6 +44,15 @@ def parse():
     ap.add_argument("--no-batched-lsq", action="store_true", help="skip the batched least-squares part")
     ap.add_argument("--batched-lsq-count", type=int, default=512)
     ap.add_argument("--no-tall", action="store_true", help="skip the 65536 x 4096 QR / dorgqr / dormqr part")
+    ap.add_argument("--prec", choices=("fp64", "fp32", "split"), default="fp64",
+                    help="precision of the trailing updates of the timed factorisation (the headline is fp64)")
+    ap.add_argument("--no-panels", action="store_true", help="skip the whole-chip panel-kernel measurement")
+    ap.add_argument("--no-library", action="store_true", help="skip the rocSOLVER / host LAPACK reference lines")
+    ap.add_argument("--only-factor", action="store_true",
+                    help="only the timed factorisation and its check (profiling runs)")
+    ap.add_argument("--only-panels", action="store_true", help="only the panel-kernel measurement (profiling runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only a 2 GiB device copy (calibrates FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--traffic-file",
                     default=os.path.join(ROOT, "profiles", "sched_kernel_traffic.txt"))
     return ap.parse_args()
@@ -54,16 +63,102 @@ def note(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def host_threads():
+    """Host cores this process may use: its CPU affinity, bounded by OMP_NUM_THREADS when that is set."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    return min(n, int(omp)) if omp.isdigit() and int(omp) > 0 else n
+
+
 def cpu_baseline(nb, ib, seed):
-    """Oracle tile QR (TileSch-style DAG, OpenMP) on a 4096 x 4096 sample."""
+    """The reference's CPU path restated by the oracle (TileSch-style dynamic
+    DAG over LAPACK-style tile kernels in plain C, OpenMP, every usable host
+    core) on an 8192 x 8192 sample of the benchmark's matrix family; beside
+    it, the host's LAPACK (numpy.linalg.qr, dgeqrf on multithreaded BLAS) on
+    the same sample as a library reference."""
+    import numpy as np
+
     import oracle
-    threads = min(os.cpu_count() or 1, 16)
-    n = 4096
+    threads = host_threads()
+    n = 8192
     t = oracle.time_tile_qr(n // nb, n // nb, nb, ib, seed, threads, 1)
-    return {"value": round(4.0 / 3.0 * n ** 3 / t / 1e12, 5), "unit": "TFLOP/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{n}x{n} FP64 tile QR nb={nb} ib={ib}, oracle C/OpenMP TileSch-style DAG, "
-                      f"{threads} threads, {t:.2f} s"}
+    out = {"value": round(4.0 / 3.0 * n ** 3 / t / 1e12, 5), "unit": "TFLOP/s", "cores": threads,
+           "kind": "port",
+           "sample": f"{n}x{n} FP64 tile QR nb={nb} ib={ib}, oracle C/OpenMP TileSch-style DAG, "
+                     f"{threads} threads, {t:.2f} s"}
+    A = oracle.fill_uniform_cm(n, n, seed)
+    t0 = time.perf_counter()
+    np.linalg.qr(A, mode="r")
+    tl = time.perf_counter() - t0
+    out["host_lapack_dgeqrf"] = {"value": round(4.0 / 3.0 * n ** 3 / tl / 1e12, 5), "unit": "TFLOP/s",
+                                 "sample": f"numpy.linalg.qr(mode='r') of the same {n}x{n} matrix, {tl:.2f} s"}
+    return out
+
+
+def rocsolver_reference(n, seed):
+    """rocSOLVER's dgeqrf (torch.geqrf) on the same n x n FP64 matrix family,
+    resident in HBM, timed with HIP events (best of 3): a library reference
+    for the headline."""
+    import torch
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    A0 = torch.rand(n, n, generator=g, dtype=torch.float64, device="cuda") * 2 - 1
+    best = float("inf")
+    for _ in range(4):
+        A = A0.clone()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        torch.geqrf(A)
+        e1.record()
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    del A0, A
+    torch.cuda.empty_cache()
+    return {"workload": f"torch.geqrf (rocSOLVER dgeqrf) {n}x{n} FP64, uniform[-1,1)", "ms": round(best, 3),
+            "tflops": round(4.0 / 3.0 * n ** 3 / (best * 1e-3) / 1e12, 3)}
+
+
+def panel_bench(nb=256, ib=32, count=2048, reps=3):
+    """The panel kernels at whole-chip load: a batch of `count` single-tile
+    matrices (one GEQRT task each) and one of `count` two-tile columns
+    (GEQRT, then TSQRT of [R; A2]), each factored in one persistent launch.
+    Algorithmic HBM bytes (the tiles read and written once, T and TAU
+    written): GEQRT 2*8*nb^2 + 8*ib*nb + 8*nb, TSQRT 3*8*nb^2 + 8*ib*nb +
+    8*nb (R triangle read and written as a whole tile).  Rates use the mean
+    of `reps` launches (HIP events)."""
+    import torch
+
+    import hybrid3_amd as hq
+    out = {"nb": nb, "ib": ib, "count": count}
+    for name, MT in (("geqrt", 1), ("geqrt_tsqrt", 2)):
+        A = torch.empty(count, MT * nb * nb, dtype=torch.float64, device="cuda")
+        TAU = torch.zeros(count, MT * nb, dtype=torch.float64, device="cuda")
+        pristine = torch.rand(count, MT * nb * nb, dtype=torch.float64, device="cuda") * 2 - 1
+        ptrs = [A[i].data_ptr() for i in range(count)]
+        tps = [TAU[i].data_ptr() for i in range(count)]
+        A.copy_(pristine)
+        hq.batched_tile_qr_dev(nb, ib, [MT] * count, [1] * count, ptrs, tps)   # schedule, code load
+        hq.sync()
+        ms = []
+        for _ in range(reps):
+            A.copy_(pristine)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            hq.batched_tile_qr_dev(nb, ib, [MT] * count, [1] * count, ptrs, tps)
+            e1.record()
+            hq.sync()
+            torch.cuda.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        t = sum(ms) / len(ms) * 1e-3
+        geq = 2 * 8 * nb * nb + 8 * ib * nb + 8 * nb
+        tsq = 3 * 8 * nb * nb + 8 * ib * nb + 8 * nb
+        byts = count * (geq + (tsq if MT == 2 else 0))
+        fl = count * (4.0 / 3.0 * nb ** 3 + (2.0 * nb ** 3 if MT == 2 else 0.0))
+        out[name] = {"ms": round(t * 1e3, 3), "algorithmic_GB_per_s": round(byts / t / 1e9, 1),
+                     "tflops": round(fl / t / 1e12, 3), "bytes": byts}
+        del A, TAU, pristine
+        torch.cuda.empty_cache()
+    return out
 
 
 def batched_bench(count):
@@ -180,9 +275,10 @@ def lsq_bench(m=32768, n=8192, kappas=(1e2, 1e6, 1e10), nb=256, ib=32):
         xt = torch.randn(n, 1, generator=g, dtype=torch.float64, device="cuda")
         b = A @ xt + 1e-8 * torch.randn(m, 1, generator=g, dtype=torch.float64, device="cuda")
         res = {"kappa": kappa}
+        if i == 0:
+            for prec in ("fp64", "fp32", "split"):   # full-size buffers and code paths, untimed
+                lsq.lstsq(A, b, nb=nb, ib=ib, prec=prec, max_iter=1, fallback=False)
         for prec in ("fp64", "fp32", "split"):
-            lsq.lstsq(A[:2048, :512].contiguous().t().contiguous().t(), b[:2048], nb=nb, ib=ib, prec=prec,
-                      max_iter=2)  # warm the schedule caches of this path
             x, info = lsq.lstsq(A, b, nb=nb, ib=ib, prec=prec, max_iter=300)
             res[prec] = {"factor_ms": round(info["factor_seconds"] * 1e3, 2),
                          "solve_ms": round(info["total_seconds"] * 1e3, 2), "steps": info["steps"],
@@ -278,6 +374,10 @@ def batched_lsq_bench(count=512, nb=256, ib=32):
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.prec != "fp64" and not args.only_factor:
+        print(json.dumps({"error": "--prec selects the profiled factorisation: use it with --only-factor "
+                                   "(the headline line is the FP64 factorisation)"}))
+        sys.exit(2)
     if args.gpus != 1 or world != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (no multi-GPU path)"}))
         sys.exit(2)
@@ -285,7 +385,23 @@ def main():
 
     import hybrid3_amd as hq
 
+    if args.copy_calibration:
+        # 2 GiB read + 2 GiB written by one device copy: the reference for
+        # the FETCH_SIZE / WRITE_SIZE scale factors
+        src = torch.ones(1 << 28, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": src.numel() * 8}))
+        return
+
+    if args.only_panels:
+        print(json.dumps(panel_bench(args.nb, args.ib)))
+        return
+
     n, nb, ib = args.n, args.nb, args.ib
+    factor = {"fp64": hq.tile_qr_dev, "fp32": hq.tile_qr_mixed_dev, "split": hq.tile_qr_split_dev}[args.prec]
+    prec_id = {"fp64": 0, "fp32": 1, "split": 2}[args.prec]
     assert n % nb == 0, "n must be a multiple of nb"
     MT = NT = n // nb
     dev = torch.device("cuda:0")
@@ -295,13 +411,13 @@ def main():
     A = torch.empty_like(pristine)
     TAU = torch.zeros(MT * NT * nb, dtype=torch.float64, device=dev)
     T = torch.zeros(MT * NT * ib * nb, dtype=torch.float64, device=dev)
-    ntasks, grid = hq.schedule_info(MT, NT, nb, ib)
+    ntasks, grid = hq.schedule_info(MT, NT, nb, ib, prec_id)
 
     def step(ev=None):
         A.copy_(pristine)
         if ev is not None:
             ev[0].record(stream)
-        hq.tile_qr_dev(MT, NT, nb, ib, A, TAU, T)
+        factor(MT, NT, nb, ib, A, TAU, T)
         if ev is not None:
             ev[1].record(stream)
 
@@ -342,7 +458,7 @@ def main():
         rn = (f * f * upper).sum(dim=(1, 3)).sqrt()
         colnorm_rel = ((an - rn).abs() / an).max().item()
         check = {"finite": R_diag_ok, "max_col_norm_rel_diff": colnorm_rel}
-        if not (R_diag_ok and colnorm_rel < 1e-12):
+        if not (R_diag_ok and colnorm_rel < (1e-12 if args.prec == "fp64" else 1e-4)):
             print(json.dumps({"error": "factorisation check failed", "check": check}))
             sys.exit(3)
 
@@ -358,6 +474,17 @@ def main():
 
     del pristine, A, TAU, T
     torch.cuda.empty_cache()
+    if args.only_factor:
+        print(json.dumps({"prec": args.prec, "n": n, "nb": nb, "ib": ib, "ms_factorization": round(mean_ms, 3),
+                          "tflops": round(tflops, 4), "check": check}))
+        return
+    library = None if args.no_library else rocsolver_reference(n, args.seed)
+    if library is not None:
+        note(f"rocSOLVER dgeqrf {n}x{n}: {library['ms']} ms, {library['tflops']} TFLOP/s")
+    panels = None if args.no_panels else panel_bench(nb, ib)
+    if panels is not None:
+        note(f"panels: GEQRT {panels['geqrt']['algorithmic_GB_per_s']} GB/s, "
+             f"GEQRT+TSQRT {panels['geqrt_tsqrt']['algorithmic_GB_per_s']} GB/s")
     batched = None if args.no_batched else batched_bench(args.batch_count)
     if batched is not None:
         note(f"batched: {batched['qrs_per_s']} QRs/s")
@@ -413,6 +540,8 @@ def main():
                      "kernel": "hq::sched_kernel<256,2> (whole factorisation, one launch)",
                      "algorithmic_flops_per_launch": flops},
         "cpu_baseline": cpu,
+        "library_reference": library,
+        "panels": panels,
         "check": check,
         "batched": batched,
         "tall": tall,
