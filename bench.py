@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven-QR measurement")
     ap.add_argument("--batch-count", type=int, default=4000)
+    ap.add_argument("--retune", action="store_true",
+                    help="measure the batched plan (autotuner) instead of reading the committed tuning table")
     ap.add_argument("--no-lsq", action="store_true", help="skip the mixed-precision least-squares part")
     ap.add_argument("--no-batched-lsq", action="store_true", help="skip the batched least-squares part")
     ap.add_argument("--batched-lsq-count", type=int, default=512)
@@ -161,21 +163,43 @@ def panel_bench(nb=256, ib=32, count=2048, reps=3):
     return out
 
 
-def batched_bench(count):
-    """Batched uneven QR: `count` FP64 matrices, m ~ U[64,1024], n ~ U[32,m]
-    (seed 4000), autotuned (nb, ib) plan, one persistent launch per (nb, ib)
-    group; against one hq_tile_qr_dev launch per matrix on the same data."""
+def batched_bench(count, retune=False):
+    """Batched uneven QR: `count` matrices, m ~ U[64,1024], n ~ U[32,m]
+    (seed 4000), (nb, ib, panel mapping) plan from the committed tuning table
+    (or measured by the autotuner), one persistent launch per group; against
+    one hq_tile_qr_dev launch per matrix on the same data.  The same sizes as
+    FP32 matrices (widened, factored in FP64, rounded back; the layout
+    launches inside the timed region).  Per-matrix ||A - QR|| / ||A|| and
+    ||I - Q^T Q|| on every 20th matrix."""
+    import os
+
     from hybrid3_amd import batched as B
     sizes = B.uneven_sizes(count, seed=4000)
-    plan, report = B.autotune(sizes)
+    if retune or not os.path.exists(B.TABLE):
+        plan, report = B.autotune(sizes)
+        source = "autotuner (measured in this run)"
+    else:
+        table = B.load_table()
+        plan, report = B.plan_from_table(sizes, table), table["measured"]
+        source = "tuning table " + os.path.relpath(B.TABLE, os.path.dirname(os.path.abspath(__file__)))
     bt = B.Batch(sizes, plan, seed0=1)
+    B.time_factor(bt, reps=1)      # schedules built and cached
     t = B.time_factor(bt, reps=3)
     err = bt.colnorm_error()
+    sample = list(range(0, count, 20))
+    res = bt.residuals(sample)
     tg = B.time_factor(bt, reps=3, graph=bt.capture())   # the same runs replayed from a hipGraph
     err = max(err, bt.colnorm_error())
     bt.release()
     tp = B.time_factor(bt, reps=1, per_matrix=True)
     fl = B.flops(sizes)
+    del bt
+    b32 = B.Batch(sizes, plan, seed0=1, fp32=True)
+    B.time_factor(b32, reps=1)
+    t32 = B.time_factor(b32, reps=3)
+    err32 = b32.colnorm_error()
+    res32 = b32.residuals(sample)
+    del b32
     return {
         "workload": f"{count} FP64 matrices, m uniform in [64,1024], n uniform in [32,m], "
                     "uniform[-1,1) entries, seed 4000; zero-padded tile layout",
@@ -185,10 +209,24 @@ def batched_bench(count):
         "graph_replay_qrs_per_s": round(count / tg, 1),
         "per_matrix_launch_qrs_per_s": round(count / tp, 1),
         "speedup_vs_per_matrix_launch": round(tp / t, 2),
-        "launches": len(bt.groups),
+        "launches": len({B._key(p) for p in plan}),
+        "plan_source": source,
         "plan": report["chosen"],
         "tuner": report,
         "max_col_norm_rel_diff": err,
+        "checked_matrices": len(sample),
+        "max_residual": max(r for r, _ in res),
+        "max_orthogonality": max(o for _, o in res),
+        "fp32": {
+            "workload": f"the same {count} sizes as FP32 column-major matrices (the FP64 values rounded), "
+                        "widened to the FP64 tile layout, factored in FP64, factor rounded back to FP32",
+            "qrs_per_s": round(count / t32, 1),
+            "ms": round(t32 * 1e3, 3),
+            "tflops_algorithmic": round(fl / t32 / 1e12, 3),
+            "max_col_norm_rel_diff": err32,
+            "max_residual": max(r for r, _ in res32),
+            "max_orthogonality": max(o for _, o in res32),
+        },
     }
 
 
@@ -485,10 +523,14 @@ def main():
     if panels is not None:
         note(f"panels: GEQRT {panels['geqrt']['algorithmic_GB_per_s']} GB/s, "
              f"GEQRT+TSQRT {panels['geqrt_tsqrt']['algorithmic_GB_per_s']} GB/s")
-    batched = None if args.no_batched else batched_bench(args.batch_count)
+    batched = None if args.no_batched else batched_bench(args.batch_count, args.retune)
     if batched is not None:
-        note(f"batched: {batched['qrs_per_s']} QRs/s")
-    if batched is not None and not batched["max_col_norm_rel_diff"] < 1e-12:
+        note(f"batched: {batched['qrs_per_s']} QRs/s, FP32 matrices {batched['fp32']['qrs_per_s']} QRs/s")
+    if batched is not None and not (batched["max_col_norm_rel_diff"] < 1e-12 and batched["max_residual"] < 1e-12
+                                    and batched["max_orthogonality"] < 1e-12
+                                    and batched["fp32"]["max_col_norm_rel_diff"] < 1e-6
+                                    and batched["fp32"]["max_residual"] < 1e-5
+                                    and batched["fp32"]["max_orthogonality"] < 1e-5):
         print(json.dumps({"error": "batched factorisation check failed", "batched": batched}))
         sys.exit(3)
 

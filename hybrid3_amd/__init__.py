@@ -94,6 +94,11 @@ SIGNATURES = {
     "hq_batched_tile_qr_prec_dev": (C.c_int, [C.c_int] * 4 + [C.POINTER(C.c_int)] * 2 + [C.POINTER(_vp)] * 3
                                     + [_vp]),
     "hq_batched_prepare_dev": (C.c_int, [C.c_int] * 4 + [C.POINTER(C.c_int)] * 2),
+    "hq_batched_tile_qr_ex_dev": (C.c_int, [C.c_int] * 5 + [C.POINTER(C.c_int)] * 2 + [C.POINTER(_vp)] * 3
+                                  + [_vp]),
+    "hq_batched_prepare_ex_dev": (C.c_int, [C.c_int] * 5 + [C.POINTER(C.c_int)] * 2),
+    "hq_batched_plan_ex_dev": (C.c_int, [C.c_int] * 4 + [C.POINTER(C.c_int)] * 2 + [C.POINTER(_vp)] * 3
+                               + [C.POINTER(C.c_int)]),
     "hq_cm32_to_tile_dev": (C.c_int, [C.c_int] * 2 + [C.POINTER(C.c_int)] * 2
                             + [C.POINTER(_vp), C.POINTER(C.c_int), C.POINTER(_vp), _vp]),
     "hq_tile_to_cm32_dev": (C.c_int, [C.c_int] * 2 + [C.POINTER(C.c_int)] * 2
@@ -240,9 +245,18 @@ def batched_tile_qr_prec_dev(prec, nb, ib, MT, NT, A, TAU, T=None, stream=None):
                                             _stream(stream)))
 
 
-def batched_prepare(prec, nb, ib, MT, NT):
-    """Build the schedule of a batch ahead of its first batched_tile_qr_prec_dev call."""
-    check(lib().hq_batched_prepare_dev(prec, len(MT), nb, ib, _iarr(MT), _iarr(NT)))
+def batched_tile_qr_ex_dev(prec, split_panels, nb, ib, MT, NT, A, TAU, T=None, stream=None):
+    """batched_tile_qr_prec_dev with the panel mapping: 0 one task per tile
+    panel QR, 1 one task per ib block plus per-slab trailing tasks."""
+    n = len(MT)
+    Tp = None if T is None else _parr(T)
+    check(lib().hq_batched_tile_qr_ex_dev(prec, split_panels, n, nb, ib, _iarr(MT), _iarr(NT), _parr(A),
+                                          _parr(TAU), Tp, _stream(stream)))
+
+
+def batched_prepare(prec, nb, ib, MT, NT, split_panels=0):
+    """Build the schedule of a batch ahead of its first batched factorisation call."""
+    check(lib().hq_batched_prepare_ex_dev(prec, split_panels, len(MT), nb, ib, _iarr(MT), _iarr(NT)))
 
 
 def cm32_to_tile_dev(nb, m, n, Cs, ldc, A, stream=None):
@@ -255,12 +269,13 @@ def tile_to_cm32_dev(nb, m, n, A, Cs, ldc, stream=None):
                                     _stream(stream)))
 
 
-def batched_plan_dev(nb, ib, MT, NT, A, TAU, T=None):
+def batched_plan_dev(nb, ib, MT, NT, A, TAU, T=None, split_panels=0):
     """Prepare a batch for repeated (graph-capturable) runs; returns the plan id."""
     n = len(MT)
     Tp = None if T is None else _parr(T)
     plan = C.c_int(0)
-    check(lib().hq_batched_plan_dev(n, nb, ib, _iarr(MT), _iarr(NT), _parr(A), _parr(TAU), Tp, C.byref(plan)))
+    check(lib().hq_batched_plan_ex_dev(split_panels, n, nb, ib, _iarr(MT), _iarr(NT), _parr(A), _parr(TAU), Tp,
+                                       C.byref(plan)))
     return plan.value
 
 

@@ -658,13 +658,17 @@ int apply_qt_impl(int MT, int NT, int nb, int ib, const double *RY, const double
 // matrix's DAG (the single-matrix DAG, canonical order) is concatenated with
 // its matrix index and the whole batch gets one claim order, so large
 // matrices' critical paths start first and small ones fill the gaps.
+// Panel mapping: split_panels 0 runs each GEQRT / TSQRT as one task per tile
+// (one workgroup owns the tile's whole panel QR), 1 as one task per ib block
+// plus one trailing task per (block, 64-column slab), as for one matrix.
 int get_batch_schedule(Context *c, int count, int nb, int ib, const int *MT, const int *NT,
-                       CachedSched **out, int prec = 0) {
+                       CachedSched **out, int prec = 0, int split_panels = 0) {
   std::vector<int> key;
-  key.reserve(3 + 2 * (size_t)count);
+  key.reserve(4 + 2 * (size_t)count);
   key.push_back(nb);
   key.push_back(ib);
   key.push_back(prec);
+  key.push_back(split_panels);
   for (int i = 0; i < count; i++) {
     key.push_back(MT[i]);
     key.push_back(NT[i]);
@@ -679,9 +683,7 @@ int get_batch_schedule(Context *c, int count, int nb, int ib, const int *MT, con
   for (int i = 0; i < count; i++) {
     auto &dag = per_geom[std::make_pair(MT[i], NT[i])];
     if (dag.empty()) {
-      // many independent matrices keep the machine busy without pipelining
-      // each tile column's panel chain: one task per panel QR
-      DagBuilder b(MT[i], NT[i], nb, ib, false, 0, false, prec);
+      DagBuilder b(MT[i], NT[i], nb, ib, false, 0, split_panels != 0, prec);
       b.build();
       dag = b.t;
     }
@@ -763,9 +765,10 @@ int batch_geoms(int count, int nb, int ib, const int *MT, const int *NT, double 
 }
 
 int batched_qr_impl(int count, int nb, int ib, const int *MT, const int *NT, double *const *A,
-                    double *const *TAU, double *const *T, void *stream, int prec = 0) {
+                    double *const *TAU, double *const *T, void *stream, int prec = 0, int split_panels = 0) {
   if (batch_args_ok(count, nb, ib, MT, NT, A, TAU) != HQ_OK) return g_err;
   if (prec < 0 || prec > 2) return set_error(HQ_ERR_ARG, "prec=%d (0 FP64, 1 FP32, 2 split)", prec);
+  if (split_panels < 0 || split_panels > 1) return set_error(HQ_ERR_ARG, "panel mapping %d (0 or 1)", split_panels);
   if (prec && nb < 128) return set_error(HQ_ERR_ARG, "reduced-precision updates need nb >= 128 (nb=%d)", nb);
   Context *c;
   if (get_ctx(&c) != HQ_OK) return g_err;
@@ -775,7 +778,7 @@ int batched_qr_impl(int count, int nb, int ib, const int *MT, const int *NT, dou
   if (batch_geoms(count, nb, ib, MT, NT, A, TAU, T, c->Tbatch, c->Wbatch, c->VDbatch, g, prec, &c->VHbatch) != HQ_OK)
     return g_err;
   CachedSched *cs;
-  if (get_batch_schedule(c, count, nb, ib, MT, NT, &cs, prec) != HQ_OK) return g_err;
+  if (get_batch_schedule(c, count, nb, ib, MT, NT, &cs, prec, split_panels) != HQ_OK) return g_err;
   if (c->geoms.reserve(sizeof(hq_geom) * (size_t)count) != HQ_OK) return g_err;
   HQ_HIP(hipMemcpyAsync(c->geoms.p, g.data(), sizeof(hq_geom) * (size_t)count, hipMemcpyHostToDevice, st));
   c->epoch++;
@@ -799,15 +802,16 @@ struct BatchPlan {
 };
 
 int batched_plan_impl(int count, int nb, int ib, const int *MT, const int *NT, double *const *A,
-                      double *const *TAU, double *const *T, int *plan) {
+                      double *const *TAU, double *const *T, int *plan, int split_panels = 0) {
   if (!plan) return set_error(HQ_ERR_ARG, "plan must not be NULL");
+  if (split_panels < 0 || split_panels > 1) return set_error(HQ_ERR_ARG, "panel mapping %d (0 or 1)", split_panels);
   if (batch_args_ok(count, nb, ib, MT, NT, A, TAU) != HQ_OK) return g_err;
   Context *c;
   if (get_ctx(&c) != HQ_OK) return g_err;
   std::unique_ptr<BatchPlan> p(new BatchPlan());
   std::vector<hq_geom> g;
   if (batch_geoms(count, nb, ib, MT, NT, A, TAU, T, p->T, p->W, p->VD, g) != HQ_OK) return g_err;
-  if (get_batch_schedule(c, count, nb, ib, MT, NT, &p->cs) != HQ_OK) return g_err;
+  if (get_batch_schedule(c, count, nb, ib, MT, NT, &p->cs, 0, split_panels) != HQ_OK) return g_err;
   if (p->geoms.reserve(sizeof(hq_geom) * (size_t)count) != HQ_OK ||
       p->flags.reserve(sizeof(int) * (size_t)p->cs->ntasks) != HQ_OK || p->counter.reserve(64) != HQ_OK)
     return g_err;
@@ -1268,6 +1272,12 @@ int hq_batched_tile_qr_dev(int count, int nb, int ib, const int *MT, const int *
 }
 
 int hq_batched_prepare_dev(int prec, int count, int nb, int ib, const int *MT, const int *NT) {
+  return hq_batched_prepare_ex_dev(prec, 0, count, nb, ib, MT, NT);
+}
+
+int hq_batched_prepare_ex_dev(int prec, int split_panels, int count, int nb, int ib, const int *MT,
+                              const int *NT) {
+  if (split_panels < 0 || split_panels > 1) return set_error(HQ_ERR_ARG, "panel mapping %d (0 or 1)", split_panels);
   if (count < 1 || count > 65535) return set_error(HQ_ERR_ARG, "batch of %d matrices (1..65535)", count);
   if (!MT || !NT) return set_error(HQ_ERR_ARG, "MT and NT arrays are required");
   if (ib > HQ_IBMAX) return set_error(HQ_ERR_ARG, "batches take ib <= %d (ib=%d)", HQ_IBMAX, ib);
@@ -1278,7 +1288,7 @@ int hq_batched_prepare_dev(int prec, int count, int nb, int ib, const int *MT, c
   Context *c;
   if (get_ctx(&c) != HQ_OK) return g_err;
   CachedSched *cs;
-  if (get_batch_schedule(c, count, nb, ib, MT, NT, &cs, prec) != HQ_OK) return g_err;
+  if (get_batch_schedule(c, count, nb, ib, MT, NT, &cs, prec, split_panels) != HQ_OK) return g_err;
   return ok();
 }
 
@@ -1287,9 +1297,20 @@ int hq_batched_tile_qr_prec_dev(int prec, int count, int nb, int ib, const int *
   return batched_qr_impl(count, nb, ib, MT, NT, A, TAU, T, stream, prec);
 }
 
+int hq_batched_tile_qr_ex_dev(int prec, int split_panels, int count, int nb, int ib, const int *MT,
+                              const int *NT, double *const *A, double *const *TAU, double *const *T,
+                              void *stream) {
+  return batched_qr_impl(count, nb, ib, MT, NT, A, TAU, T, stream, prec, split_panels);
+}
+
 int hq_batched_plan_dev(int count, int nb, int ib, const int *MT, const int *NT, double *const *A,
                         double *const *TAU, double *const *T, int *plan) {
   return batched_plan_impl(count, nb, ib, MT, NT, A, TAU, T, plan);
+}
+
+int hq_batched_plan_ex_dev(int split_panels, int count, int nb, int ib, const int *MT, const int *NT,
+                           double *const *A, double *const *TAU, double *const *T, int *plan) {
+  return batched_plan_impl(count, nb, ib, MT, NT, A, TAU, T, plan, split_panels);
 }
 
 int hq_batched_run_dev(int plan, void *stream) {

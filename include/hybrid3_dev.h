@@ -224,9 +224,24 @@ int hq_batched_tile_qr_dev(int count, int nb, int ib, const int *MT, const int *
 int hq_batched_tile_qr_prec_dev(int prec, int count, int nb, int ib, const int *MT, const int *NT,
                                 double *const *A, double *const *TAU, double *const *T, void *stream);
 
+/* hq_batched_tile_qr_prec_dev with a choice of panel mapping (no counterpart
+ * in the reference; the batch autotuner's wave-to-task dimension):
+ * split_panels 0 runs each GEQRT / TSQRT of a tile as one task (one
+ * workgroup factors the whole nb-column panel and applies it to its own
+ * tile), 1 as one task per ib block plus one trailing task per (block,
+ * 64-column slab), spreading a panel over several workgroups as the
+ * single-matrix entry points do.  Same results either way (same reflectors,
+ * same update order per tile). */
+int hq_batched_tile_qr_ex_dev(int prec, int split_panels, int count, int nb, int ib, const int *MT,
+                              const int *NT, double *const *A, double *const *TAU, double *const *T,
+                              void *stream);
+
 /* Builds and caches the schedule of a batch (the shapes, nb, ib and update
- * precision of a later hq_batched_tile_qr_prec_dev call) ahead of it. */
+ * precision of a later hq_batched_tile_qr_prec_dev call) ahead of it;
+ * _ex: also the panel mapping of a later hq_batched_tile_qr_ex_dev call. */
 int hq_batched_prepare_dev(int prec, int count, int nb, int ib, const int *MT, const int *NT);
+int hq_batched_prepare_ex_dev(int prec, int split_panels, int count, int nb, int ib, const int *MT,
+                              const int *NT);
 
 /* The same factorisation as a plan prepared once and run many times (no
  * counterpart in the reference; for callers that capture the launch in a
@@ -243,6 +258,9 @@ int hq_batched_prepare_dev(int prec, int count, int nb, int ib, const int *MT, c
 int hq_batched_plan_dev(int count, int nb, int ib, const int *MT, const int *NT,
                         double *const *A, double *const *TAU, double *const *T,
                         int *plan);
+/* hq_batched_plan_dev with the panel mapping of hq_batched_tile_qr_ex_dev. */
+int hq_batched_plan_ex_dev(int split_panels, int count, int nb, int ib, const int *MT, const int *NT,
+                           double *const *A, double *const *TAU, double *const *T, int *plan);
 int hq_batched_run_dev(int plan, void *stream);
 int hq_batched_plan_check(int plan, void *stream);
 int hq_batched_plan_free(int plan);

@@ -65,6 +65,78 @@ def test_batched_matches_oracle_and_single(hq, orc, nb, ib):
         assert torch.equal(a1, A[i]) and torch.equal(t1, TAU[i]), (m, n)
 
 
+@pytest.mark.parametrize("nb,ib", [(64, 16), (128, 32), (256, 32)])
+def test_split_panel_mapping_matches_oracle(hq, orc, nb, ib):
+    """Panel mapping 1 (one task per ib block plus per-slab trailing tasks)
+    gives the oracle's factor as mapping 0 does."""
+    import torch
+    print(f"batched split panels nb={nb} ib={ib}", flush=True)
+    sizes = SIZES + [(600, 300), (513, 513)]
+    tiles, A = make_batch(hq, sizes, nb, 930)
+    A0 = [a.clone() for a in A]
+    TAU = [torch.zeros(MT * NT * nb, dtype=torch.float64, device="cuda") for MT, NT in tiles]
+    hq.batched_tile_qr_ex_dev(0, 1, nb, ib, [t[0] for t in tiles], [t[1] for t in tiles], A, TAU)
+    hq.sync()
+    A1 = [a.clone() for a in A0]
+    TAU1 = [torch.zeros_like(t) for t in TAU]
+    hq.batched_tile_qr_ex_dev(0, 0, nb, ib, [t[0] for t in tiles], [t[1] for t in tiles], A1, TAU1)
+    hq.sync()
+    same = all(torch.equal(a, b) for a, b in zip(A, A1))
+    print(f"  bitwise equal to whole-tile panels: {same}", flush=True)
+    for i, ((m, n), (MT, NT)) in enumerate(zip(sizes, tiles)):
+        if m * n > 300 * 300:
+            continue   # the oracle's time; the column-norm test below covers large ones
+        RYo, TAUo = orc.tile_qr(A0[i].cpu().numpy(), MT, NT, nb, ib)
+        assert np.abs(A[i].cpu().numpy() - RYo).max() / np.abs(RYo).max() < 1e-12, (m, n)
+        assert np.abs(TAU[i].cpu().numpy() - TAUo).max() < 1e-12, (m, n)
+    for a, b in zip(A, A1):
+        assert (a - b).abs().max().item() <= 1e-12 * b.abs().max().item()
+
+
+def test_fp32_batch_is_the_rounded_fp64_factor(hq, orc):
+    """FP32 matrices: widened to the FP64 tile layout, factored, rounded back.
+    The stored FP32 factor is the oracle's FP64 factor of the FP32 inputs
+    rounded to FP32 (to one unit in the last place, the FP64 results agree
+    to 1e-12), and the FP32 factor is a QR of the input to FP32 accuracy."""
+    import torch
+    from hybrid3_amd import batched as B
+    sizes = SIZES + [(64, 33), (191, 64)]
+    plan = [(64, 16, i % 2) if i % 3 else (128, 32, i % 2) for i in range(len(sizes))]
+    print("batched FP32 matrices", flush=True)
+    bt = B.Batch(sizes, plan, seed0=960, fp32=True)
+    inputs = bt.pristine.cpu().numpy()
+    bt.factor()
+    sync_out = bt.C32.cpu().numpy()
+    for i, ((m, n), (MT, NT), (nb, ib, _)) in enumerate(zip(bt.sizes, bt.tiles, bt.plan)):
+        lo = int(bt.c_off[i])
+        a32 = inputs[lo:lo + m * n].reshape(n, m).T
+        # the FP32 inputs are the generator's values rounded
+        assert np.array_equal(a32, orc.fill_uniform_cm(m, n, 960 + i).astype(np.float32)), (m, n)
+        cm = np.zeros((MT * nb, NT * nb), order="F")
+        cm[:m, :n] = a32.astype(np.float64)
+        RYo, _ = orc.tile_qr(orc.cm_to_tile(cm, nb), MT, NT, nb, ib)
+        want = orc.tile_to_cm(RYo, MT, NT, nb)[:m, :n]
+        got = sync_out[lo:lo + m * n].reshape(n, m).T.astype(np.float64)
+        ulp = np.spacing(np.abs(want).astype(np.float32)).astype(np.float64)
+        assert np.all(np.abs(got - want) <= ulp + 1e-12 * np.abs(want).max()), (m, n)
+    res = bt.residuals(range(len(sizes)))
+    assert max(r for r, _ in res) < 1e-5 and max(o for _, o in res) < 1e-5, res
+    assert bt.colnorm_error() < 1e-6
+
+
+def test_batch_residual_and_orthogonality(hq):
+    """Per-matrix ||A - QR|| / ||A|| and ||I - Q^T Q|| of an FP64 batch
+    (dormqr / dorgqr of the engine on the batch's reflectors)."""
+    from hybrid3_amd import batched as B
+    sizes = B.uneven_sizes(40, seed=12, m_range=(64, 900))
+    plan = [(64, 16, 0) if m < 256 else (128, 32, 1) for m, _ in sizes]
+    print("batched residuals", flush=True)
+    bt = B.Batch(sizes, plan, seed0=40)
+    bt.factor()
+    res = bt.residuals(range(len(sizes)))
+    assert max(r for r, _ in res) < 1e-13 and max(o for _, o in res) < 1e-12, res
+
+
 def test_padding_is_exact_vs_lapack(hq, orc):
     import torch
     nb, ib = 64, 16
@@ -152,6 +224,9 @@ def test_autotuner_plan_and_batch(hq):
     bt = B.Batch(sizes, plan, seed0=3)
     B.time_factor(bt, reps=1)
     assert bt.colnorm_error() < 1e-12
+    # the JSON table of the report gives back the same plan
+    table = B.table_from_report(report)
+    assert B.plan_from_table(sizes, table) == [B._key(p) for p in plan]
 
 
 def test_plan_runs_and_graph_replays(hq):
