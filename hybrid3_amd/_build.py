@@ -11,7 +11,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libhybrid3.so")
 MAIN = os.path.join(HERE, "hybrid3")  # `hybrid3 N NB [IB]`, the reference's main.c
-SOURCES = ["hq_factor.hip", "hq_homegrown.hip", "hq_runtime.cpp", "hq_harness.cpp"]
+SOURCES = ["hq_factor.hip", "hq_generic.hip", "hq_homegrown.hip", "hq_runtime.cpp", "hq_harness.cpp"]
 ARCH = os.environ.get("HQ_OFFLOAD_ARCH", "gfx950")
 
 # Packed FP32 ops are switched off for every device compile (the hardware

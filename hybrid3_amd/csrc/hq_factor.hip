@@ -3075,10 +3075,28 @@ static hipError_t hq_dispatch(int nb, int ib, Fn &&fn) {
   constexpr int NB = decltype(nbc)::value;      \
   constexpr int WB = decltype(wbc)::value;
 
+// Tile sizes without a specialised kernel set run the generic kernels
+// (hq_generic.hip).
+static bool hq_tuned_nb(int nb) { return nb == 64 || nb == 128 || nb == 256; }
+hipError_t hqg_launch_sched(const hq_geom &G, const hq_geom *geoms, const hq_task *tasks, int ntasks, int *flags,
+                            int epoch, int *counter, int grid, long long *trace, hipStream_t st, int *epoch_dev,
+                            unsigned wait_limit);
+hipError_t hqg_launch_task(const hq_geom &G, const hq_task &tk, hipStream_t st);
+hipError_t hqg_launch_geqrt(int nb, int ib, double *A, double *T, double *TAU, hipStream_t st);
+hipError_t hqg_launch_tsqrt(int nb, int ib, double *A1, double *A2, double *T, double *TAU, hipStream_t st);
+hipError_t hqg_launch_larfb(int nb, int ib, const double *V, const double *T, double *C, hipStream_t st);
+hipError_t hqg_launch_ssrfb(int nb, int ib, double *A1, double *A2, const double *V, const double *T,
+                            hipStream_t st);
+hipError_t hqg_launch_tbuild(const hq_geom &G, const int2 *tiles, int ntiles, int drop_last, hipStream_t st);
+hipError_t hqg_launch_tbuild_tile(int nb, int ib, const double *V, const double *tau, double *T, int diag,
+                                  hipStream_t st);
+
 hipError_t hq_launch_sched(const hq_geom &G, const hq_geom *geoms, const hq_task *tasks, int ntasks,
                            int *flags, int epoch, int *counter, int grid, long long *trace, hipStream_t st,
                            int *epoch_dev, unsigned wait_limit) {
   hipLaunchKernelGGL(hq::reset_counter_kernel, dim3(1), dim3(64), 0, st, counter, epoch_dev);
+  if (!hq_tuned_nb(G.nb))
+    return hqg_launch_sched(G, geoms, tasks, ntasks, flags, epoch, counter, grid, trace, st, epoch_dev, wait_limit);
   return hq_dispatch(G.nb, G.ib, [&](auto nbc, auto wbc) {
     HQ_NBWB
     if constexpr (NB >= 128) {
@@ -3121,6 +3139,7 @@ hipError_t hq_launch_batched_layout(int op, const hq_bmat *mats, int count, int 
 }
 
 hipError_t hq_launch_task(const hq_geom &G, const hq_task &tk, hipStream_t st) {
+  if (!hq_tuned_nb(G.nb)) return hqg_launch_task(G, tk, st);
   return hq_dispatch(G.nb, G.ib, [&](auto nbc, auto wbc) {
     HQ_NBWB
     hipLaunchKernelGGL((hq::task_kernel<NB, WB>), dim3(1), dim3(HQ_THREADS), 0, st, G, tk);
@@ -3128,6 +3147,7 @@ hipError_t hq_launch_task(const hq_geom &G, const hq_task &tk, hipStream_t st) {
 }
 
 hipError_t hq_launch_geqrt(int nb, int ib, double *A, double *T, double *TAU, hipStream_t st) {
+  if (!hq_tuned_nb(nb)) return hqg_launch_geqrt(nb, ib, A, T, TAU, st);
   return hq_dispatch(nb, ib, [&](auto nbc, auto wbc) {
     HQ_NBWB
     hipLaunchKernelGGL((hq::geqrt_kernel<NB, WB>), dim3(1), dim3(HQ_THREADS), 0, st, A, T, TAU, ib);
@@ -3135,6 +3155,7 @@ hipError_t hq_launch_geqrt(int nb, int ib, double *A, double *T, double *TAU, hi
 }
 hipError_t hq_launch_tsqrt(int nb, int ib, double *A1, double *A2, double *T, double *TAU,
                            hipStream_t st) {
+  if (!hq_tuned_nb(nb)) return hqg_launch_tsqrt(nb, ib, A1, A2, T, TAU, st);
   return hq_dispatch(nb, ib, [&](auto nbc, auto wbc) {
     HQ_NBWB
     hipLaunchKernelGGL((hq::tsqrt_kernel<NB, WB>), dim3(1), dim3(HQ_THREADS), 0, st, A1, A2, T, TAU,
@@ -3143,6 +3164,7 @@ hipError_t hq_launch_tsqrt(int nb, int ib, double *A1, double *A2, double *T, do
 }
 hipError_t hq_launch_larfb(int nb, int ib, const double *V, const double *T, double *C,
                            hipStream_t st) {
+  if (!hq_tuned_nb(nb)) return hqg_launch_larfb(nb, ib, V, T, C, st);
   return hq_dispatch(nb, ib, [&](auto nbc, auto wbc) {
     HQ_NBWB
     hipLaunchKernelGGL((hq::larfb_kernel<NB, WB>), dim3((NB + HQ_SLAB - 1) / HQ_SLAB),
@@ -3151,6 +3173,7 @@ hipError_t hq_launch_larfb(int nb, int ib, const double *V, const double *T, dou
 }
 hipError_t hq_launch_ssrfb(int nb, int ib, double *A1, double *A2, const double *V, const double *T,
                            hipStream_t st) {
+  if (!hq_tuned_nb(nb)) return hqg_launch_ssrfb(nb, ib, A1, A2, V, T, st);
   return hq_dispatch(nb, ib, [&](auto nbc, auto wbc) {
     HQ_NBWB
     hipLaunchKernelGGL((hq::ssrfb_kernel<NB, WB>), dim3((NB + HQ_SLAB - 1) / HQ_SLAB),
@@ -3159,6 +3182,7 @@ hipError_t hq_launch_ssrfb(int nb, int ib, double *A1, double *A2, const double 
 }
 
 hipError_t hq_launch_tbuild(const hq_geom &G, const int2 *tiles, int ntiles, hipStream_t st) {
+  if (!hq_tuned_nb(G.nb)) return hqg_launch_tbuild(G, tiles, ntiles, 1, st);
   return hq_dispatch(G.nb, G.ib, [&](auto nbc, auto wbc) {
     HQ_NBWB
     hipLaunchKernelGGL((hq::tbuild_kernel<NB, WB>), dim3((NB + G.ib - 1) / G.ib, ntiles),
@@ -3172,6 +3196,7 @@ static unsigned grid_for(size_t total) {
 }
 
 hipError_t hq_launch_tbuild_wide(const hq_geom &G, const int2 *tiles, int ntiles, int drop_last, hipStream_t st) {
+  if (!hq_tuned_nb(G.nb)) return hqg_launch_tbuild(G, tiles, ntiles, drop_last, st);
   return hq_dispatch(G.nb, G.ib, [&](auto nbc, auto wbc) {
     HQ_NBWB
     (void)WB;
@@ -3182,6 +3207,7 @@ hipError_t hq_launch_tbuild_wide(const hq_geom &G, const int2 *tiles, int ntiles
 
 hipError_t hq_launch_tbuild_wide_tile(int nb, int ib, const double *V, const double *tau, double *T, int diag,
                                       hipStream_t st) {
+  if (!hq_tuned_nb(nb)) return hqg_launch_tbuild_tile(nb, ib, V, tau, T, diag, st);
   return hq_dispatch(nb, ib, [&](auto nbc, auto wbc) {
     HQ_NBWB
     (void)WB;

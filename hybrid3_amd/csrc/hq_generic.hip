@@ -1,0 +1,623 @@
+// hq_generic.hip -- tile kernels and DAG scheduler for tile sizes without a
+// register-resident specialisation: any nb up to 512 other than 64, 128 and
+// 256 (e.g. nb = 384 or 512, or an nb that is not a multiple of 16).
+//
+// The task set, the DAG and the results are those of the specialised kernels
+// (flat-tree GEQRT / TSQRT / LARFB / SSRFB, PLASMA's T layout with ld = ib,
+// Hybrid_tile_QR.c:960-1363); only the data movement differs.  At these sizes
+// the nb x 16 strip of a 64-column slab does not fit a wave's registers next
+// to its operands (512 rows x 16 columns is 128 doubles per lane), so:
+//   * a panel block (<= 32 columns of <= 512 rows) is factored in LDS: one
+//     pass per column computes the column's squared norm and its dot products
+//     with every column right of it at once (8 row groups per column), and one
+//     pass applies the reflector;
+//   * a block reflector is applied to a slab of <= 64 columns by two sweeps
+//     over 64-row chunks: W = V^T X (X chunk and V chunk staged in LDS, each
+//     thread 8 reflectors of one column), then W2 = T^T W in LDS, then
+//     X -= V W2 (each thread one row and 16 columns, its row of V in
+//     registers).  The slab makes three passes over L2 per ib block instead
+//     of staying in registers, so these sizes run below the specialised ones.
+// FP64 everywhere (reduced-precision updates exist for nb = 128 and 256).
+#include <hip/hip_runtime.h>
+
+#include "hq_internal.h"
+
+namespace hqg {
+
+constexpr int NBMAX = 512;
+constexpr int IBM = HQ_IBMAX;  // reflectors per block (the run's ib <= 32)
+constexpr int LDP = NBMAX + 1; // panel column stride
+constexpr int LDT = IBM + 1;   // T / Gram / triangle column stride
+constexpr int CH = 64;         // rows per chunk of a block application
+constexpr int LDX = CH + 1;    // X chunk column stride
+constexpr int LDVC = IBM + 1;  // V chunk row stride
+constexpr int NT = HQ_THREADS;
+
+struct Smem {
+  union {
+    double P[IBM * LDP];  // panel block: column c at P + c*LDP, row i = panel row i
+    struct {
+      double V[CH * LDVC];     // V chunk: V(i, p) at V[i*LDVC + p]
+      double X[HQ_SLAB * LDX]; // X chunk: X(i, q) at X[q*LDX + i]
+      double W[IBM * HQ_SLAB]; // W / W2: W(p, q) at W[p*HQ_SLAB + q]
+    } u;
+  };
+  double R[IBM * LDT];   // TSQRT: triangle block of R(k,k), R(p, c) at R[c*LDT + p]
+  double G[IBM * LDT];   // Gram v_p^T v_c at G[c*LDT + p]
+  double TT[IBM * LDT];  // T(p, c) at TT[c*LDT + p]
+  double tau[IBM];
+  double red[4 * IBM];   // per-wave partial sums of a column step
+  int task;
+  hq_task next;
+};
+
+__device__ __forceinline__ double *tile_ptr(const hq_geom &G, int i, int j) {
+  return G.A + ((size_t)i + (size_t)j * G.MT) * (size_t)G.nb * G.nb;
+}
+__device__ __forceinline__ double *xtile_ptr(const hq_geom &G, int i, int j) {
+  return G.X + ((size_t)i + (size_t)j * G.MT) * (size_t)G.nb * G.nb;
+}
+__device__ __forceinline__ double *t_ptr(const hq_geom &G, int i, int j) {
+  return G.T + ((size_t)i + (size_t)j * G.MT) * (size_t)G.ib * G.nb;
+}
+__device__ __forceinline__ double *tau_ptr(const hq_geom &G, int i, int j) {
+  return G.TAU + ((size_t)i + (size_t)j * G.MT) * (size_t)G.nb;
+}
+
+// T of the block from the Gram matrix in S.G and tau in S.tau (dlarft,
+// forward, columnwise): T(c, c) = tau_c, T(0:c, c) = -tau_c T(0:c, 0:c)
+// G(0:c, c).  Wave 0, lane p owns row p.  T (ld ib, rows below the diagonal
+// zero) and tau to global.
+__device__ void build_t(Smem &S, int kb, double *Tb, int ib, double *taug) {
+  const int p = threadIdx.x;
+  if (p >= 64) return;
+  for (int c = 0; c < IBM; c++) {
+    double val = 0.0;
+    if (c < kb && p < c) {
+      double s = 0.0;
+      for (int q = p; q < c; q++) s += S.TT[q * LDT + p] * S.G[c * LDT + q];
+      val = -S.tau[c] * s;
+    } else if (c < kb && p == c) {
+      val = S.tau[c];
+    }
+    if (p < IBM) S.TT[c * LDT + p] = val;
+  }
+  if (p < ib)
+    for (int c = 0; c < kb; c++) Tb[(size_t)c * ib + p] = p < IBM ? S.TT[c * LDT + p] : 0.0;
+  if (taug && p < kb) taug[p] = S.tau[p];
+}
+
+// Householder QR of the block of columns ii .. ii+kb-1 (kb = min(ib, nb-ii)):
+//   GEQRT (!TS): rows ii.. of tile A (dgeqr2; R on and above the diagonal,
+//                V below it);
+//   TSQRT (TS):  [R(ii:ii+kb, ii:ii+kb) of the diagonal tile A1; tile A],
+//                the TSQRT reflectors (identity on top, V in A).
+// Writes the block back, its T block (T + ii*ib, ld ib) and TAU(ii..).
+// Leaves V (rows of the panel) in S.P and T in S.TT.
+template <bool TS>
+__device__ void panel(Smem &S, double *A, double *A1, double *T, double *TAU, int nb, int ib, int ii) {
+  const int tid = threadIdx.x;
+  const int kb = min(ib, nb - ii);
+  const int r0 = TS ? 0 : ii;  // first tile row of the panel
+  const int h = nb - r0;       // panel rows
+  for (int e = tid; e < kb * h; e += NT) {
+    const int c = e / h, i = e - c * h;
+    S.P[c * LDP + i] = A[(size_t)(ii + c) * nb + r0 + i];
+  }
+  if (TS)
+    for (int e = tid; e < kb * kb; e += NT) {
+      const int c = e / kb, p = e - c * kb;
+      S.R[c * LDT + p] = p <= c ? A1[(size_t)(ii + c) * nb + ii + p] : 0.0;
+    }
+  __syncthreads();
+  for (int c = 0; c < kb; c++) {
+    const int i0 = TS ? 0 : c + 1;  // rows of the reflector's tail
+    {
+      // squared norm of the tail (slot j = c) and its dot products with the
+      // tails of the columns right of it (slots j > c): 8 row groups per slot
+      const int j = tid & 31, rg = tid >> 5;
+      double s = 0.0;
+      if (j >= c && j < kb) {
+        const double *x = S.P + c * LDP, *y = S.P + j * LDP;
+        for (int i = i0 + rg; i < h; i += 8) s += x[i] * y[i];
+      }
+      s += __shfl_xor(s, 32, 64);
+      if ((tid & 63) < 32) S.red[(tid >> 6) * IBM + j] = s;
+    }
+    __syncthreads();
+    // dlarfg: beta = -sign(alpha) ||(alpha, x)||, tau = (beta - alpha) / beta,
+    // v = x / (alpha - beta); a zero tail gives tau = 0 and leaves x as it is
+    const double sigma = S.red[c] + S.red[IBM + c] + S.red[2 * IBM + c] + S.red[3 * IBM + c];
+    const double alpha = TS ? S.R[c * LDT + c] : S.P[c * LDP + c];
+    double tau = 0.0, beta = alpha, scale = 1.0;
+    if (sigma != 0.0) {
+      beta = -copysign(sqrt(alpha * alpha + sigma), alpha);
+      tau = (beta - alpha) / beta;
+      scale = 1.0 / (alpha - beta);
+    }
+    // the tail rows: v_i = x_i * scale, then y_j -= tau v_i w_j with
+    // w_j = top_j + v^T y_j = top_j + scale d_j (top = row c of the panel)
+    double v[2];
+    int rows[2];
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+      rows[r] = i0 + tid + r * NT;
+      v[r] = rows[r] < h ? S.P[c * LDP + rows[r]] * scale : 0.0;
+    }
+    if (tau != 0.0) {
+      for (int j = c + 1; j < kb; j++) {
+        const double d = S.red[j] + S.red[IBM + j] + S.red[2 * IBM + j] + S.red[3 * IBM + j];
+        const double top = TS ? S.R[j * LDT + c] : S.P[j * LDP + c];
+        const double tw = tau * (top + scale * d);
+#pragma unroll
+        for (int r = 0; r < 2; r++)
+          if (rows[r] < h) S.P[j * LDP + rows[r]] -= v[r] * tw;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+      if (rows[r] < h) S.P[c * LDP + rows[r]] = v[r];
+    __syncthreads();  // every thread has read row c (top) before it changes
+    // row c: R(c, j) -= tau w_j right of the column, R(c, c) = beta
+    if (tid < kb && tid >= c) {
+      const int j = tid;
+      double *top = TS ? &S.R[j * LDT + c] : &S.P[j * LDP + c];
+      if (j == c) {
+        *top = beta;
+      } else if (tau != 0.0) {
+        const double d = S.red[j] + S.red[IBM + j] + S.red[2 * IBM + j] + S.red[3 * IBM + j];
+        *top -= tau * (*top + scale * d);
+      }
+    }
+    if (tid == 0) S.tau[c] = tau;
+    __syncthreads();
+  }
+  // write back
+  for (int e = tid; e < kb * h; e += NT) {
+    const int c = e / h, i = e - c * h;
+    A[(size_t)(ii + c) * nb + r0 + i] = S.P[c * LDP + i];
+  }
+  if (TS)
+    for (int e = tid; e < kb * kb; e += NT) {
+      const int c = e / kb, p = e - c * kb;
+      if (p <= c) A1[(size_t)(ii + c) * nb + ii + p] = S.R[c * LDT + p];
+    }
+  // Gram G(p, c) = v_p^T v_c, p < c: one wave per pair.  GEQRT: v_p is 1 at
+  // panel row p and v_c is 1 at row c, zero above.
+  {
+    const int lane = tid & 63, wave = tid >> 6;
+    const int npair = kb * (kb - 1) / 2;
+    for (int e = wave; e < npair; e += NT / 64) {
+      int c = 1;
+      while ((c + 1) * c / 2 <= e) c++;
+      const int p = e - c * (c - 1) / 2;
+      const double *vp = S.P + p * LDP, *vc = S.P + c * LDP;
+      double s = 0.0;
+      for (int i = (TS ? 0 : c + 1) + lane; i < h; i += 64) s += vp[i] * vc[i];
+      for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+      if (lane == 0) S.G[c * LDT + p] = TS ? s : s + vp[c];
+    }
+  }
+  __syncthreads();
+  build_t(S, kb, T + (size_t)ii * ib, ib, TAU + ii);
+  __syncthreads();
+}
+
+// X <- H^T X for the block reflector H = I - V T V^T of the block at column ii
+// (kb reflectors of tile Vt, T block Tb with ld ib), on X's columns
+// c0 .. c1-1 (c1 - c0 <= 64):
+//   !TS: V unit lower trapezoidal (rows ii.. of Vt), X rows ii.. of tile X;
+//   TS:  [X1 rows ii..ii+kb; X] with V = [I; Vt columns], X all rows.
+template <bool TS>
+__device__ void apply_block(Smem &S, const double *Vt, const double *Tb, int ib, double *X, double *X1, int nb,
+                            int ii, int kb, int c0, int c1) {
+  const int tid = threadIdx.x;
+  const int r0 = TS ? 0 : ii;
+  const int h = nb - r0;
+  const int w = c1 - c0;
+  for (int e = tid; e < IBM * IBM; e += NT) {
+    const int c = e / IBM, p = e - c * IBM;
+    S.TT[c * LDT + p] = (c < kb && p <= c) ? Tb[(size_t)c * ib + p] : 0.0;
+  }
+  // W = V^T X (+ X1): thread (q, pg) sums reflectors 8pg .. 8pg+7 of column q
+  const int q = tid & 63, pg = tid >> 6;
+  double acc[8];
+#pragma unroll
+  for (int pp = 0; pp < 8; pp++) acc[pp] = 0.0;
+  for (int b0 = 0; b0 < h; b0 += CH) {
+    const int rows = min(CH, h - b0);
+    __syncthreads();  // the previous chunk has been read
+    for (int e = tid; e < IBM * CH; e += NT) {
+      const int p = e / CH, i = e - p * CH;
+      double val = 0.0;
+      if (p < kb && i < rows) {
+        const int row = b0 + i;  // row of the panel (tile row r0 + row)
+        if (TS || row > p)
+          val = Vt[(size_t)(ii + p) * nb + r0 + row];
+        else
+          val = row == p ? 1.0 : 0.0;
+      }
+      S.u.V[i * LDVC + p] = val;
+    }
+    for (int e = tid; e < HQ_SLAB * CH; e += NT) {
+      const int qq = e / CH, i = e - qq * CH;
+      S.u.X[qq * LDX + i] = (qq < w && i < rows) ? X[(size_t)(c0 + qq) * nb + r0 + b0 + i] : 0.0;
+    }
+    __syncthreads();
+    for (int i = 0; i < rows; i++) {
+      const double x = S.u.X[q * LDX + i];
+      const double *vr = S.u.V + i * LDVC + 8 * pg;
+#pragma unroll
+      for (int pp = 0; pp < 8; pp++) acc[pp] += vr[pp] * x;
+    }
+  }
+  if (TS && q < w) {
+#pragma unroll
+    for (int pp = 0; pp < 8; pp++) {
+      const int p = 8 * pg + pp;
+      if (p < kb) acc[pp] += X1[(size_t)(c0 + q) * nb + ii + p];
+    }
+  }
+  __syncthreads();  // the last chunk has been read (W overlaps nothing, V / X do)
+#pragma unroll
+  for (int pp = 0; pp < 8; pp++) S.u.W[(8 * pg + pp) * HQ_SLAB + q] = acc[pp];
+  __syncthreads();
+  // W2 = T^T W (W2(p, q) = sum_{r <= p} T(r, p) W(r, q)); TS: X1 -= W2
+  double w2[8];
+#pragma unroll
+  for (int pp = 0; pp < 8; pp++) {
+    const int p = 8 * pg + pp;
+    double s = 0.0;
+    for (int r = 0; r <= p; r++) s += S.TT[p * LDT + r] * S.u.W[r * HQ_SLAB + q];
+    w2[pp] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int pp = 0; pp < 8; pp++) {
+    const int p = 8 * pg + pp;
+    S.u.W[p * HQ_SLAB + q] = w2[pp];
+    if (TS && q < w && p < kb) X1[(size_t)(c0 + q) * nb + ii + p] -= w2[pp];
+  }
+  __syncthreads();
+  // X -= V W2: thread (row i of the chunk, 16-column group qg), its row of V
+  // in registers
+  const int i = tid & 63, qg = tid >> 6;
+  for (int b0 = 0; b0 < h; b0 += CH) {
+    const int row = b0 + i;
+    if (row >= h) continue;
+    double vr[IBM];
+#pragma unroll
+    for (int p = 0; p < IBM; p++) {
+      double val = 0.0;
+      if (p < kb) {
+        if (TS || row > p)
+          val = Vt[(size_t)(ii + p) * nb + r0 + row];
+        else
+          val = row == p ? 1.0 : 0.0;
+      }
+      vr[p] = val;
+    }
+    for (int qq = 0; qq < 16; qq++) {
+      const int qc = 16 * qg + qq;
+      if (qc >= w) break;
+      double *xp = X + (size_t)(c0 + qc) * nb + r0 + row;
+      double s = 0.0;
+#pragma unroll
+      for (int p = 0; p < IBM; p++) s += vr[p] * S.u.W[p * HQ_SLAB + qc];
+      *xp -= s;
+    }
+  }
+  __syncthreads();
+}
+
+// The block's reflectors applied to every column right of it in the tile
+// (whole-tile GEQRT / TSQRT), 64 columns at a time.
+template <bool TS>
+__device__ void panel_rest(Smem &S, double *A, double *A1, const double *T, int nb, int ib, int ii) {
+  const int kb = min(ib, nb - ii);
+  for (int c0 = ii + kb; c0 < nb; c0 += HQ_SLAB)
+    apply_block<TS>(S, A, T + (size_t)ii * ib, ib, A, A1, nb, ii, kb, c0, min(nb, c0 + HQ_SLAB));
+}
+
+// One task of the DAG (the task kinds and fields of hq_task).
+__device__ void run_task(Smem &S, const hq_geom &G, const hq_task &t) {
+  const int nb = G.nb, ib = G.ib;
+  switch (t.kind) {
+    case HQ_GEQRT: {
+      double *A = tile_ptr(G, t.k, t.k), *T = t_ptr(G, t.k, t.k), *TAU = tau_ptr(G, t.k, t.k);
+      if (t.j >= 0) {
+        panel<false>(S, A, nullptr, T, TAU, nb, ib, t.j * ib);
+      } else {
+        for (int ii = 0; ii < nb; ii += ib) {
+          panel<false>(S, A, nullptr, T, TAU, nb, ib, ii);
+          panel_rest<false>(S, A, nullptr, T, nb, ib, ii);
+        }
+      }
+      break;
+    }
+    case HQ_TSQRT: {
+      double *A1 = tile_ptr(G, t.k, t.k), *A2 = tile_ptr(G, t.r, t.k);
+      double *T = t_ptr(G, t.r, t.k), *TAU = tau_ptr(G, t.r, t.k);
+      if (t.j >= 0) {
+        panel<true>(S, A2, A1, T, TAU, nb, ib, t.j * ib);
+      } else {
+        for (int ii = 0; ii < nb; ii += ib) {
+          panel<true>(S, A2, A1, T, TAU, nb, ib, ii);
+          panel_rest<true>(S, A2, A1, T, nb, ib, ii);
+        }
+      }
+      break;
+    }
+    case HQ_GEQRT_U: {
+      double *A = tile_ptr(G, t.k, t.k);
+      const int ii = t.j * ib, kb = min(ib, nb - ii);
+      const int c0 = max(t.slab * HQ_SLAB, ii + kb), c1 = min(nb, (t.slab + 1) * HQ_SLAB);
+      if (c0 < c1) apply_block<false>(S, A, t_ptr(G, t.k, t.k) + (size_t)ii * ib, ib, A, nullptr, nb, ii, kb, c0, c1);
+      break;
+    }
+    case HQ_TSQRT_U: {
+      double *A1 = tile_ptr(G, t.k, t.k), *A2 = tile_ptr(G, t.r, t.k);
+      const int ii = t.j * ib, kb = min(ib, nb - ii);
+      const int c0 = max(t.slab * HQ_SLAB, ii + kb), c1 = min(nb, (t.slab + 1) * HQ_SLAB);
+      if (c0 < c1) apply_block<true>(S, A2, t_ptr(G, t.r, t.k) + (size_t)ii * ib, ib, A2, A1, nb, ii, kb, c0, c1);
+      break;
+    }
+    case HQ_LARFB: {
+      const double *V = tile_ptr(G, t.k, t.k), *T = t_ptr(G, t.k, t.k);
+      double *X = xtile_ptr(G, t.k, t.j);
+      const int uw = hq_upd_slab(nb, 0), c0 = t.slab * uw, c1 = min(nb, c0 + uw);
+      for (int ii = 0; ii < nb; ii += ib)
+        apply_block<false>(S, V, T + (size_t)ii * ib, ib, X, nullptr, nb, ii, min(ib, nb - ii), c0, c1);
+      break;
+    }
+    case HQ_SSRFB: {
+      const double *V = tile_ptr(G, t.r, t.k), *T = t_ptr(G, t.r, t.k);
+      double *X1 = xtile_ptr(G, t.k, t.j), *X2 = xtile_ptr(G, t.r, t.j);
+      const int uw = hq_upd_slab(nb, 0), c0 = t.slab * uw, c1 = min(nb, c0 + uw);
+      for (int ii = 0; ii < nb; ii += ib)
+        apply_block<true>(S, V, T + (size_t)ii * ib, ib, X2, X1, nb, ii, min(ib, nb - ii), c0, c1);
+      break;
+    }
+    default:
+      break;
+  }
+}
+
+// Persistent DAG scheduler: the claim / wait / publish protocol of the
+// specialised scheduler kernel (tasks claimed in order from one atomic
+// counter, dependency flags equal to this run's epoch, release before
+// publishing, acquire after the wait, a bounded wait that reports the task in
+// counter[1] and drains the grid), without its look-ahead claim.
+__global__ __launch_bounds__(NT) void sched_kernel(hq_geom G0, const hq_geom *__restrict__ geoms,
+                                                   const hq_task *__restrict__ tasks, int ntasks, int *flags,
+                                                   int epoch_arg, int *counter, long long *trace,
+                                                   const int *epoch_dev, unsigned wait_limit) {
+  __shared__ Smem S;
+  const int epoch = epoch_dev ? __hip_atomic_load(epoch_dev, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                              : epoch_arg;
+  int done = -1;
+  long long t_claim = 0;
+  for (;;) {
+    // publish the previous task and claim the next one in one thread-0 region
+    if (threadIdx.x == 0) {
+      if (done >= 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __builtin_amdgcn_s_waitcnt(0);
+        __hip_atomic_store(&flags[done], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (trace) trace[4 * done + 2] = wall_clock64();
+      }
+      int t = atomicAdd(counter, 1);
+      if (trace) t_claim = wall_clock64();
+      if (__hip_atomic_load(&counter[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) t = ntasks;
+      if (t < ntasks) {
+        const hq_task &tk = tasks[t];
+        S.next = tk;
+        const int nd = tk.ndep;
+        unsigned spins = 0;
+        for (;;) {
+          int f[HQ_MAXDEP];
+#pragma unroll
+          for (int i = 0; i < HQ_MAXDEP; i++)
+            f[i] = i < nd ? __hip_atomic_load(&flags[tk.dep[i]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                          : epoch;
+          bool ready = true;
+#pragma unroll
+          for (int i = 0; i < HQ_MAXDEP; i++) ready = ready && f[i] == epoch;
+          if (ready) break;
+          __builtin_amdgcn_s_sleep(2);
+          if (++spins > wait_limit) {
+            __hip_atomic_store(&counter[1], 1 + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            t = ntasks;
+            break;
+          }
+        }
+        if (t < ntasks) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      }
+      S.task = t;
+    }
+    __syncthreads();
+    const int t = __builtin_amdgcn_readfirstlane(S.task);
+    if (t >= ntasks) break;
+    hq_task tk;
+    tk.kind = __builtin_amdgcn_readfirstlane(S.next.kind);
+    tk.k = __builtin_amdgcn_readfirstlane(S.next.k);
+    tk.r = __builtin_amdgcn_readfirstlane(S.next.r);
+    tk.j = __builtin_amdgcn_readfirstlane(S.next.j);
+    tk.slab = __builtin_amdgcn_readfirstlane(S.next.slab);
+    tk.mat = __builtin_amdgcn_readfirstlane(S.next.mat);
+    tk.ndep = 0;
+    if (trace && threadIdx.x == 0) {
+      trace[4 * t + 0] = t_claim;
+      trace[4 * t + 1] = wall_clock64();
+      trace[4 * t + 3] = blockIdx.x;
+    }
+    run_task(S, geoms ? geoms[tk.mat] : G0, tk);
+    // every wave's stores have completed before the flag can be published
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    done = t;
+  }
+  if (trace && threadIdx.x == 0 && done >= 0) trace[4 * done + 2] = wall_clock64();
+}
+
+// One task per launch (stream order provides the dependencies).
+__global__ __launch_bounds__(NT) void task_kernel(hq_geom G, hq_task tk) {
+  __shared__ Smem S;
+  run_task(S, G, tk);
+}
+
+// Standalone tile kernels on explicit tile pointers (ld = nb, T ld = ib <= 32).
+__global__ __launch_bounds__(NT) void geqrt_kernel(double *A, double *T, double *TAU, int nb, int ib) {
+  __shared__ Smem S;
+  for (int ii = 0; ii < nb; ii += ib) {
+    panel<false>(S, A, nullptr, T, TAU, nb, ib, ii);
+    panel_rest<false>(S, A, nullptr, T, nb, ib, ii);
+  }
+}
+__global__ __launch_bounds__(NT) void tsqrt_kernel(double *A1, double *A2, double *T, double *TAU, int nb, int ib) {
+  __shared__ Smem S;
+  for (int ii = 0; ii < nb; ii += ib) {
+    panel<true>(S, A2, A1, T, TAU, nb, ib, ii);
+    panel_rest<true>(S, A2, A1, T, nb, ib, ii);
+  }
+}
+// grid.x = 64-column slab of C / of [A1; A2]
+__global__ __launch_bounds__(NT) void larfb_kernel(const double *V, const double *T, double *C, int nb, int ib) {
+  __shared__ Smem S;
+  const int c0 = blockIdx.x * HQ_SLAB, c1 = min(nb, c0 + HQ_SLAB);
+  for (int ii = 0; ii < nb; ii += ib)
+    apply_block<false>(S, V, T + (size_t)ii * ib, ib, C, nullptr, nb, ii, min(ib, nb - ii), c0, c1);
+}
+__global__ __launch_bounds__(NT) void ssrfb_kernel(double *A1, double *A2, const double *V, const double *T, int nb,
+                                                   int ib) {
+  __shared__ Smem S;
+  const int c0 = blockIdx.x * HQ_SLAB, c1 = min(nb, c0 + HQ_SLAB);
+  for (int ii = 0; ii < nb; ii += ib)
+    apply_block<true>(S, V, T + (size_t)ii * ib, ib, A2, A1, nb, ii, min(ib, nb - ii), c0, c1);
+}
+
+// T of one block of kb <= ib reflectors (any ib <= nb) rebuilt from the
+// reflector tile V (ld nb; diag: unit lower with the diagonal at row i0 + c,
+// else the V of a TSQRT tile under an identity) and tau: Gram entries one
+// wave per pair into T's strict upper part, then dlarft column by column
+// (column c of the Gram matrix read into LDS before the column is
+// overwritten); rows below the diagonal written as zero.  drop_last: the
+// last reflector of a diagonal tile (a 1-row column) counts with tau = 0.
+__device__ void tbuild_body(const double *V, const double *tau, double *Tb, int nb, int ib, int i0, int kb,
+                            bool diag, bool drop_last) {
+  __shared__ double gcol[NBMAX];
+  __shared__ double tv[NBMAX];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int c = tid; c < kb; c += NT) {
+    double t = tau[c];
+    if (drop_last && diag && i0 + c == nb - 1) t = 0.0;
+    tv[c] = t;
+  }
+  const int npair = kb * (kb - 1) / 2;
+  for (int e = wave; e < npair; e += NT / 64) {
+    int c = 1;
+    while ((c + 1) * c / 2 <= e) c++;
+    const int p = e - c * (c - 1) / 2;
+    const double *vp = V + (size_t)(i0 + p) * nb, *vc = V + (size_t)(i0 + c) * nb;
+    double s = 0.0;
+    for (int row = lane; row < nb; row += 64) {
+      double a = vp[row], b = vc[row];
+      if (diag) {
+        const int dp = i0 + p, dc = i0 + c;
+        a = row > dp ? a : (row == dp ? 1.0 : 0.0);
+        b = row > dc ? b : (row == dc ? 1.0 : 0.0);
+      }
+      s += a * b;
+    }
+    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) Tb[(size_t)c * ib + p] = s;
+  }
+  __syncthreads();
+  for (int c = 0; c < kb; c++) {
+    for (int q = tid; q < c; q += NT) gcol[q] = Tb[(size_t)c * ib + q];
+    __syncthreads();
+    const double tc = tv[c];
+    for (int p = tid; p < ib; p += NT) {
+      double val = 0.0;
+      if (p < c) {
+        double s = 0.0;
+        for (int q = p; q < c; q++) s += Tb[(size_t)q * ib + p] * gcol[q];
+        val = -tc * s;
+      } else if (p == c) {
+        val = tc;
+      }
+      Tb[(size_t)c * ib + p] = val;
+    }
+    __syncthreads();
+  }
+}
+
+// Every reflector tile of a tile grid (grid.x = ib block, grid.y = entry of
+// the lower tile list (r, k)).
+__global__ __launch_bounds__(NT) void tbuild_kernel(hq_geom G, const int2 *tiles, int drop_last) {
+  const int2 rk = tiles[blockIdx.y];
+  const int r = rk.x, k = rk.y, ib = G.ib, i0 = blockIdx.x * ib;
+  if (i0 >= G.nb) return;
+  tbuild_body(tile_ptr(G, r, k), tau_ptr(G, r, k) + i0, t_ptr(G, r, k) + (size_t)i0 * ib, G.nb, ib, i0,
+              min(ib, G.nb - i0), r == k, drop_last != 0);
+}
+__global__ __launch_bounds__(NT) void tbuild_tile_kernel(const double *V, const double *tau, double *T, int nb,
+                                                         int ib, int diag) {
+  const int i0 = blockIdx.x * ib;
+  if (i0 >= nb) return;
+  tbuild_body(V, tau + i0, T + (size_t)i0 * ib, nb, ib, i0, min(ib, nb - i0), diag != 0, false);
+}
+
+}  // namespace hqg
+
+// ---------------------------------------------------------------------------
+// Launch wrappers (called by the nb dispatch of the specialised launchers).
+// ---------------------------------------------------------------------------
+hipError_t hqg_launch_sched(const hq_geom &G, const hq_geom *geoms, const hq_task *tasks, int ntasks, int *flags,
+                            int epoch, int *counter, int grid, long long *trace, hipStream_t st, int *epoch_dev,
+                            unsigned wait_limit) {
+  if (G.nb > hqg::NBMAX || G.ib > hqg::IBM || G.prec != 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(hqg::sched_kernel, dim3(grid), dim3(hqg::NT), 0, st, G, geoms, tasks, ntasks, flags, epoch,
+                     counter, trace, (const int *)epoch_dev, wait_limit);
+  return hipGetLastError();
+}
+hipError_t hqg_launch_task(const hq_geom &G, const hq_task &tk, hipStream_t st) {
+  if (G.nb > hqg::NBMAX || G.ib > hqg::IBM) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(hqg::task_kernel, dim3(1), dim3(hqg::NT), 0, st, G, tk);
+  return hipGetLastError();
+}
+hipError_t hqg_launch_geqrt(int nb, int ib, double *A, double *T, double *TAU, hipStream_t st) {
+  if (nb > hqg::NBMAX || ib > hqg::IBM) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(hqg::geqrt_kernel, dim3(1), dim3(hqg::NT), 0, st, A, T, TAU, nb, ib);
+  return hipGetLastError();
+}
+hipError_t hqg_launch_tsqrt(int nb, int ib, double *A1, double *A2, double *T, double *TAU, hipStream_t st) {
+  if (nb > hqg::NBMAX || ib > hqg::IBM) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(hqg::tsqrt_kernel, dim3(1), dim3(hqg::NT), 0, st, A1, A2, T, TAU, nb, ib);
+  return hipGetLastError();
+}
+hipError_t hqg_launch_larfb(int nb, int ib, const double *V, const double *T, double *C, hipStream_t st) {
+  if (nb > hqg::NBMAX || ib > hqg::IBM) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(hqg::larfb_kernel, dim3((nb + HQ_SLAB - 1) / HQ_SLAB), dim3(hqg::NT), 0, st, V, T, C, nb, ib);
+  return hipGetLastError();
+}
+hipError_t hqg_launch_ssrfb(int nb, int ib, double *A1, double *A2, const double *V, const double *T,
+                            hipStream_t st) {
+  if (nb > hqg::NBMAX || ib > hqg::IBM) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(hqg::ssrfb_kernel, dim3((nb + HQ_SLAB - 1) / HQ_SLAB), dim3(hqg::NT), 0, st, A1, A2, V, T, nb,
+                     ib);
+  return hipGetLastError();
+}
+hipError_t hqg_launch_tbuild(const hq_geom &G, const int2 *tiles, int ntiles, int drop_last, hipStream_t st) {
+  if (G.nb > hqg::NBMAX) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(hqg::tbuild_kernel, dim3((G.nb + G.ib - 1) / G.ib, ntiles), dim3(hqg::NT), 0, st, G, tiles,
+                     drop_last);
+  return hipGetLastError();
+}
+hipError_t hqg_launch_tbuild_tile(int nb, int ib, const double *V, const double *tau, double *T, int diag,
+                                  hipStream_t st) {
+  if (nb > hqg::NBMAX) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(hqg::tbuild_tile_kernel, dim3((nb + ib - 1) / ib), dim3(hqg::NT), 0, st, V, tau, T, nb, ib,
+                     diag);
+  return hipGetLastError();
+}

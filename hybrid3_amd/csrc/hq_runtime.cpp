@@ -87,11 +87,16 @@ int ok() {
       return set_error(HQ_ERR_DEVICE, "%s failed: %s", #call, hipGetErrorString(e_));     \
   } while (0)
 
-bool nb_ok(int nb) { return nb == 64 || nb == 128 || nb == 256; }
+// Any tile size up to 512 runs; 64, 128 and 256 have the specialised kernel
+// sets (register-resident updates, MFMA), the others the generic ones, and
+// only 128 and 256 have the reduced-precision updates.
+bool nb_ok(int nb) { return nb >= 1 && nb <= 512; }
+bool tuned_nb(int nb) { return nb == 64 || nb == 128 || nb == 256; }
+bool rp_nb(int nb) { return nb == 128 || nb == 256; }
 
 bool geometry_ok(int MT, int NT, int nb, int ib) {
   if (MT < 1 || NT < 1) return set_error(HQ_ERR_ARG, "tile grid %d x %d is empty", MT, NT), false;
-  if (!nb_ok(nb)) return set_error(HQ_ERR_ARG, "nb=%d unsupported (64, 128 or 256)", nb), false;
+  if (!nb_ok(nb)) return set_error(HQ_ERR_ARG, "nb=%d unsupported (1..512)", nb), false;
   if (ib < 1 || ib > nb) return set_error(HQ_ERR_ARG, "ib=%d unsupported (1..nb=%d)", ib, nb), false;
   return true;
 }
@@ -535,7 +540,7 @@ int tile_list(Context *c, int MT, int NT, const int2 **out, int *n);
 // with), reserved and entered into G.
 int reserve_scratch(Context *c, hq_geom &G) {
   const int MT = G.MT, NT = G.NT, nb = G.nb, ik = G.ib;
-  if (!G.prec && nb >= 128 && ik % 16 == 0) {
+  if (!G.prec && nb >= 128 && tuned_nb(nb) && ik % 16 == 0) {
     // the GEQRT panels also write W = V T16 and the unit lower V of the
     // diagonal tiles; LARFB then needs no T multiplication between passes
     const size_t tiles = sizeof(double) * (size_t)nb * nb * std::min(MT, NT);
@@ -555,7 +560,7 @@ int reserve_scratch(Context *c, hq_geom &G) {
 int tile_qr_dev_impl(int MT, int NT, int nb, int ib, double *A, double *TAU, double *T,
                      void *stream, bool serial, long long *trace = nullptr, int prec = 0) {
   if (!geometry_ok(MT, NT, nb, ib)) return g_err;
-  if (prec && nb < 128) return set_error(HQ_ERR_ARG, "reduced-precision updates need nb >= 128 (nb=%d)", nb);
+  if (prec && !rp_nb(nb)) return set_error(HQ_ERR_ARG, "reduced-precision updates need nb = 128 or 256 (nb=%d)", nb);
   if (!A || !TAU) return set_error(HQ_ERR_ARG, "A and TAU must be device pointers");
   Context *c;
   if (get_ctx(&c) != HQ_OK) return g_err;
@@ -638,7 +643,7 @@ int apply_qt_impl(int MT, int NT, int nb, int ib, const double *RY, const double
   if (order_in(c, st) != HQ_OK) return g_err;
   double *Tw = (double *)T;
   int iq = ib;
-  if (Tw && nb < 128 && ib > HQ_IBMAX) Tw = nullptr;
+  if (Tw && (nb < 128 || !tuned_nb(nb)) && ib > HQ_IBMAX) Tw = nullptr;
   if (!Tw) {
     iq = ibi(ib);
     if (ensure_T(c, MT, NT, nb, iq, &Tw) != HQ_OK) return g_err;
@@ -730,7 +735,7 @@ int batch_geoms(int count, int nb, int ib, const int *MT, const int *NT, double 
   for (int i = 0; i < count; i++)
     if (!T || !T[i]) tdoubles += (size_t)MT[i] * NT[i] * ib * nb;
   if (tdoubles && Tb.reserve(sizeof(double) * tdoubles) != HQ_OK) return g_err;
-  const bool wpath = !prec && nb >= 128 && ib % 16 == 0;
+  const bool wpath = !prec && nb >= 128 && tuned_nb(nb) && ib % 16 == 0;
   size_t vhalves = 0;
   if (prec == 2) {
     for (int i = 0; i < count; i++) vhalves += hq_vh_tile(nb, ib) * MT[i] * NT[i];
@@ -769,7 +774,7 @@ int batched_qr_impl(int count, int nb, int ib, const int *MT, const int *NT, dou
   if (batch_args_ok(count, nb, ib, MT, NT, A, TAU) != HQ_OK) return g_err;
   if (prec < 0 || prec > 2) return set_error(HQ_ERR_ARG, "prec=%d (0 FP64, 1 FP32, 2 split)", prec);
   if (split_panels < 0 || split_panels > 1) return set_error(HQ_ERR_ARG, "panel mapping %d (0 or 1)", split_panels);
-  if (prec && nb < 128) return set_error(HQ_ERR_ARG, "reduced-precision updates need nb >= 128 (nb=%d)", nb);
+  if (prec && !rp_nb(nb)) return set_error(HQ_ERR_ARG, "reduced-precision updates need nb = 128 or 256 (nb=%d)", nb);
   Context *c;
   if (get_ctx(&c) != HQ_OK) return g_err;
   hipStream_t st = pick_stream(c, stream);
@@ -912,7 +917,7 @@ int hq_tile_qr_dev(int MT, int NT, int nb, int ib, double *A, double *TAU, doubl
 int hq_tile_qr_prepare_dev(int MT, int NT, int nb, int ib, int prec) {
   if (!geometry_ok(MT, NT, nb, ib)) return g_err;
   if (prec < 0 || prec > 2) return set_error(HQ_ERR_ARG, "prec=%d (0 FP64, 1 FP32, 2 split)", prec);
-  if (prec && nb < 128) return set_error(HQ_ERR_ARG, "reduced-precision updates need nb >= 128 (nb=%d)", nb);
+  if (prec && !rp_nb(nb)) return set_error(HQ_ERR_ARG, "reduced-precision updates need nb = 128 or 256 (nb=%d)", nb);
   Context *c;
   if (get_ctx(&c) != HQ_OK) return g_err;
   const int ik = ibi(ib);
@@ -1079,7 +1084,7 @@ int hq_tile_qr_traced_prec_dev(int MT, int NT, int nb, int ib, int prec, double 
 
 static int prec_ok(int prec, int nb) {
   if (prec < 0 || prec > 2) return set_error(HQ_ERR_ARG, "prec=%d (0 FP64, 1 FP32, 2 split)", prec);
-  if (prec && nb < 128) return set_error(HQ_ERR_ARG, "reduced-precision updates need nb >= 128 (nb=%d)", nb);
+  if (prec && !rp_nb(nb)) return set_error(HQ_ERR_ARG, "reduced-precision updates need nb = 128 or 256 (nb=%d)", nb);
   return HQ_OK;
 }
 
@@ -1179,7 +1184,7 @@ int hq_schedule_info(int MT, int NT, int nb, int ib, int *ntasks, int *grid) {
 // whole ib blocks (ib <= HQ_IBMAX there).
 static int tile_args_ok(int nb, int ib, bool consumer) {
   if (!nb_ok(nb) || ib < 1 || ib > nb) return set_error(HQ_ERR_ARG, "unsupported nb=%d / ib=%d", nb, ib);
-  if (consumer && nb < 128 && ib > HQ_IBMAX)
+  if (consumer && (nb < 128 || !tuned_nb(nb)) && ib > HQ_IBMAX)
     return set_error(HQ_ERR_ARG, "nb=%d takes ib <= %d in LARFB / SSRFB (ib=%d)", nb, HQ_IBMAX, ib);
   return HQ_OK;
 }
@@ -1282,7 +1287,7 @@ int hq_batched_prepare_ex_dev(int prec, int split_panels, int count, int nb, int
   if (!MT || !NT) return set_error(HQ_ERR_ARG, "MT and NT arrays are required");
   if (ib > HQ_IBMAX) return set_error(HQ_ERR_ARG, "batches take ib <= %d (ib=%d)", HQ_IBMAX, ib);
   if (prec < 0 || prec > 2) return set_error(HQ_ERR_ARG, "prec=%d (0 FP64, 1 FP32, 2 split)", prec);
-  if (prec && nb < 128) return set_error(HQ_ERR_ARG, "reduced-precision updates need nb >= 128 (nb=%d)", nb);
+  if (prec && !rp_nb(nb)) return set_error(HQ_ERR_ARG, "reduced-precision updates need nb = 128 or 256 (nb=%d)", nb);
   for (int i = 0; i < count; i++)
     if (!geometry_ok(MT[i], NT[i], nb, ib)) return g_err;
   Context *c;
@@ -1750,7 +1755,7 @@ void Hybrid_dgemm(mat_stat_t *status, double *A, double *B, double *Cm, int my_r
 // (Hybrid_tile_QR.c:985-990,1135-1141,1276-1283,1356-1363), on the GPU.
 // Host pointers with any leading dimension; every array is packed to the
 // kernels' layout (ld = nb, T ld = ib) on a lane's stream and unpacked after.
-// Square nb x nb tiles (nb in {64, 128, 256}), K = nb, side = Left,
+// Square nb x nb tiles (nb <= 512), K = nb, side = Left,
 // trans = Trans (the only forms the reference uses).  Return values follow
 // PLASMA: 0 on success, -i when argument i is illegal or outside what this
 // shim supports (message in hq_last_error_message()).
@@ -1816,7 +1821,7 @@ int CORE_dgeqrt(int M, int N, int IB, double *A, int LDA, double *T, int LDT, do
   if (LDA < (M > 1 ? M : 1)) return core_bad(5, fn, "LDA < max(1, M)");
   if (LDT < (IB > 1 ? IB : 1)) return core_bad(7, fn, "LDT < max(1, IB)");
   if (M == 0 || N == 0 || IB == 0) return (ok(), 0);
-  if (M != N || !nb_ok(M)) return core_bad(1, fn, "square tiles of 64, 128 or 256 only");
+  if (M != N || !nb_ok(M)) return core_bad(1, fn, "square tiles of at most 512 only");
   if (IB > N) return core_bad(3, fn, "IB > N");
   const int nb = M;
   const int rc = packed_call({{A, nb, nb, LDA, true, true}, {T, IB, nb, LDT, false, true}, {TAU, nb, 1, nb, false, true}},
@@ -1837,7 +1842,7 @@ int CORE_dtsqrt(int M, int N, int IB, double *A1, int LDA1, double *A2, int LDA2
   if (LDA2 < (M > 1 ? M : 1)) return core_bad(7, fn, "LDA2 < max(1, M)");
   if (LDT < (IB > 1 ? IB : 1)) return core_bad(9, fn, "LDT < max(1, IB)");
   if (M == 0 || N == 0 || IB == 0) return (ok(), 0);
-  if (M != N || !nb_ok(M)) return core_bad(1, fn, "square tiles of 64, 128 or 256 only");
+  if (M != N || !nb_ok(M)) return core_bad(1, fn, "square tiles of at most 512 only");
   if (IB > N) return core_bad(3, fn, "IB > N");
   const int nb = M;
   const int rc = packed_call({{A1, nb, nb, LDA1, true, true}, {A2, nb, nb, LDA2, true, true},
@@ -1863,7 +1868,7 @@ int CORE_dormqr(int side, int trans, int M, int N, int K, int IB, const double *
   if (LDT < (IB > 1 ? IB : 1)) return core_bad(10, fn, "LDT < max(1, IB)");
   if (LDC < (M > 1 ? M : 1)) return core_bad(12, fn, "LDC < max(1, M)");
   if (M == 0 || N == 0 || K == 0) return (ok(), 0);
-  if (M != N || K != M || !nb_ok(M)) return core_bad(3, fn, "M = N = K in {64, 128, 256} only");
+  if (M != N || K != M || !nb_ok(M)) return core_bad(3, fn, "M = N = K <= 512 only");
   if (IB == 0 || IB > K) return core_bad(6, fn, "IB outside 1..K");
   if (M < 128 && IB > HQ_IBMAX) return core_bad(6, fn, "nb = 64 takes IB <= 32");
   const int nb = M;
@@ -1894,7 +1899,7 @@ int CORE_dtsmqr(int side, int trans, int M1, int N1, int M2, int N2, int K, int 
   if (LDV < (M2 > 1 ? M2 : 1)) return core_bad(14, fn, "LDV < max(1, M2)");
   if (LDT < (IB > 1 ? IB : 1)) return core_bad(16, fn, "LDT < max(1, IB)");
   if (M1 == 0 || N1 == 0 || M2 == 0 || K == 0) return (ok(), 0);
-  if (M1 != N1 || M2 != M1 || K != M1 || !nb_ok(M1)) return core_bad(3, fn, "M1 = N1 = M2 = N2 = K in {64, 128, 256} only");
+  if (M1 != N1 || M2 != M1 || K != M1 || !nb_ok(M1)) return core_bad(3, fn, "M1 = N1 = M2 = N2 = K <= 512 only");
   if (IB == 0 || IB > K) return core_bad(8, fn, "IB outside 1..K");
   if (M1 < 128 && IB > HQ_IBMAX) return core_bad(8, fn, "nb = 64 takes IB <= 32");
   const int nb = M1;

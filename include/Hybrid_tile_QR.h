@@ -9,7 +9,8 @@
  * rejected (hq_last_error() != 0, arrays untouched).  Errors never abort the
  * process; query hq_last_error() / hq_last_error_message() (hybrid3_dev.h).
  *
- * Supported geometry: tile_M == tile_N == nb in {64, 128, 256},
+ * Supported geometry: tile_M == tile_N == nb, 1 <= nb <= 512 (64, 128 and
+ * 256 run the specialised MFMA kernels, other sizes the generic ones),
  * 1 <= ib <= nb (main.c:85-88 defaults ib to nb); the matrix is
  * glob_tile_M x glob_tile_N tiles (any glob_tile_M >= 1, glob_tile_N >= 1,
  * tall and wide included).  As in the reference, mat_init's integer

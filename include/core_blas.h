@@ -14,8 +14,9 @@
  * device, runs one kernel and copies the results back, synchronously.  Calls
  * are reentrant (each takes its own stream and scratch).
  *
- * Supported shapes: square nb x nb tiles with nb in {64, 128, 256}, K = nb,
- * 1 <= IB <= nb (IB <= 32 for CORE_dormqr / CORE_dtsmqr at nb = 64),
+ * Supported shapes: square nb x nb tiles with 1 <= nb <= 512, K = nb,
+ * 1 <= IB <= nb (IB <= 32 for CORE_dormqr / CORE_dtsmqr at nb other than
+ * 128 and 256),
  * side = PlasmaLeft, trans = PlasmaTrans.  Anything else returns -i (i = the
  * offending argument, message in hq_last_error_message()); a device failure
  * returns -1000 - (hq error code).  WORK / LDWORK are not used.

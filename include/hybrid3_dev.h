@@ -156,9 +156,9 @@ int hq_schedule_info(int MT, int NT, int nb, int ib, int *ntasks, int *grid);
 /* hq_schedule_info of the factorisation with updates in precision prec. */
 int hq_schedule_info_prec(int MT, int NT, int nb, int ib, int prec, int *ntasks, int *grid);
 
-/* PLASMA-style single-tile kernels on device pointers (nb in {64,128,256},
- * 1 <= ib <= nb (LARFB / SSRFB at nb = 64: ib <= 32), every leading
- * dimension = nb, T leading dimension = ib):
+/* PLASMA-style single-tile kernels on device pointers (1 <= nb <= 512,
+ * 1 <= ib <= nb (LARFB / SSRFB at nb other than 128 and 256: ib <= 32),
+ * every leading dimension = nb, T leading dimension = ib):
  *   GEQRT = CORE_dgeqrt(nb,nb,ib,A,nb,T,ib,TAU)        (Hybrid_tile_QR.c:985)
  *   TSQRT = CORE_dtsqrt(nb,nb,ib,A1,nb,A2,nb,T,ib,TAU) (Hybrid_tile_QR.c:1135)
  *   LARFB = CORE_dormqr(Left,Trans,nb,nb,nb,ib,V,nb,T,ib,C,nb) (:1276)

@@ -75,10 +75,10 @@ def test_rank_and_geometry_errors_leave_arrays_untouched(hq):
     assert hq.last_error()[0] == 1 and np.array_equal(A, A0)
     L.Hybrid_tile_QR(C.byref(st), _p(A), _p(TAU), 0, 1, 65)   # ib above nb = 64
     assert hq.last_error()[0] == 1
-    st2 = hq.mat_init(300, 300, 100)                           # nb = 100 unsupported
+    st2 = hq.mat_init(1200, 1200, 600)                         # nb = 600 unsupported (above 512)
     L.Hybrid_tile_QR(C.byref(st2), _p(A), _p(TAU), 0, 1, 16)
     code, msg = hq.last_error()
-    assert code == 1 and "nb=100" in msg
+    assert code == 1 and "nb=600" in msg
     L.Hybrid_dorgqr(C.byref(st), _p(A), _p(A0), _p(TAU), 0, 4)
     assert hq.last_error()[0] == 2
     L.tile_dgeqrt(0, 8, 8, _p(A), 8, _p(TAU), _p(A), _p(A), 8, None)
@@ -88,7 +88,7 @@ def test_rank_and_geometry_errors_leave_arrays_untouched(hq):
 
 def test_device_entry_points_validate_before_launch(hq):
     L = hq.lib()
-    assert L.hq_tile_qr_dev(4, 4, 96, 16, None, None, None, None) == 1
+    assert L.hq_tile_qr_dev(4, 4, 600, 16, None, None, None, None) == 1
     assert L.hq_tile_qr_dev(0, 4, 64, 16, None, None, None, None) == 1
     assert L.hq_dgeqrt_dev(64, 0, None, None, None, None) == 1
     assert L.hq_tile_ormqr_dev(4, 4, 64, 16, None, None, None, None, 0, None) == 1
@@ -194,7 +194,8 @@ def test_core_shim_validates_without_device(hq):
     A = np.zeros(64 * 64)
     assert L.CORE_dgeqrt(-1, 64, 32, _p(A), 64, _p(A), 64, _p(A), None) == -1
     assert L.CORE_dgeqrt(64, 64, 32, _p(A), 10, _p(A), 64, _p(A), None) == -5
-    assert L.CORE_dgeqrt(96, 96, 32, _p(A), 96, _p(A), 96, _p(A), None) == -1   # unsupported tile
+    assert L.CORE_dgeqrt(96, 80, 32, _p(A), 96, _p(A), 96, _p(A), None) == -1   # not square
+    assert L.CORE_dgeqrt(600, 600, 32, _p(A), 600, _p(A), 600, _p(A), None) == -1   # above 512
     assert L.CORE_dgeqrt(0, 64, 32, _p(A), 1, _p(A), 32, _p(A), None) == 0      # quick return
     assert L.CORE_dtsqrt(64, 64, 32, _p(A), 64, _p(A), 64, _p(A), 8, _p(A), None) == -9
     assert L.CORE_dormqr(142, 112, 64, 64, 64, 32, _p(A), 64, _p(A), 64, _p(A), 64, None, 64) == -1
