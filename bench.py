@@ -50,6 +50,8 @@ def parse():
                     help="precision of the trailing updates of the timed factorisation (the headline is fp64)")
     ap.add_argument("--no-panels", action="store_true", help="skip the whole-chip panel-kernel measurement")
     ap.add_argument("--no-library", action="store_true", help="skip the rocSOLVER / host LAPACK reference lines")
+    ap.add_argument("--no-tile-sizes", action="store_true",
+                    help="skip the factorisation with nb = 384 / 512 (generic kernels)")
     ap.add_argument("--only-factor", action="store_true",
                     help="only the timed factorisation and its check (profiling runs)")
     ap.add_argument("--only-panels", action="store_true", help="only the panel-kernel measurement (profiling runs)")
@@ -94,6 +96,56 @@ def cpu_baseline(nb, ib, seed):
     tl = time.perf_counter() - t0
     out["host_lapack_dgeqrf"] = {"value": round(4.0 / 3.0 * n ** 3 / tl / 1e12, 5), "unit": "TFLOP/s",
                                  "sample": f"numpy.linalg.qr(mode='r') of the same {n}x{n} matrix, {tl:.2f} s"}
+    return out
+
+
+def tile_size_bench(n, cases, seed, reps=2):
+    """The same n x n FP64 factorisation with other tile sizes (nb, ib): the
+    tiles nb = 384 and 512 run the generic kernels (16384 is not a multiple of
+    384: edge tiles, hq_tile_qr_mn_dev).  HIP events around each
+    factorisation, best of `reps` after one warm-up; rates use 4/3 n^3 of the
+    n x n matrix; column-norm check of R against A."""
+    import torch
+
+    import hybrid3_amd as hq
+    stream = torch.cuda.current_stream()
+    out = []
+    for nb, ib in cases:
+        MT = -(-n // nb)
+        pristine = torch.empty(MT * MT * nb * nb, dtype=torch.float64, device="cuda")
+        hq.fill_uniform_dev(MT, MT, nb, seed, pristine)
+        A = torch.empty_like(pristine)
+        TAU = torch.zeros(MT * MT * nb, dtype=torch.float64, device="cuda")
+        best = float("inf")
+        for _ in range(reps + 1):
+            A.copy_(pristine)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            hq.tile_qr_mn_dev(n, n, nb, ib, A, TAU)
+            e1.record(stream)
+            hq.sync()
+            torch.cuda.synchronize()
+            best = min(best, e0.elapsed_time(e1) * 1e-3)
+        # the call zeroed the padding of A; the pristine copy still holds it
+        live = torch.zeros(MT * nb, dtype=torch.bool, device="cuda")
+        live[:n] = True
+        a = pristine.view(MT, MT, nb, nb)
+        f = A.view(MT, MT, nb, nb)
+        tj = torch.arange(MT, device="cuda").view(MT, 1, 1, 1)
+        ti = torch.arange(MT, device="cuda").view(1, MT, 1, 1)
+        col = torch.arange(nb, device="cuda").view(1, 1, nb, 1)
+        row = torch.arange(nb, device="cuda").view(1, 1, 1, nb)
+        rowok = (ti * nb + row) < n
+        upper = (ti * nb + row) <= (tj * nb + col)
+        an = (a * a * rowok).sum(dim=(1, 3)).sqrt().reshape(-1)[:n]
+        rn = (f * f * upper).sum(dim=(1, 3)).sqrt().reshape(-1)[:n]
+        err = ((an - rn).abs() / an).max().item()
+        out.append({"nb": nb, "ib": ib, "ms": round(best * 1e3, 3),
+                    "tflops": round(4.0 / 3.0 * n ** 3 / best / 1e12, 3),
+                    "kernels": "specialised" if nb in (64, 128, 256) else "generic",
+                    "max_col_norm_rel_diff": err})
+        del pristine, A, TAU
+        torch.cuda.empty_cache()
     return out
 
 
@@ -516,6 +568,12 @@ def main():
         print(json.dumps({"prec": args.prec, "n": n, "nb": nb, "ib": ib, "ms_factorization": round(mean_ms, 3),
                           "tflops": round(tflops, 4), "check": check}))
         return
+    tile_sizes = None if args.no_tile_sizes else tile_size_bench(n, [(384, 32), (512, 32)], args.seed)
+    if tile_sizes is not None:
+        note("other tile sizes: " + ", ".join(f"nb={c['nb']} ib={c['ib']} {c['tflops']} TFLOP/s" for c in tile_sizes))
+        if not all(c["max_col_norm_rel_diff"] < 1e-12 for c in tile_sizes):
+            print(json.dumps({"error": "tile-size factorisation check failed", "tile_sizes": tile_sizes}))
+            sys.exit(3)
     library = None if args.no_library else rocsolver_reference(n, args.seed)
     if library is not None:
         note(f"rocSOLVER dgeqrf {n}x{n}: {library['ms']} ms, {library['tflops']} TFLOP/s")
@@ -583,6 +641,7 @@ def main():
                      "algorithmic_flops_per_launch": flops},
         "cpu_baseline": cpu,
         "library_reference": library,
+        "other_tile_sizes": tile_sizes,
         "panels": panels,
         "check": check,
         "batched": batched,

@@ -30,16 +30,16 @@ constexpr int LDP = NBMAX + 1; // panel column stride
 constexpr int LDT = IBM + 1;   // T / Gram / triangle column stride
 constexpr int CH = 64;         // rows per chunk of a block application
 constexpr int LDX = CH + 1;    // X chunk column stride
-constexpr int LDVC = IBM + 1;  // V chunk row stride
+constexpr int LDVC = IBM + 2;  // V chunk row stride (16-byte aligned rows)
 constexpr int NT = HQ_THREADS;
 
 struct Smem {
   union {
     double P[IBM * LDP];  // panel block: column c at P + c*LDP, row i = panel row i
     struct {
-      double V[CH * LDVC];     // V chunk: V(i, p) at V[i*LDVC + p]
-      double X[HQ_SLAB * LDX]; // X chunk: X(i, q) at X[q*LDX + i]
-      double W[IBM * HQ_SLAB]; // W / W2: W(p, q) at W[p*HQ_SLAB + q]
+      alignas(16) double V[CH * LDVC];     // V chunk: V(i, p) at V[i*LDVC + p]
+      alignas(16) double X[HQ_SLAB * LDX]; // X chunk: X(i, q) at X[q*LDX + i]
+      alignas(16) double W[IBM * HQ_SLAB]; // W / W2: W(p, q) at W[p*HQ_SLAB + q]
     } u;
   };
   double R[IBM * LDT];   // TSQRT: triangle block of R(k,k), R(p, c) at R[c*LDT + p]
@@ -203,15 +203,26 @@ __device__ void panel(Smem &S, double *A, double *A1, double *T, double *TAU, in
   __syncthreads();
 }
 
+// Entry (row, p) of the block's reflector matrix V (rows of the panel, i.e.
+// tile row r0 + row): the stored tile value, or the unit lower trapezoid's 1
+// and 0 for GEQRT / LARFB.
+template <bool TS>
+__device__ __forceinline__ double v_at(const double *Vt, int nb, int ii, int r0, int p, int row) {
+  if (TS || row > p) return Vt[(size_t)(ii + p) * nb + r0 + row];
+  return row == p ? 1.0 : 0.0;
+}
+
 // X <- H^T X for the block reflector H = I - V T V^T of the block at column ii
 // (kb reflectors of tile Vt, T block Tb with ld ib), on X's columns
 // c0 .. c1-1 (c1 - c0 <= 64):
 //   !TS: V unit lower trapezoidal (rows ii.. of Vt), X rows ii.. of tile X;
 //   TS:  [X1 rows ii..ii+kb; X] with V = [I; Vt columns], X all rows.
+// Each sweep over the 64-row chunks keeps the next chunk's global loads in
+// flight (registers) while the current chunk is computed from LDS.
 template <bool TS>
 __device__ void apply_block(Smem &S, const double *Vt, const double *Tb, int ib, double *X, double *X1, int nb,
                             int ii, int kb, int c0, int c1) {
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int r0 = TS ? 0 : ii;
   const int h = nb - r0;
   const int w = c1 - c0;
@@ -219,36 +230,54 @@ __device__ void apply_block(Smem &S, const double *Vt, const double *Tb, int ib,
     const int c = e / IBM, p = e - c * IBM;
     S.TT[c * LDT + p] = (c < kb && p <= c) ? Tb[(size_t)c * ib + p] : 0.0;
   }
+  // chunk loads: V(row, p) for p = wave + 4k (k < 8) and X(row, qq) for
+  // qq = wave + 4k (k < 16), row = b0 + lane; zero outside the block
+  double vreg[8], xreg[16];
+  auto load_v = [&](int b0) {
+    const int row = b0 + lane;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int p = wave + 4 * k;
+      vreg[k] = (p < kb && row < h) ? v_at<TS>(Vt, nb, ii, r0, p, row) : 0.0;
+    }
+  };
+  auto load_x = [&](int b0) {
+    const int row = b0 + lane;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      const int qq = wave + 4 * k;
+      xreg[k] = (qq < w && row < h) ? X[(size_t)(c0 + qq) * nb + r0 + row] : 0.0;
+    }
+  };
   // W = V^T X (+ X1): thread (q, pg) sums reflectors 8pg .. 8pg+7 of column q
-  const int q = tid & 63, pg = tid >> 6;
+  const int q = lane, pg = wave;
   double acc[8];
 #pragma unroll
   for (int pp = 0; pp < 8; pp++) acc[pp] = 0.0;
+  load_v(0);
+  load_x(0);
   for (int b0 = 0; b0 < h; b0 += CH) {
-    const int rows = min(CH, h - b0);
     __syncthreads();  // the previous chunk has been read
-    for (int e = tid; e < IBM * CH; e += NT) {
-      const int p = e / CH, i = e - p * CH;
-      double val = 0.0;
-      if (p < kb && i < rows) {
-        const int row = b0 + i;  // row of the panel (tile row r0 + row)
-        if (TS || row > p)
-          val = Vt[(size_t)(ii + p) * nb + r0 + row];
-        else
-          val = row == p ? 1.0 : 0.0;
-      }
-      S.u.V[i * LDVC + p] = val;
-    }
-    for (int e = tid; e < HQ_SLAB * CH; e += NT) {
-      const int qq = e / CH, i = e - qq * CH;
-      S.u.X[qq * LDX + i] = (qq < w && i < rows) ? X[(size_t)(c0 + qq) * nb + r0 + b0 + i] : 0.0;
-    }
-    __syncthreads();
-    for (int i = 0; i < rows; i++) {
-      const double x = S.u.X[q * LDX + i];
-      const double *vr = S.u.V + i * LDVC + 8 * pg;
 #pragma unroll
-      for (int pp = 0; pp < 8; pp++) acc[pp] += vr[pp] * x;
+    for (int k = 0; k < 8; k++) S.u.V[lane * LDVC + wave + 4 * k] = vreg[k];
+#pragma unroll
+    for (int k = 0; k < 16; k++) S.u.X[(wave + 4 * k) * LDX + lane] = xreg[k];
+    __syncthreads();
+    if (b0 + CH < h) {
+      load_v(b0 + CH);
+      load_x(b0 + CH);
+    }
+    // rows past the block were staged as zeros: a fixed trip count
+#pragma unroll 8
+    for (int i = 0; i < CH; i++) {
+      const double x = S.u.X[q * LDX + i];
+      const double2 *vr = reinterpret_cast<const double2 *>(S.u.V + i * LDVC + 8 * pg);
+#pragma unroll
+      for (int pp = 0; pp < 4; pp++) {
+        const double2 v2 = vr[pp];
+        acc[2 * pp] += v2.x * x;
+        acc[2 * pp + 1] += v2.y * x;
+      }
     }
   }
   if (TS && q < w) {
@@ -258,7 +287,7 @@ __device__ void apply_block(Smem &S, const double *Vt, const double *Tb, int ib,
       if (p < kb) acc[pp] += X1[(size_t)(c0 + q) * nb + ii + p];
     }
   }
-  __syncthreads();  // the last chunk has been read (W overlaps nothing, V / X do)
+  __syncthreads();  // the last chunk has been read
 #pragma unroll
   for (int pp = 0; pp < 8; pp++) S.u.W[(8 * pg + pp) * HQ_SLAB + q] = acc[pp];
   __syncthreads();
@@ -278,33 +307,52 @@ __device__ void apply_block(Smem &S, const double *Vt, const double *Tb, int ib,
     S.u.W[p * HQ_SLAB + q] = w2[pp];
     if (TS && q < w && p < kb) X1[(size_t)(c0 + q) * nb + ii + p] -= w2[pp];
   }
-  __syncthreads();
-  // X -= V W2: thread (row i of the chunk, 16-column group qg), its row of V
-  // in registers
-  const int i = tid & 63, qg = tid >> 6;
-  for (int b0 = 0; b0 < h; b0 += CH) {
+  // X -= V W2: thread (row i of the chunk, 16-column group qg) with its 16
+  // entries of X in registers (16 independent FMA chains); the V chunk
+  // staged column-major in LDS (lane-consecutive reads), W2 read as
+  // broadcast 16-byte LDS loads.  The next chunk's V and X loads are in
+  // flight during the arithmetic.
+  const int i = lane, qg = wave;
+  const int nq = min(16, w - 16 * qg);  // this thread's live columns (may be <= 0)
+  double *VT = S.u.X;                   // V chunk, V(i, p) at VT[p*LDX + i]
+  double xn[16];
+  auto load_rows = [&](int b0) {
     const int row = b0 + i;
-    if (row >= h) continue;
-    double vr[IBM];
 #pragma unroll
-    for (int p = 0; p < IBM; p++) {
-      double val = 0.0;
-      if (p < kb) {
-        if (TS || row > p)
-          val = Vt[(size_t)(ii + p) * nb + r0 + row];
-        else
-          val = row == p ? 1.0 : 0.0;
-      }
-      vr[p] = val;
+    for (int qq = 0; qq < 16; qq++)
+      xn[qq] = (qq < nq && row < h) ? X[(size_t)(c0 + 16 * qg + qq) * nb + r0 + row] : 0.0;
+  };
+  load_v(0);
+  load_rows(0);
+  for (int b0 = 0; b0 < h; b0 += CH) {
+    __syncthreads();  // W2 written / the previous chunk's V read
+#pragma unroll
+    for (int k = 0; k < 8; k++) VT[(wave + 4 * k) * LDX + lane] = vreg[k];
+    double x[16];
+#pragma unroll
+    for (int qq = 0; qq < 16; qq++) x[qq] = xn[qq];
+    __syncthreads();
+    const bool live = b0 + i < h && nq > 0;
+    if (b0 + CH < h) {
+      load_v(b0 + CH);
+      load_rows(b0 + CH);
     }
-    for (int qq = 0; qq < 16; qq++) {
-      const int qc = 16 * qg + qq;
-      if (qc >= w) break;
-      double *xp = X + (size_t)(c0 + qc) * nb + r0 + row;
-      double s = 0.0;
+#pragma unroll 4
+    for (int p = 0; p < IBM; p++) {
+      const double v = VT[p * LDX + i];
+      const double2 *wr = reinterpret_cast<const double2 *>(S.u.W + p * HQ_SLAB + 16 * qg);
 #pragma unroll
-      for (int p = 0; p < IBM; p++) s += vr[p] * S.u.W[p * HQ_SLAB + qc];
-      *xp -= s;
+      for (int qq = 0; qq < 8; qq++) {
+        const double2 w2v = wr[qq];
+        x[2 * qq] -= v * w2v.x;
+        x[2 * qq + 1] -= v * w2v.y;
+      }
+    }
+    if (live) {
+      double *xrow = X + (size_t)(c0 + 16 * qg) * nb + r0 + b0 + i;
+#pragma unroll
+      for (int qq = 0; qq < 16; qq++)
+        if (qq < nq) xrow[(size_t)qq * nb] = x[qq];
     }
   }
   __syncthreads();
