@@ -13,10 +13,12 @@
 //     pass applies the reflector;
 //   * a block reflector is applied to a slab of <= 64 columns by two sweeps
 //     over 64-row chunks: W = V^T X (X chunk and V chunk staged in LDS, each
-//     thread 8 reflectors of one column), then W2 = T^T W in LDS, then
-//     X -= V W2 (each thread one row and 16 columns, its row of V in
-//     registers).  The slab makes three passes over L2 per ib block instead
-//     of staying in registers, so these sizes run below the specialised ones.
+//     thread 4 reflectors of one column), then W2 = T^T W in LDS, then
+//     X -= V W2 (each thread one row and 8 columns); each sweep keeps the
+//     next chunk's loads in flight.  Workgroups of 8 waves (two per SIMD).
+//     The slab makes three passes over memory per ib block instead of
+//     staying in registers, so these sizes run well below the specialised
+//     ones (16384^2: about 9 TFLOP/s at nb = 384 / 512 against 46.7 at 256).
 // FP64 everywhere (reduced-precision updates exist for nb = 128 and 256).
 #include <hip/hip_runtime.h>
 
@@ -31,7 +33,13 @@ constexpr int LDT = IBM + 1;   // T / Gram / triangle column stride
 constexpr int CH = 64;         // rows per chunk of a block application
 constexpr int LDX = CH + 1;    // X chunk column stride
 constexpr int LDVC = IBM + 2;  // V chunk row stride (16-byte aligned rows)
-constexpr int NT = HQ_THREADS;
+constexpr int NT = 512;        // threads per workgroup: 8 waves, two per SIMD, to keep more loads in flight
+constexpr int NW = NT / 64;    // waves
+constexpr int PR = IBM / NW;   // reflectors per thread in W = V^T X
+constexpr int QC = HQ_SLAB / NW; // columns per thread in X -= V W2
+constexpr int RPT = (NBMAX + NT - 1) / NT;  // panel rows per thread
+static_assert(PR * NW == IBM && QC * NW == HQ_SLAB && PR % 2 == 0 && QC % 2 == 0, "thread mappings");
+static_assert(CH == 64, "chunk rows are the lanes of a wave");
 
 struct Smem {
   union {
@@ -46,7 +54,7 @@ struct Smem {
   double G[IBM * LDT];   // Gram v_p^T v_c at G[c*LDT + p]
   double TT[IBM * LDT];  // T(p, c) at TT[c*LDT + p]
   double tau[IBM];
-  double red[4 * IBM];   // per-wave partial sums of a column step
+  double red[NW * IBM];  // per-wave partial sums of a column step
   int task;
   hq_task next;
 };
@@ -114,12 +122,13 @@ __device__ void panel(Smem &S, double *A, double *A1, double *T, double *TAU, in
     const int i0 = TS ? 0 : c + 1;  // rows of the reflector's tail
     {
       // squared norm of the tail (slot j = c) and its dot products with the
-      // tails of the columns right of it (slots j > c): 8 row groups per slot
+      // tails of the columns right of it (slots j > c): NT/32 row groups per
+      // slot
       const int j = tid & 31, rg = tid >> 5;
       double s = 0.0;
       if (j >= c && j < kb) {
         const double *x = S.P + c * LDP, *y = S.P + j * LDP;
-        for (int i = i0 + rg; i < h; i += 8) s += x[i] * y[i];
+        for (int i = i0 + rg; i < h; i += NT / 32) s += x[i] * y[i];
       }
       s += __shfl_xor(s, 32, 64);
       if ((tid & 63) < 32) S.red[(tid >> 6) * IBM + j] = s;
@@ -127,7 +136,9 @@ __device__ void panel(Smem &S, double *A, double *A1, double *T, double *TAU, in
     __syncthreads();
     // dlarfg: beta = -sign(alpha) ||(alpha, x)||, tau = (beta - alpha) / beta,
     // v = x / (alpha - beta); a zero tail gives tau = 0 and leaves x as it is
-    const double sigma = S.red[c] + S.red[IBM + c] + S.red[2 * IBM + c] + S.red[3 * IBM + c];
+    double sigma = 0.0;
+#pragma unroll
+    for (int g = 0; g < NW; g++) sigma += S.red[g * IBM + c];
     const double alpha = TS ? S.R[c * LDT + c] : S.P[c * LDP + c];
     double tau = 0.0, beta = alpha, scale = 1.0;
     if (sigma != 0.0) {
@@ -137,25 +148,27 @@ __device__ void panel(Smem &S, double *A, double *A1, double *T, double *TAU, in
     }
     // the tail rows: v_i = x_i * scale, then y_j -= tau v_i w_j with
     // w_j = top_j + v^T y_j = top_j + scale d_j (top = row c of the panel)
-    double v[2];
-    int rows[2];
+    double v[RPT];
+    int rows[RPT];
 #pragma unroll
-    for (int r = 0; r < 2; r++) {
+    for (int r = 0; r < RPT; r++) {
       rows[r] = i0 + tid + r * NT;
       v[r] = rows[r] < h ? S.P[c * LDP + rows[r]] * scale : 0.0;
     }
     if (tau != 0.0) {
       for (int j = c + 1; j < kb; j++) {
-        const double d = S.red[j] + S.red[IBM + j] + S.red[2 * IBM + j] + S.red[3 * IBM + j];
+        double d = 0.0;
+#pragma unroll
+        for (int g = 0; g < NW; g++) d += S.red[g * IBM + j];
         const double top = TS ? S.R[j * LDT + c] : S.P[j * LDP + c];
         const double tw = tau * (top + scale * d);
 #pragma unroll
-        for (int r = 0; r < 2; r++)
+        for (int r = 0; r < RPT; r++)
           if (rows[r] < h) S.P[j * LDP + rows[r]] -= v[r] * tw;
       }
     }
 #pragma unroll
-    for (int r = 0; r < 2; r++)
+    for (int r = 0; r < RPT; r++)
       if (rows[r] < h) S.P[c * LDP + rows[r]] = v[r];
     __syncthreads();  // every thread has read row c (top) before it changes
     // row c: R(c, j) -= tau w_j right of the column, R(c, c) = beta
@@ -165,7 +178,9 @@ __device__ void panel(Smem &S, double *A, double *A1, double *T, double *TAU, in
       if (j == c) {
         *top = beta;
       } else if (tau != 0.0) {
-        const double d = S.red[j] + S.red[IBM + j] + S.red[2 * IBM + j] + S.red[3 * IBM + j];
+        double d = 0.0;
+#pragma unroll
+        for (int g = 0; g < NW; g++) d += S.red[g * IBM + j];
         *top -= tau * (*top + scale * d);
       }
     }
@@ -230,38 +245,39 @@ __device__ void apply_block(Smem &S, const double *Vt, const double *Tb, int ib,
     const int c = e / IBM, p = e - c * IBM;
     S.TT[c * LDT + p] = (c < kb && p <= c) ? Tb[(size_t)c * ib + p] : 0.0;
   }
-  // chunk loads: V(row, p) for p = wave + 4k (k < 8) and X(row, qq) for
-  // qq = wave + 4k (k < 16), row = b0 + lane; zero outside the block
-  double vreg[8], xreg[16];
+  // chunk loads: V(row, p) for p = wave + NW k (k < PR) and X(row, qq) for
+  // qq = wave + NW k (k < QC), row = b0 + lane; zero outside the block
+  double vreg[PR], xreg[QC];
   auto load_v = [&](int b0) {
     const int row = b0 + lane;
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-      const int p = wave + 4 * k;
+    for (int k = 0; k < PR; k++) {
+      const int p = wave + NW * k;
       vreg[k] = (p < kb && row < h) ? v_at<TS>(Vt, nb, ii, r0, p, row) : 0.0;
     }
   };
   auto load_x = [&](int b0) {
     const int row = b0 + lane;
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
-      const int qq = wave + 4 * k;
+    for (int k = 0; k < QC; k++) {
+      const int qq = wave + NW * k;
       xreg[k] = (qq < w && row < h) ? X[(size_t)(c0 + qq) * nb + r0 + row] : 0.0;
     }
   };
-  // W = V^T X (+ X1): thread (q, pg) sums reflectors 8pg .. 8pg+7 of column q
+  // W = V^T X (+ X1): thread (q, pg) sums reflectors PR pg .. PR pg + PR-1 of
+  // column q
   const int q = lane, pg = wave;
-  double acc[8];
+  double acc[PR];
 #pragma unroll
-  for (int pp = 0; pp < 8; pp++) acc[pp] = 0.0;
+  for (int pp = 0; pp < PR; pp++) acc[pp] = 0.0;
   load_v(0);
   load_x(0);
   for (int b0 = 0; b0 < h; b0 += CH) {
     __syncthreads();  // the previous chunk has been read
 #pragma unroll
-    for (int k = 0; k < 8; k++) S.u.V[lane * LDVC + wave + 4 * k] = vreg[k];
+    for (int k = 0; k < PR; k++) S.u.V[lane * LDVC + wave + NW * k] = vreg[k];
 #pragma unroll
-    for (int k = 0; k < 16; k++) S.u.X[(wave + 4 * k) * LDX + lane] = xreg[k];
+    for (int k = 0; k < QC; k++) S.u.X[(wave + NW * k) * LDX + lane] = xreg[k];
     __syncthreads();
     if (b0 + CH < h) {
       load_v(b0 + CH);
@@ -271,9 +287,9 @@ __device__ void apply_block(Smem &S, const double *Vt, const double *Tb, int ib,
 #pragma unroll 8
     for (int i = 0; i < CH; i++) {
       const double x = S.u.X[q * LDX + i];
-      const double2 *vr = reinterpret_cast<const double2 *>(S.u.V + i * LDVC + 8 * pg);
+      const double2 *vr = reinterpret_cast<const double2 *>(S.u.V + i * LDVC + PR * pg);
 #pragma unroll
-      for (int pp = 0; pp < 4; pp++) {
+      for (int pp = 0; pp < PR / 2; pp++) {
         const double2 v2 = vr[pp];
         acc[2 * pp] += v2.x * x;
         acc[2 * pp + 1] += v2.y * x;
@@ -282,55 +298,55 @@ __device__ void apply_block(Smem &S, const double *Vt, const double *Tb, int ib,
   }
   if (TS && q < w) {
 #pragma unroll
-    for (int pp = 0; pp < 8; pp++) {
-      const int p = 8 * pg + pp;
+    for (int pp = 0; pp < PR; pp++) {
+      const int p = PR * pg + pp;
       if (p < kb) acc[pp] += X1[(size_t)(c0 + q) * nb + ii + p];
     }
   }
   __syncthreads();  // the last chunk has been read
 #pragma unroll
-  for (int pp = 0; pp < 8; pp++) S.u.W[(8 * pg + pp) * HQ_SLAB + q] = acc[pp];
+  for (int pp = 0; pp < PR; pp++) S.u.W[(PR * pg + pp) * HQ_SLAB + q] = acc[pp];
   __syncthreads();
   // W2 = T^T W (W2(p, q) = sum_{r <= p} T(r, p) W(r, q)); TS: X1 -= W2
-  double w2[8];
+  double w2[PR];
 #pragma unroll
-  for (int pp = 0; pp < 8; pp++) {
-    const int p = 8 * pg + pp;
+  for (int pp = 0; pp < PR; pp++) {
+    const int p = PR * pg + pp;
     double s = 0.0;
     for (int r = 0; r <= p; r++) s += S.TT[p * LDT + r] * S.u.W[r * HQ_SLAB + q];
     w2[pp] = s;
   }
   __syncthreads();
 #pragma unroll
-  for (int pp = 0; pp < 8; pp++) {
-    const int p = 8 * pg + pp;
+  for (int pp = 0; pp < PR; pp++) {
+    const int p = PR * pg + pp;
     S.u.W[p * HQ_SLAB + q] = w2[pp];
     if (TS && q < w && p < kb) X1[(size_t)(c0 + q) * nb + ii + p] -= w2[pp];
   }
-  // X -= V W2: thread (row i of the chunk, 16-column group qg) with its 16
-  // entries of X in registers (16 independent FMA chains); the V chunk
-  // staged column-major in LDS (lane-consecutive reads), W2 read as
-  // broadcast 16-byte LDS loads.  The next chunk's V and X loads are in
-  // flight during the arithmetic.
+  // X -= V W2: thread (row i of the chunk, QC-column group qg) with its QC
+  // entries of X in registers (independent FMA chains); the V chunk staged
+  // column-major in LDS (lane-consecutive reads), W2 read as broadcast
+  // 16-byte LDS loads.  The next chunk's V and X loads are in flight during
+  // the arithmetic.
   const int i = lane, qg = wave;
-  const int nq = min(16, w - 16 * qg);  // this thread's live columns (may be <= 0)
+  const int nq = min(QC, w - QC * qg);  // this thread's live columns (may be <= 0)
   double *VT = S.u.X;                   // V chunk, V(i, p) at VT[p*LDX + i]
-  double xn[16];
+  double xn[QC];
   auto load_rows = [&](int b0) {
     const int row = b0 + i;
 #pragma unroll
-    for (int qq = 0; qq < 16; qq++)
-      xn[qq] = (qq < nq && row < h) ? X[(size_t)(c0 + 16 * qg + qq) * nb + r0 + row] : 0.0;
+    for (int qq = 0; qq < QC; qq++)
+      xn[qq] = (qq < nq && row < h) ? X[(size_t)(c0 + QC * qg + qq) * nb + r0 + row] : 0.0;
   };
   load_v(0);
   load_rows(0);
   for (int b0 = 0; b0 < h; b0 += CH) {
     __syncthreads();  // W2 written / the previous chunk's V read
 #pragma unroll
-    for (int k = 0; k < 8; k++) VT[(wave + 4 * k) * LDX + lane] = vreg[k];
-    double x[16];
+    for (int k = 0; k < PR; k++) VT[(wave + NW * k) * LDX + lane] = vreg[k];
+    double x[QC];
 #pragma unroll
-    for (int qq = 0; qq < 16; qq++) x[qq] = xn[qq];
+    for (int qq = 0; qq < QC; qq++) x[qq] = xn[qq];
     __syncthreads();
     const bool live = b0 + i < h && nq > 0;
     if (b0 + CH < h) {
@@ -340,18 +356,18 @@ __device__ void apply_block(Smem &S, const double *Vt, const double *Tb, int ib,
 #pragma unroll 4
     for (int p = 0; p < IBM; p++) {
       const double v = VT[p * LDX + i];
-      const double2 *wr = reinterpret_cast<const double2 *>(S.u.W + p * HQ_SLAB + 16 * qg);
+      const double2 *wr = reinterpret_cast<const double2 *>(S.u.W + p * HQ_SLAB + QC * qg);
 #pragma unroll
-      for (int qq = 0; qq < 8; qq++) {
+      for (int qq = 0; qq < QC / 2; qq++) {
         const double2 w2v = wr[qq];
         x[2 * qq] -= v * w2v.x;
         x[2 * qq + 1] -= v * w2v.y;
       }
     }
     if (live) {
-      double *xrow = X + (size_t)(c0 + 16 * qg) * nb + r0 + b0 + i;
+      double *xrow = X + (size_t)(c0 + QC * qg) * nb + r0 + b0 + i;
 #pragma unroll
-      for (int qq = 0; qq < 16; qq++)
+      for (int qq = 0; qq < QC; qq++)
         if (qq < nq) xrow[(size_t)qq * nb] = x[qq];
     }
   }
