@@ -103,7 +103,7 @@ def _addr(t):
     return t.data_ptr()
 
 
-def _cgls(Amul, ATmul, rsolve, rtsolve, b, max_iter, normA, patience=2):
+class _Cgls:
     """R-preconditioned CGLS on every column of b (device tensors with the
     right-hand sides in the last dimension; normA: ||A||_F, a scalar or one
     per problem broadcasting against (.., 1, k)).
@@ -120,68 +120,104 @@ def _cgls(Amul, ATmul, rsolve, rtsolve, b, max_iter, normA, patience=2):
     `patience` steps without progress (as LAPACK's refinement stops once the
     backward error stops falling by a fixed factor) -- counted once the
     update is below SMALL_UPDATE or after TRANSIENT steps -- or after max_iter
-    steps.  Converged means every
-    column's best eta is within the rounding floor of its FP64 residual,
-    eps (||b|| + ||A||_F ||x||) / ||r||.
-    Returns (x, steps, history, converged)."""
-    import torch
+    steps.  Converged means every column's best eta is within the rounding
+    floor of its FP64 residual, eps (||b|| + ||A||_F ||x||) / ||r||.
 
-    def norms(v):
-        return torch.linalg.vector_norm(v, dim=-2, keepdim=True)
+    step() enqueues one step and returns a 4-vector on the device (progress
+    anywhere, every update small, largest eta, largest relative update);
+    decide() takes it on the host and says whether to stop, so several
+    problems can step together with one host synchronisation per step."""
 
-    k = b.shape[-1]
-    x = rsolve(rtsolve(ATmul(b)))              # semi-normal start
-    r = b - Amul(x)
-    atr = ATmul(r)
-    s = rtsolve(atr)
-    p = s
-    gam = (s * s).sum(dim=-2, keepdim=True)
-    eta = norms(atr) / (normA * norms(r))
-    best_eta, best_x = eta, x
-    best_dx = None
-    hist, steps, since = [], 0, 0
-    for _ in range(max_iter):
-        t = rsolve(p)
-        AXT = Amul(torch.cat([x, t], dim=-1))
-        r = b - AXT[..., :k]
-        q = AXT[..., k:]
-        qq = (q * q).sum(dim=-2, keepdim=True)
-        alpha = torch.where(qq > 0, gam / torch.where(qq > 0, qq, torch.ones_like(qq)), torch.zeros_like(qq))
-        dx = alpha * t
-        x = x + dx
-        r = r - alpha * q
+    def __init__(self, Amul, ATmul, rsolve, rtsolve, b, max_iter, normA, patience=2):
+        self.Amul, self.ATmul, self.rsolve, self.rtsolve = Amul, ATmul, rsolve, rtsolve
+        self.b, self.normA, self.max_iter, self.patience = b, normA, max_iter, patience
+        self.k = b.shape[-1]
+        x = rsolve(rtsolve(ATmul(b)))              # semi-normal start
+        r = b - Amul(x)
         atr = ATmul(r)
         s = rtsolve(atr)
+        self.x, self.p = x, s
+        self.gam = (s * s).sum(dim=-2, keepdim=True)
+        eta = self._norms(atr) / (normA * self._norms(r))
+        self.best_eta, self.best_x, self.best_dx = eta, x, None
+        self.hist, self.steps, self.since, self.done = [], 0, 0, False
+
+    @staticmethod
+    def _norms(v):
+        import torch
+        return torch.linalg.vector_norm(v, dim=-2, keepdim=True)
+
+    def step(self):
+        import torch
+        k, norms = self.k, self._norms
+        t = self.rsolve(self.p)
+        AXT = self.Amul(torch.cat([self.x, t], dim=-1))
+        r = self.b - AXT[..., :k]
+        q = AXT[..., k:]
+        qq = (q * q).sum(dim=-2, keepdim=True)
+        alpha = torch.where(qq > 0, self.gam / torch.where(qq > 0, qq, torch.ones_like(qq)), torch.zeros_like(qq))
+        dx = alpha * t
+        self.x = self.x + dx
+        r = r - alpha * q
+        atr = self.ATmul(r)
+        s = self.rtsolve(atr)
         gn = (s * s).sum(dim=-2, keepdim=True)
+        gam = self.gam
         beta = torch.where(gam > 0, gn / torch.where(gam > 0, gam, torch.ones_like(gam)), torch.zeros_like(gam))
-        p = s + beta * p
-        gam = gn
-        steps += 1
-        eta = norms(atr) / (normA * norms(r))
-        nx = norms(x)
+        self.p = s + beta * self.p
+        self.gam = gn
+        self.steps += 1
+        eta = norms(atr) / (self.normA * norms(r))
+        nx = norms(self.x)
         rel = norms(dx) / torch.where(nx > 0, nx, torch.ones_like(eta))
-        progress = eta < PROGRESS * best_eta
-        if best_dx is not None:
-            progress = progress | (rel < PROGRESS * best_dx)
-        best_dx = rel if best_dx is None else torch.minimum(rel, best_dx)
-        better = eta < best_eta
-        best_x = torch.where(better, x, best_x)
-        best_eta = torch.where(better, eta, best_eta)
-        hist.append({"eta": eta.max().item(), "dx_rel": rel.max().item()})
+        progress = eta < PROGRESS * self.best_eta
+        if self.best_dx is not None:
+            progress = progress | (rel < PROGRESS * self.best_dx)
+        self.best_dx = rel if self.best_dx is None else torch.minimum(rel, self.best_dx)
+        better = eta < self.best_eta
+        self.best_x = torch.where(better, self.x, self.best_x)
+        self.best_eta = torch.where(better, eta, self.best_eta)
+        return torch.stack([progress.any().double(), (rel < SMALL_UPDATE).all().double(), eta.max(), rel.max()])
+
+    def decide(self, flags):
+        """flags: the host values of step()'s vector; True when finished."""
+        progress, small, eta_max, rel_max = flags
+        self.hist.append({"eta": float(eta_max), "dx_rel": float(rel_max)})
         # a step without progress counts toward the stop once the update is
         # small (at the rounding floor) or the start-up transient is over: with
         # factors far less accurate than 1/kappa, eta oscillates over the first
         # steps while x still moves by O(1)
-        if bool(progress.any()):
-            since = 0
-        elif bool((rel < SMALL_UPDATE).all()) or steps > TRANSIENT:
-            since += 1
-        if since >= patience:
-            break
-    r = b - Amul(best_x)
-    floor = EPS * (norms(b) + normA * norms(best_x)) / norms(r)
-    conv = bool((best_eta <= floor).all())
-    return best_x, steps, hist, conv
+        if progress:
+            self.since = 0
+        elif small or self.steps > TRANSIENT:
+            self.since += 1
+        self.done = self.since >= self.patience or self.steps >= self.max_iter
+        return self.done
+
+    def finish(self):
+        """(best x, steps, history, converged flag on the device)."""
+        norms = self._norms
+        r = self.b - self.Amul(self.best_x)
+        floor = EPS * (norms(self.b) + self.normA * norms(self.best_x)) / norms(r)
+        return self.best_x, self.steps, self.hist, (self.best_eta <= floor).all()
+
+
+def _cgls_many(solvers):
+    """Step every solver together until each has stopped: one host
+    synchronisation per step for all of them."""
+    import torch
+    live = [c for c in solvers if c.max_iter > 0]
+    while live:
+        flags = torch.stack([c.step() for c in live]).cpu().numpy()
+        live = [c for c, f in zip(live, flags) if not c.decide(f)]
+    out = [c.finish() for c in solvers]
+    conv = torch.stack([o[3] for o in out]).cpu().numpy() if out else []
+    return [(x, st, h, bool(cv)) for (x, st, h, _), cv in zip(out, conv)]
+
+
+def _cgls(Amul, ATmul, rsolve, rtsolve, b, max_iter, normA, patience=2):
+    """One problem (see _Cgls).  Returns (x, steps, history, converged)."""
+    return _cgls_many([_Cgls(Amul, ATmul, rsolve, rtsolve, b, max_iter, normA, patience)])[0]
 
 
 def _tri_inv(R, leaf=1024):
@@ -360,7 +396,9 @@ def batched_lstsq(groups, nb=256, ib=32, prec="fp64", max_iter=100):
     sync()
     torch.cuda.synchronize()
     t_fact = time.perf_counter() - t0
-    out, steps, conv = {}, {}, True
+    # every shape group's CGLS steps together (one host synchronisation per
+    # step for all groups)
+    solvers = []
     for g, sh in enumerate(shapes):
         m, n = sh
         A, b = groups[sh]
@@ -372,8 +410,11 @@ def batched_lstsq(groups, nb=256, ib=32, prec="fp64", max_iter=100):
         Ri = _tri_inv(R)            # preconditioner, explicit
         RiT = Ri.transpose(1, 2).contiguous()
         normA = torch.linalg.matrix_norm(A).view(B, 1, 1)
-        x, st, _, cv = _cgls(lambda v: torch.bmm(At, v), lambda v: torch.bmm(A, v), lambda c: torch.bmm(Ri, c),
-                             lambda c: torch.bmm(RiT, c), b, max_iter, normA)
+        solvers.append(_Cgls(lambda v, At=At: torch.bmm(At, v), lambda v, A=A: torch.bmm(A, v),
+                             lambda c, Ri=Ri: torch.bmm(Ri, c), lambda c, RiT=RiT: torch.bmm(RiT, c), b, max_iter,
+                             normA))
+    out, steps, conv = {}, {}, True
+    for sh, (x, st, _, cv) in zip(shapes, _cgls_many(solvers)):
         out[sh] = x
         steps[sh] = st
         conv = conv and cv
