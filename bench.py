@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--only-factor", action="store_true",
                     help="only the timed factorisation and its check (profiling runs)")
     ap.add_argument("--only-panels", action="store_true", help="only the panel-kernel measurement (profiling runs)")
+    ap.add_argument("--only-batched", action="store_true",
+                    help="only the batched uneven-QR measurement, FP64 and FP32 matrices (profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only a 2 GiB device copy (calibrates FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--traffic-file",
@@ -215,7 +217,7 @@ def panel_bench(nb=256, ib=32, count=2048, reps=3):
     return out
 
 
-def batched_bench(count, retune=False):
+def batched_bench(count, retune=False, check=True):
     """Batched uneven QR: `count` matrices, m ~ U[64,1024], n ~ U[32,m]
     (seed 4000), (nb, ib, panel mapping) plan from the committed tuning table
     (or measured by the autotuner), one persistent launch per group; against
@@ -237,20 +239,21 @@ def batched_bench(count, retune=False):
     bt = B.Batch(sizes, plan, seed0=1)
     B.time_factor(bt, reps=1)      # schedules built and cached
     t = B.time_factor(bt, reps=3)
-    err = bt.colnorm_error()
-    sample = list(range(0, count, 20))
-    res = bt.residuals(sample)
+    sample = list(range(0, count, 20)) if check else []
+    err = bt.colnorm_error() if check else float("nan")
+    res = bt.residuals(sample) or [(float("nan"), float("nan"))]
     tg = B.time_factor(bt, reps=3, graph=bt.capture())   # the same runs replayed from a hipGraph
-    err = max(err, bt.colnorm_error())
+    if check:
+        err = max(err, bt.colnorm_error())
     bt.release()
-    tp = B.time_factor(bt, reps=1, per_matrix=True)
+    tp = B.time_factor(bt, reps=1, per_matrix=True) if check else float("nan")
     fl = B.flops(sizes)
     del bt
     b32 = B.Batch(sizes, plan, seed0=1, fp32=True)
     B.time_factor(b32, reps=1)
     t32 = B.time_factor(b32, reps=3)
-    err32 = b32.colnorm_error()
-    res32 = b32.residuals(sample)
+    err32 = b32.colnorm_error() if check else float("nan")
+    res32 = b32.residuals(sample) or [(float("nan"), float("nan"))]
     del b32
     return {
         "workload": f"{count} FP64 matrices, m uniform in [64,1024], n uniform in [32,m], "
@@ -487,6 +490,13 @@ def main():
 
     if args.only_panels:
         print(json.dumps(panel_bench(args.nb, args.ib)))
+        return
+    if args.only_batched:
+        # (--no-check: no per-matrix checks and no per-matrix launches, for
+        # profiler runs that should see the batched launches only)
+        out = batched_bench(args.batch_count, args.retune, check=not args.no_check)
+        out.pop("tuner", None)
+        print(json.dumps(out))
         return
 
     n, nb, ib = args.n, args.nb, args.ib
