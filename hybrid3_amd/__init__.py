@@ -94,10 +94,10 @@ SIGNATURES = {
     "hq_batched_tile_qr_prec_dev": (C.c_int, [C.c_int] * 4 + [C.POINTER(C.c_int)] * 2 + [C.POINTER(_vp)] * 3
                                     + [_vp]),
     "hq_batched_prepare_dev": (C.c_int, [C.c_int] * 4 + [C.POINTER(C.c_int)] * 2),
-    "hq_batched_tile_qr_ex_dev": (C.c_int, [C.c_int] * 5 + [C.POINTER(C.c_int)] * 2 + [C.POINTER(_vp)] * 3
+    "hq_batched_tile_qr_ex_dev": (C.c_int, [C.c_int] * 5 + [C.POINTER(C.c_int)] * 3 + [C.POINTER(_vp)] * 3
                                   + [_vp]),
     "hq_batched_prepare_ex_dev": (C.c_int, [C.c_int] * 5 + [C.POINTER(C.c_int)] * 2),
-    "hq_batched_plan_ex_dev": (C.c_int, [C.c_int] * 4 + [C.POINTER(C.c_int)] * 2 + [C.POINTER(_vp)] * 3
+    "hq_batched_plan_ex_dev": (C.c_int, [C.c_int] * 4 + [C.POINTER(C.c_int)] * 3 + [C.POINTER(_vp)] * 3
                                + [C.POINTER(C.c_int)]),
     "hq_cm32_to_tile_dev": (C.c_int, [C.c_int] * 2 + [C.POINTER(C.c_int)] * 2
                             + [C.POINTER(_vp), C.POINTER(C.c_int), C.POINTER(_vp), _vp]),
@@ -245,12 +245,15 @@ def batched_tile_qr_prec_dev(prec, nb, ib, MT, NT, A, TAU, T=None, stream=None):
                                             _stream(stream)))
 
 
-def batched_tile_qr_ex_dev(prec, split_panels, nb, ib, MT, NT, A, TAU, T=None, stream=None):
-    """batched_tile_qr_prec_dev with the panel mapping: 0 one task per tile
-    panel QR, 1 one task per ib block plus per-slab trailing tasks."""
+def batched_tile_qr_ex_dev(prec, split_panels, nb, ib, MT, NT, A, TAU, T=None, stream=None, ncol=None):
+    """batched_tile_qr_prec_dev with the panel mapping (0 one task per tile
+    panel QR, 1 one task per ib block plus per-slab trailing tasks) and,
+    optionally, each matrix's column count (the rest of its last tile column
+    is zero padding, whose tasks are skipped)."""
     n = len(MT)
     Tp = None if T is None else _parr(T)
-    check(lib().hq_batched_tile_qr_ex_dev(prec, split_panels, n, nb, ib, _iarr(MT), _iarr(NT), _parr(A),
+    nc = None if ncol is None else _iarr(ncol)
+    check(lib().hq_batched_tile_qr_ex_dev(prec, split_panels, n, nb, ib, _iarr(MT), _iarr(NT), nc, _parr(A),
                                           _parr(TAU), Tp, _stream(stream)))
 
 
@@ -269,12 +272,13 @@ def tile_to_cm32_dev(nb, m, n, A, Cs, ldc, stream=None):
                                     _stream(stream)))
 
 
-def batched_plan_dev(nb, ib, MT, NT, A, TAU, T=None, split_panels=0):
+def batched_plan_dev(nb, ib, MT, NT, A, TAU, T=None, split_panels=0, ncol=None):
     """Prepare a batch for repeated (graph-capturable) runs; returns the plan id."""
     n = len(MT)
     Tp = None if T is None else _parr(T)
+    nc = None if ncol is None else _iarr(ncol)
     plan = C.c_int(0)
-    check(lib().hq_batched_plan_ex_dev(split_panels, n, nb, ib, _iarr(MT), _iarr(NT), _parr(A), _parr(TAU), Tp,
+    check(lib().hq_batched_plan_ex_dev(split_panels, n, nb, ib, _iarr(MT), _iarr(NT), nc, _parr(A), _parr(TAU), Tp,
                                        C.byref(plan)))
     return plan.value
 

@@ -139,7 +139,8 @@ class Batch:
                 m, n, a, c, ld = self._group_args(idx)
                 cm32_to_tile_dev(nb, m, n, c, ld, a)
             batched_tile_qr_ex_dev(0, mp, nb, ib, [self.tiles[i][0] for i in idx], [self.tiles[i][1] for i in idx],
-                                   [self.a_ptr[i] for i in idx], [self.t_ptr[i] for i in idx])
+                                   [self.a_ptr[i] for i in idx], [self.t_ptr[i] for i in idx],
+                                   ncol=[self.sizes[i][1] for i in idx])
             if self.fp32:
                 tile_to_cm32_dev(nb, m, n, a, c, ld)
 
@@ -156,7 +157,7 @@ class Batch:
             self.plans.append(batched_plan_dev(nb, ib, [self.tiles[i][0] for i in idx],
                                                [self.tiles[i][1] for i in idx],
                                                [self.a_ptr[i] for i in idx], [self.t_ptr[i] for i in idx],
-                                               split_panels=mp))
+                                               split_panels=mp, ncol=[self.sizes[i][1] for i in idx]))
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
             for p in self.plans:

@@ -804,28 +804,52 @@ __device__ __forceinline__ void panel_trailing(Smem<NB> &S, double *X, double *A
   }
 }
 
-// GEQRT = CORE_dgeqrt(nb, nb, ib, A, nb, T, ib, TAU)
+// The outputs of panel blocks wholly inside a batch matrix's zero padding
+// (columns ii.. of the tile, from column `live` on): the block's reflectors
+// are identities, so TAU = 0 and T = 0 for its columns, and for a diagonal
+// tile W = 0 and unit columns of V.
+template <int NB>
+__device__ __forceinline__ void pad_panel_blocks(double *T, double *TAU, int ib, int ii, double *Wt, double *Vd) {
+  for (int e = threadIdx.x; e < (NB - ii) * ib; e += HQ_THREADS) T[(size_t)ii * ib + e] = 0.0;
+  for (int c = ii + threadIdx.x; c < NB; c += HQ_THREADS) TAU[c] = 0.0;
+  if (Wt)
+    for (int e = threadIdx.x; e < (NB - ii) * NB; e += HQ_THREADS) {
+      const int c = ii + e / NB, r = e % NB;
+      Wt[(size_t)c * NB + r] = 0.0;
+      Vd[(size_t)c * NB + r] = r == c ? 1.0 : 0.0;
+    }
+}
+
+// GEQRT = CORE_dgeqrt(nb, nb, ib, A, nb, T, ib, TAU); columns from `live` on
+// are zero padding (pad_panel_blocks)
 template <int NB, int WB>
 __device__ __forceinline__ void task_geqrt(Smem<NB> &S, double *A, double *T, double *TAU, int ib,
                                            double *Wt = nullptr, double *Vd = nullptr,
-                                           unsigned short *VHt = nullptr) {
-  for (int ii = 0; ii < NB; ii += ib) {
+                                           unsigned short *VHt = nullptr, int live = NB) {
+  const int c1 = min(NB, (live + 15) & ~15);
+  int ii = 0;
+  for (; ii < live; ii += ib) {
     geqrt_panel<NB, WB>(S, A, T, TAU, ib, ii, Wt, Vd, VHt);
-    panel_trailing<NB, WB, false>(S, A, nullptr, ib, ii, ii + min(ib, NB - ii), NB);
+    panel_trailing<NB, WB, false>(S, A, nullptr, ib, ii, ii + min(ib, NB - ii), c1);
     __syncthreads();
   }
+  if (ii < NB) pad_panel_blocks<NB>(T, TAU, ib, ii, Wt, Vd);
 }
 
-// TSQRT = CORE_dtsqrt(nb, nb, ib, A1, nb, A2, nb, T, ib, TAU)
+// TSQRT = CORE_dtsqrt(nb, nb, ib, A1, nb, A2, nb, T, ib, TAU); columns from
+// `live` on are zero padding
 template <int NB, int WB>
 __device__ __forceinline__ void task_tsqrt(Smem<NB> &S, double *A1, double *A2, double *T, double *TAU, int ib,
-                                           unsigned short *VHt = nullptr) {
-  for (int ii = 0; ii < NB; ii += ib) {
+                                           unsigned short *VHt = nullptr, int live = NB) {
+  const int c1 = min(NB, (live + 15) & ~15);
+  int ii = 0;
+  for (; ii < live; ii += ib) {
     tsqrt_panel<NB, WB>(S, A1, A2, T, TAU, ib, ii, VHt);
-    panel_trailing<NB, WB, true>(S, A2, A1, ib, ii, ii + min(ib, NB - ii), NB);
+    panel_trailing<NB, WB, true>(S, A2, A1, ib, ii, ii + min(ib, NB - ii), c1);
     __syncthreads();
     HQ_PHASE(5);
   }
+  if (ii < NB) pad_panel_blocks<NB>(T, TAU, ib, ii, nullptr, nullptr);
 }
 
 // Split tasks of the DAG scheduler: panel b, and the trailing update of
@@ -2585,14 +2609,25 @@ __device__ __forceinline__ double *tau_ptr(const hq_geom &G, int i, int j) {
   return G.TAU + ((size_t)i + (size_t)j * G.MT) * (size_t)G.nb;
 }
 
+// Columns of tile column j that hold matrix entries (hq_geom::ncol).
+__device__ __forceinline__ int live_cols(const hq_geom &G, int j) {
+  return G.ncol > 0 ? min(G.nb, max(0, G.ncol - j * G.nb)) : G.nb;
+}
+
 // Returns the look-ahead of a slab task (see task_update_f), else (-1, 0).
+// Tasks wholly inside a batch matrix's zero padding write only the zeros
+// their outputs hold (pad_panel_blocks) or nothing.
 template <int NB, int WB, int PREC = 0>
 __device__ __forceinline__ int2 run_task(Smem<NB> &S, const hq_geom &G, const hq_task &t, Ahead ah = Ahead{}) {
   switch (t.kind) {
     case HQ_GEQRT:  // j = ib block of the panel, or -1: the whole tile
       if (t.j < 0)
         task_geqrt<NB, WB>(S, tile_ptr(G, t.k, t.k), t_ptr(G, t.k, t.k), tau_ptr(G, t.k, t.k), G.ib,
-                           wd_ptr(G, t.k), vd_ptr(G, t.k), PREC == 2 ? vh_ptr(G, t.k, t.k) : nullptr);
+                           wd_ptr(G, t.k), vd_ptr(G, t.k), PREC == 2 ? vh_ptr(G, t.k, t.k) : nullptr,
+                           live_cols(G, t.k));
+      else if (t.j * G.ib >= live_cols(G, t.k))
+        pad_panel_blocks<NB>(t_ptr(G, t.k, t.k), tau_ptr(G, t.k, t.k), G.ib, t.j * G.ib, wd_ptr(G, t.k),
+                             vd_ptr(G, t.k));
       else
         geqrt_panel<NB, WB>(S, tile_ptr(G, t.k, t.k), t_ptr(G, t.k, t.k), tau_ptr(G, t.k, t.k), G.ib,
                             t.j * G.ib, wd_ptr(G, t.k), vd_ptr(G, t.k), PREC == 2 ? vh_ptr(G, t.k, t.k) : nullptr);
@@ -2600,23 +2635,29 @@ __device__ __forceinline__ int2 run_task(Smem<NB> &S, const hq_geom &G, const hq
     case HQ_TSQRT:
       if (t.j < 0)
         task_tsqrt<NB, WB>(S, tile_ptr(G, t.k, t.k), tile_ptr(G, t.r, t.k), t_ptr(G, t.r, t.k),
-                           tau_ptr(G, t.r, t.k), G.ib, PREC == 2 ? vh_ptr(G, t.r, t.k) : nullptr);
+                           tau_ptr(G, t.r, t.k), G.ib, PREC == 2 ? vh_ptr(G, t.r, t.k) : nullptr, live_cols(G, t.k));
+      else if (t.j * G.ib >= live_cols(G, t.k))
+        pad_panel_blocks<NB>(t_ptr(G, t.r, t.k), tau_ptr(G, t.r, t.k), G.ib, t.j * G.ib, nullptr, nullptr);
       else
         tsqrt_panel<NB, WB>(S, tile_ptr(G, t.k, t.k), tile_ptr(G, t.r, t.k), t_ptr(G, t.r, t.k),
                             tau_ptr(G, t.r, t.k), G.ib, t.j * G.ib, PREC == 2 ? vh_ptr(G, t.r, t.k) : nullptr);
       break;
     case HQ_GEQRT_U:
-      task_geqrt_update<NB, WB>(S, tile_ptr(G, t.k, t.k), t_ptr(G, t.k, t.k), G.ib, t.j, t.slab);
+      if (t.j * G.ib < live_cols(G, t.k) && t.slab * HQ_SLAB < live_cols(G, t.k))
+        task_geqrt_update<NB, WB>(S, tile_ptr(G, t.k, t.k), t_ptr(G, t.k, t.k), G.ib, t.j, t.slab);
       break;
     case HQ_TSQRT_U:
-      task_tsqrt_update<NB, WB>(S, tile_ptr(G, t.k, t.k), tile_ptr(G, t.r, t.k), t_ptr(G, t.r, t.k), G.ib, t.j,
-                                t.slab);
+      if (t.j * G.ib < live_cols(G, t.k) && t.slab * HQ_SLAB < live_cols(G, t.k))
+        task_tsqrt_update<NB, WB>(S, tile_ptr(G, t.k, t.k), tile_ptr(G, t.r, t.k), t_ptr(G, t.r, t.k), G.ib, t.j,
+                                  t.slab);
       break;
     case HQ_LARFB:
+      if (t.slab * hq_upd_slab(NB, PREC) >= live_cols(G, t.j)) break;
       return task_larfb<NB, WB, PREC>(S, tile_ptr(G, t.k, t.k), t_ptr(G, t.k, t.k), xtile_ptr(G, t.k, t.j), G.ib,
                                       t.slab, wd_ptr(G, t.k), vd_ptr(G, t.k), ah, vh_ptr(G, t.k, t.k),
                                       ce_ptr(G, t.j));
     case HQ_SSRFB:
+      if (t.slab * hq_upd_slab(NB, PREC) >= live_cols(G, t.j)) break;
       return task_ssrfb<NB, WB, PREC>(S, xtile_ptr(G, t.k, t.j), xtile_ptr(G, t.r, t.j), tile_ptr(G, t.r, t.k),
                                       t_ptr(G, t.r, t.k), G.ib, t.slab, ah, vh_ptr(G, t.r, t.k), ce_ptr(G, t.j));
     default:

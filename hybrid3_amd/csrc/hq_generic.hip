@@ -375,71 +375,101 @@ __device__ void apply_block(Smem &S, const double *Vt, const double *Tb, int ib,
 }
 
 // The block's reflectors applied to every column right of it in the tile
-// (whole-tile GEQRT / TSQRT), 64 columns at a time.
+// up to column `live` (whole-tile GEQRT / TSQRT), 64 columns at a time.
 template <bool TS>
-__device__ void panel_rest(Smem &S, double *A, double *A1, const double *T, int nb, int ib, int ii) {
+__device__ void panel_rest(Smem &S, double *A, double *A1, const double *T, int nb, int ib, int ii, int live) {
   const int kb = min(ib, nb - ii);
-  for (int c0 = ii + kb; c0 < nb; c0 += HQ_SLAB)
-    apply_block<TS>(S, A, T + (size_t)ii * ib, ib, A, A1, nb, ii, kb, c0, min(nb, c0 + HQ_SLAB));
+  for (int c0 = ii + kb; c0 < live; c0 += HQ_SLAB)
+    apply_block<TS>(S, A, T + (size_t)ii * ib, ib, A, A1, nb, ii, kb, c0, min(live, c0 + HQ_SLAB));
 }
 
-// One task of the DAG (the task kinds and fields of hq_task).
+// Columns of tile column j that hold matrix entries (hq_geom::ncol).
+__device__ __forceinline__ int live_cols(const hq_geom &G, int j) {
+  return G.ncol > 0 ? min(G.nb, max(0, G.ncol - j * G.nb)) : G.nb;
+}
+
+// TAU = 0 and T = 0 for the panel blocks from column ii on, wholly inside a
+// batch matrix's zero padding (their reflectors are identities).
+__device__ void pad_panel_blocks(double *T, double *TAU, int nb, int ib, int ii) {
+  for (int e = threadIdx.x; e < (nb - ii) * ib; e += NT) T[(size_t)ii * ib + e] = 0.0;
+  for (int c = ii + threadIdx.x; c < nb; c += NT) TAU[c] = 0.0;
+}
+
+// One task of the DAG (the task kinds and fields of hq_task).  Tasks wholly
+// inside a batch matrix's zero padding write only the zeros their outputs
+// hold, or nothing; updates stop at the last column that holds entries.
 __device__ void run_task(Smem &S, const hq_geom &G, const hq_task &t) {
   const int nb = G.nb, ib = G.ib;
   switch (t.kind) {
     case HQ_GEQRT: {
       double *A = tile_ptr(G, t.k, t.k), *T = t_ptr(G, t.k, t.k), *TAU = tau_ptr(G, t.k, t.k);
+      const int live = live_cols(G, t.k);
       if (t.j >= 0) {
-        panel<false>(S, A, nullptr, T, TAU, nb, ib, t.j * ib);
+        if (t.j * ib >= live)
+          pad_panel_blocks(T, TAU, nb, ib, t.j * ib);
+        else
+          panel<false>(S, A, nullptr, T, TAU, nb, ib, t.j * ib);
       } else {
-        for (int ii = 0; ii < nb; ii += ib) {
+        int ii = 0;
+        for (; ii < live; ii += ib) {
           panel<false>(S, A, nullptr, T, TAU, nb, ib, ii);
-          panel_rest<false>(S, A, nullptr, T, nb, ib, ii);
+          panel_rest<false>(S, A, nullptr, T, nb, ib, ii, live);
         }
+        if (ii < nb) pad_panel_blocks(T, TAU, nb, ib, ii);
       }
       break;
     }
     case HQ_TSQRT: {
       double *A1 = tile_ptr(G, t.k, t.k), *A2 = tile_ptr(G, t.r, t.k);
       double *T = t_ptr(G, t.r, t.k), *TAU = tau_ptr(G, t.r, t.k);
+      const int live = live_cols(G, t.k);
       if (t.j >= 0) {
-        panel<true>(S, A2, A1, T, TAU, nb, ib, t.j * ib);
+        if (t.j * ib >= live)
+          pad_panel_blocks(T, TAU, nb, ib, t.j * ib);
+        else
+          panel<true>(S, A2, A1, T, TAU, nb, ib, t.j * ib);
       } else {
-        for (int ii = 0; ii < nb; ii += ib) {
+        int ii = 0;
+        for (; ii < live; ii += ib) {
           panel<true>(S, A2, A1, T, TAU, nb, ib, ii);
-          panel_rest<true>(S, A2, A1, T, nb, ib, ii);
+          panel_rest<true>(S, A2, A1, T, nb, ib, ii, live);
         }
+        if (ii < nb) pad_panel_blocks(T, TAU, nb, ib, ii);
       }
       break;
     }
     case HQ_GEQRT_U: {
       double *A = tile_ptr(G, t.k, t.k);
-      const int ii = t.j * ib, kb = min(ib, nb - ii);
-      const int c0 = max(t.slab * HQ_SLAB, ii + kb), c1 = min(nb, (t.slab + 1) * HQ_SLAB);
-      if (c0 < c1) apply_block<false>(S, A, t_ptr(G, t.k, t.k) + (size_t)ii * ib, ib, A, nullptr, nb, ii, kb, c0, c1);
+      const int ii = t.j * ib, kb = min(ib, nb - ii), live = live_cols(G, t.k);
+      const int c0 = max(t.slab * HQ_SLAB, ii + kb), c1 = min(live, (t.slab + 1) * HQ_SLAB);
+      if (ii < live && c0 < c1)
+        apply_block<false>(S, A, t_ptr(G, t.k, t.k) + (size_t)ii * ib, ib, A, nullptr, nb, ii, kb, c0, c1);
       break;
     }
     case HQ_TSQRT_U: {
       double *A1 = tile_ptr(G, t.k, t.k), *A2 = tile_ptr(G, t.r, t.k);
-      const int ii = t.j * ib, kb = min(ib, nb - ii);
-      const int c0 = max(t.slab * HQ_SLAB, ii + kb), c1 = min(nb, (t.slab + 1) * HQ_SLAB);
-      if (c0 < c1) apply_block<true>(S, A2, t_ptr(G, t.r, t.k) + (size_t)ii * ib, ib, A2, A1, nb, ii, kb, c0, c1);
+      const int ii = t.j * ib, kb = min(ib, nb - ii), live = live_cols(G, t.k);
+      const int c0 = max(t.slab * HQ_SLAB, ii + kb), c1 = min(live, (t.slab + 1) * HQ_SLAB);
+      if (ii < live && c0 < c1)
+        apply_block<true>(S, A2, t_ptr(G, t.r, t.k) + (size_t)ii * ib, ib, A2, A1, nb, ii, kb, c0, c1);
       break;
     }
     case HQ_LARFB: {
       const double *V = tile_ptr(G, t.k, t.k), *T = t_ptr(G, t.k, t.k);
       double *X = xtile_ptr(G, t.k, t.j);
-      const int uw = hq_upd_slab(nb, 0), c0 = t.slab * uw, c1 = min(nb, c0 + uw);
-      for (int ii = 0; ii < nb; ii += ib)
-        apply_block<false>(S, V, T + (size_t)ii * ib, ib, X, nullptr, nb, ii, min(ib, nb - ii), c0, c1);
+      const int uw = hq_upd_slab(nb, 0), c0 = t.slab * uw, c1 = min(live_cols(G, t.j), c0 + uw);
+      if (c0 < c1)
+        for (int ii = 0; ii < nb; ii += ib)
+          apply_block<false>(S, V, T + (size_t)ii * ib, ib, X, nullptr, nb, ii, min(ib, nb - ii), c0, c1);
       break;
     }
     case HQ_SSRFB: {
       const double *V = tile_ptr(G, t.r, t.k), *T = t_ptr(G, t.r, t.k);
       double *X1 = xtile_ptr(G, t.k, t.j), *X2 = xtile_ptr(G, t.r, t.j);
-      const int uw = hq_upd_slab(nb, 0), c0 = t.slab * uw, c1 = min(nb, c0 + uw);
-      for (int ii = 0; ii < nb; ii += ib)
-        apply_block<true>(S, V, T + (size_t)ii * ib, ib, X2, X1, nb, ii, min(ib, nb - ii), c0, c1);
+      const int uw = hq_upd_slab(nb, 0), c0 = t.slab * uw, c1 = min(live_cols(G, t.j), c0 + uw);
+      if (c0 < c1)
+        for (int ii = 0; ii < nb; ii += ib)
+          apply_block<true>(S, V, T + (size_t)ii * ib, ib, X2, X1, nb, ii, min(ib, nb - ii), c0, c1);
       break;
     }
     default:
@@ -535,14 +565,14 @@ __global__ __launch_bounds__(NT) void geqrt_kernel(double *A, double *T, double 
   __shared__ Smem S;
   for (int ii = 0; ii < nb; ii += ib) {
     panel<false>(S, A, nullptr, T, TAU, nb, ib, ii);
-    panel_rest<false>(S, A, nullptr, T, nb, ib, ii);
+    panel_rest<false>(S, A, nullptr, T, nb, ib, ii, nb);
   }
 }
 __global__ __launch_bounds__(NT) void tsqrt_kernel(double *A1, double *A2, double *T, double *TAU, int nb, int ib) {
   __shared__ Smem S;
   for (int ii = 0; ii < nb; ii += ib) {
     panel<true>(S, A2, A1, T, TAU, nb, ib, ii);
-    panel_rest<true>(S, A2, A1, T, nb, ib, ii);
+    panel_rest<true>(S, A2, A1, T, nb, ib, ii, nb);
   }
 }
 // grid.x = 64-column slab of C / of [A1; A2]

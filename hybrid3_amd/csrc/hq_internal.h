@@ -71,6 +71,13 @@ struct hq_geom {
   // factorisation; nullptr: each update task finds a scale from its own
   // columns (two sweeps over them).
   const int *CE;
+  // Columns of X (of A while factorising) that hold matrix entries: the rest
+  // of the last tile column is zero padding (a batch matrix of n columns);
+  // 0: every column.  Padding columns are zero and stay zero, and their
+  // reflectors are identities (tau = 0, T = 0), so tasks wholly inside the
+  // padding only write those zeros (and the diagonal tile's W = 0 and unit
+  // V columns) and skip their arithmetic.
+  int ncol;
 };
 
 // halves of split images per tile: one image of 64 nb halves per 16-reflector

@@ -730,7 +730,12 @@ int batch_args_ok(int count, int nb, int ib, const int *MT, const int *NT, doubl
 // are carved from Tb / Wb / VDb, reserved here.
 int batch_geoms(int count, int nb, int ib, const int *MT, const int *NT, double *const *A,
                 double *const *TAU, double *const *T, DevBuf &Tb, DevBuf &Wb, DevBuf &VDb,
-                std::vector<hq_geom> &g, int prec = 0, DevBuf *VHb = nullptr) {
+                std::vector<hq_geom> &g, int prec = 0, DevBuf *VHb = nullptr, const int *ncol = nullptr) {
+  if (ncol)
+    for (int i = 0; i < count; i++)
+      if (ncol[i] <= (NT[i] - 1) * nb || ncol[i] > NT[i] * nb)
+        return set_error(HQ_ERR_ARG, "matrix %d: %d columns do not end in its last tile column (NT=%d, nb=%d)", i,
+                         ncol[i], NT[i], nb);
   size_t tdoubles = 0;
   for (int i = 0; i < count; i++)
     if (!T || !T[i]) tdoubles += (size_t)MT[i] * NT[i] * ib * nb;
@@ -756,6 +761,7 @@ int batch_geoms(int count, int nb, int ib, const int *MT, const int *NT, double 
       toff += (size_t)MT[i] * NT[i] * ib * nb;
     }
     g[i] = hq_geom{A[i], TAU[i], Ti, MT[i], NT[i], nb, ib, A[i], NT[i], prec};
+    g[i].ncol = ncol ? ncol[i] : 0;
     if (prec == 2) {
       g[i].VH = (unsigned short *)VHb->p + vhoff;
       vhoff += hq_vh_tile(nb, ib) * MT[i] * NT[i];
@@ -770,7 +776,8 @@ int batch_geoms(int count, int nb, int ib, const int *MT, const int *NT, double 
 }
 
 int batched_qr_impl(int count, int nb, int ib, const int *MT, const int *NT, double *const *A,
-                    double *const *TAU, double *const *T, void *stream, int prec = 0, int split_panels = 0) {
+                    double *const *TAU, double *const *T, void *stream, int prec = 0, int split_panels = 0,
+                    const int *ncol = nullptr) {
   if (batch_args_ok(count, nb, ib, MT, NT, A, TAU) != HQ_OK) return g_err;
   if (prec < 0 || prec > 2) return set_error(HQ_ERR_ARG, "prec=%d (0 FP64, 1 FP32, 2 split)", prec);
   if (split_panels < 0 || split_panels > 1) return set_error(HQ_ERR_ARG, "panel mapping %d (0 or 1)", split_panels);
@@ -780,7 +787,8 @@ int batched_qr_impl(int count, int nb, int ib, const int *MT, const int *NT, dou
   hipStream_t st = pick_stream(c, stream);
   if (order_in(c, st) != HQ_OK) return g_err;
   std::vector<hq_geom> g;
-  if (batch_geoms(count, nb, ib, MT, NT, A, TAU, T, c->Tbatch, c->Wbatch, c->VDbatch, g, prec, &c->VHbatch) != HQ_OK)
+  if (batch_geoms(count, nb, ib, MT, NT, A, TAU, T, c->Tbatch, c->Wbatch, c->VDbatch, g, prec, &c->VHbatch,
+                  ncol) != HQ_OK)
     return g_err;
   CachedSched *cs;
   if (get_batch_schedule(c, count, nb, ib, MT, NT, &cs, prec, split_panels) != HQ_OK) return g_err;
@@ -807,7 +815,8 @@ struct BatchPlan {
 };
 
 int batched_plan_impl(int count, int nb, int ib, const int *MT, const int *NT, double *const *A,
-                      double *const *TAU, double *const *T, int *plan, int split_panels = 0) {
+                      double *const *TAU, double *const *T, int *plan, int split_panels = 0,
+                      const int *ncol = nullptr) {
   if (!plan) return set_error(HQ_ERR_ARG, "plan must not be NULL");
   if (split_panels < 0 || split_panels > 1) return set_error(HQ_ERR_ARG, "panel mapping %d (0 or 1)", split_panels);
   if (batch_args_ok(count, nb, ib, MT, NT, A, TAU) != HQ_OK) return g_err;
@@ -815,7 +824,7 @@ int batched_plan_impl(int count, int nb, int ib, const int *MT, const int *NT, d
   if (get_ctx(&c) != HQ_OK) return g_err;
   std::unique_ptr<BatchPlan> p(new BatchPlan());
   std::vector<hq_geom> g;
-  if (batch_geoms(count, nb, ib, MT, NT, A, TAU, T, p->T, p->W, p->VD, g) != HQ_OK) return g_err;
+  if (batch_geoms(count, nb, ib, MT, NT, A, TAU, T, p->T, p->W, p->VD, g, 0, nullptr, ncol) != HQ_OK) return g_err;
   if (get_batch_schedule(c, count, nb, ib, MT, NT, &p->cs, 0, split_panels) != HQ_OK) return g_err;
   if (p->geoms.reserve(sizeof(hq_geom) * (size_t)count) != HQ_OK ||
       p->flags.reserve(sizeof(int) * (size_t)p->cs->ntasks) != HQ_OK || p->counter.reserve(64) != HQ_OK)
@@ -1303,9 +1312,9 @@ int hq_batched_tile_qr_prec_dev(int prec, int count, int nb, int ib, const int *
 }
 
 int hq_batched_tile_qr_ex_dev(int prec, int split_panels, int count, int nb, int ib, const int *MT,
-                              const int *NT, double *const *A, double *const *TAU, double *const *T,
-                              void *stream) {
-  return batched_qr_impl(count, nb, ib, MT, NT, A, TAU, T, stream, prec, split_panels);
+                              const int *NT, const int *ncol, double *const *A, double *const *TAU,
+                              double *const *T, void *stream) {
+  return batched_qr_impl(count, nb, ib, MT, NT, A, TAU, T, stream, prec, split_panels, ncol);
 }
 
 int hq_batched_plan_dev(int count, int nb, int ib, const int *MT, const int *NT, double *const *A,
@@ -1314,8 +1323,8 @@ int hq_batched_plan_dev(int count, int nb, int ib, const int *MT, const int *NT,
 }
 
 int hq_batched_plan_ex_dev(int split_panels, int count, int nb, int ib, const int *MT, const int *NT,
-                           double *const *A, double *const *TAU, double *const *T, int *plan) {
-  return batched_plan_impl(count, nb, ib, MT, NT, A, TAU, T, plan, split_panels);
+                           const int *ncol, double *const *A, double *const *TAU, double *const *T, int *plan) {
+  return batched_plan_impl(count, nb, ib, MT, NT, A, TAU, T, plan, split_panels, ncol);
 }
 
 int hq_batched_run_dev(int plan, void *stream) {

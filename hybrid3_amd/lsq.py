@@ -32,7 +32,7 @@ from __future__ import annotations
 
 import time
 
-from . import (batched_prepare, batched_tile_qr_prec_dev, cm_to_tile_dev, sync, tile_ormqr_dev, tile_qr_dev,
+from . import (batched_prepare, batched_tile_qr_ex_dev, cm_to_tile_dev, sync, tile_ormqr_dev, tile_qr_dev,
                tile_qr_mixed_dev, tile_qr_prepare, tile_qr_split_dev, tile_to_cm_dev)
 
 __all__ = ["TileQR", "lstsq", "batched_lstsq", "backward_indicator", "gaussian_orthonormal", "svd_matrix",
@@ -371,7 +371,7 @@ def batched_lstsq(groups, nb=256, ib=32, prec="fp64", max_iter=100):
     import torch
     pv = PRECISIONS[prec]
     shapes = list(groups)
-    MT, NT, RY, TAU, cols, owners = [], [], [], [], [], []
+    MT, NT, NC, RY, TAU, cols, owners = [], [], [], [], [], [], []
     for sh in shapes:
         m, n = sh
         A, _ = groups[sh]
@@ -386,13 +386,14 @@ def batched_lstsq(groups, nb=256, ib=32, prec="fp64", max_iter=100):
         for i in range(B):
             MT.append(mt)
             NT.append(nt)
+            NC.append(n)
             cols.append(ry[i].data_ptr())
             owners.append(ta[i].data_ptr())
     batched_prepare(pv, nb, ib, MT, NT)   # schedule (cached)
     sync()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    batched_tile_qr_prec_dev(pv, nb, ib, MT, NT, cols, owners)
+    batched_tile_qr_ex_dev(pv, 0, nb, ib, MT, NT, cols, owners, ncol=NC)   # padding columns skipped
     sync()
     torch.cuda.synchronize()
     t_fact = time.perf_counter() - t0

@@ -224,17 +224,22 @@ int hq_batched_tile_qr_dev(int count, int nb, int ib, const int *MT, const int *
 int hq_batched_tile_qr_prec_dev(int prec, int count, int nb, int ib, const int *MT, const int *NT,
                                 double *const *A, double *const *TAU, double *const *T, void *stream);
 
-/* hq_batched_tile_qr_prec_dev with a choice of panel mapping (no counterpart
- * in the reference; the batch autotuner's wave-to-task dimension):
+/* hq_batched_tile_qr_prec_dev with a choice of panel mapping and the
+ * matrices' column counts (no counterpart in the reference; the batch
+ * autotuner's wave-to-task dimension):
  * split_panels 0 runs each GEQRT / TSQRT of a tile as one task (one
  * workgroup factors the whole nb-column panel and applies it to its own
  * tile), 1 as one task per ib block plus one trailing task per (block,
  * 64-column slab), spreading a panel over several workgroups as the
  * single-matrix entry points do.  Same results either way (same reflectors,
- * same update order per tile). */
+ * same update order per tile).
+ * ncol (NULL: none): ncol[i] columns of matrix i hold entries and the rest of
+ * its last tile column is zero padding ((NT[i]-1)*nb < ncol[i] <= NT[i]*nb);
+ * tasks wholly inside the padding then only write the zeros their outputs
+ * hold (TAU, T), and the results are bitwise those of the full run. */
 int hq_batched_tile_qr_ex_dev(int prec, int split_panels, int count, int nb, int ib, const int *MT,
-                              const int *NT, double *const *A, double *const *TAU, double *const *T,
-                              void *stream);
+                              const int *NT, const int *ncol, double *const *A, double *const *TAU,
+                              double *const *T, void *stream);
 
 /* Builds and caches the schedule of a batch (the shapes, nb, ib and update
  * precision of a later hq_batched_tile_qr_prec_dev call) ahead of it;
@@ -258,9 +263,10 @@ int hq_batched_prepare_ex_dev(int prec, int split_panels, int count, int nb, int
 int hq_batched_plan_dev(int count, int nb, int ib, const int *MT, const int *NT,
                         double *const *A, double *const *TAU, double *const *T,
                         int *plan);
-/* hq_batched_plan_dev with the panel mapping of hq_batched_tile_qr_ex_dev. */
+/* hq_batched_plan_dev with the panel mapping and column counts of
+ * hq_batched_tile_qr_ex_dev. */
 int hq_batched_plan_ex_dev(int split_panels, int count, int nb, int ib, const int *MT, const int *NT,
-                           double *const *A, double *const *TAU, double *const *T, int *plan);
+                           const int *ncol, double *const *A, double *const *TAU, double *const *T, int *plan);
 int hq_batched_run_dev(int plan, void *stream);
 int hq_batched_plan_check(int plan, void *stream);
 int hq_batched_plan_free(int plan);

@@ -93,6 +93,32 @@ def test_split_panel_mapping_matches_oracle(hq, orc, nb, ib):
         assert (a - b).abs().max().item() <= 1e-12 * b.abs().max().item()
 
 
+@pytest.mark.parametrize("nb,ib,prec", [(64, 16, 0), (128, 32, 0), (256, 32, 0), (96, 16, 0), (256, 32, 1),
+                                         (128, 32, 2)])
+def test_padding_skip_is_bitwise(hq, nb, ib, prec):
+    """With each matrix's column count given, the tasks wholly inside the
+    zero padding of its last tile column only write zeros: A, TAU and T are
+    bitwise those of the full run, for both panel mappings."""
+    import torch
+    sizes = SIZES + [(600, 300), (513, 40), (300, 257)]
+    print(f"batched padding skip nb={nb} ib={ib} prec={prec}", flush=True)
+    tiles, A0 = make_batch(hq, sizes, nb, 970)
+    MTs, NTs = [t[0] for t in tiles], [t[1] for t in tiles]
+    for mapping in (0, 1):
+        out = []
+        for ncol in (None, [n for _, n in sizes]):
+            A = [a.clone() for a in A0]
+            TAU = [torch.full((MT * NT * nb,), 5.0, dtype=torch.float64, device="cuda") for MT, NT in tiles]
+            T = [torch.full((MT * NT * ib * nb,), 7.0, dtype=torch.float64, device="cuda") for MT, NT in tiles]
+            hq.batched_tile_qr_ex_dev(prec, mapping, nb, ib, MTs, NTs, A, TAU, T, ncol=ncol)
+            hq.sync()
+            out.append((A, TAU, T))
+        for name, x, y in zip(("A", "TAU", "T"), out[0], out[1]):
+            assert all(torch.equal(a, b) for a, b in zip(x, y)), (name, mapping)
+    with pytest.raises(hq.HQError):   # a column count that does not end in the last tile column
+        hq.batched_tile_qr_ex_dev(0, 0, nb, ib, MTs[:1], NTs[:1], A0[:1], TAU[:1], ncol=[NTs[0] * nb + 1])
+
+
 def test_fp32_batch_is_the_rounded_fp64_factor(hq, orc):
     """FP32 matrices: widened to the FP64 tile layout, factored, rounded back.
     The stored FP32 factor is the oracle's FP64 factor of the FP32 inputs
