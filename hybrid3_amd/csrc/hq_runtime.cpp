@@ -558,7 +558,7 @@ int reserve_scratch(Context *c, hq_geom &G) {
 }
 
 int tile_qr_dev_impl(int MT, int NT, int nb, int ib, double *A, double *TAU, double *T,
-                     void *stream, bool serial, long long *trace = nullptr, int prec = 0) {
+                     void *stream, bool serial, long long *trace = nullptr, int prec = 0, int ncol = 0) {
   if (!geometry_ok(MT, NT, nb, ib)) return g_err;
   if (prec && !rp_nb(nb)) return set_error(HQ_ERR_ARG, "reduced-precision updates need nb = 128 or 256 (nb=%d)", nb);
   if (!A || !TAU) return set_error(HQ_ERR_ARG, "A and TAU must be device pointers");
@@ -573,6 +573,7 @@ int tile_qr_dev_impl(int MT, int NT, int nb, int ib, double *A, double *TAU, dou
   if (ensure_T(c, MT, NT, nb, ik, &T) != HQ_OK) return g_err;
   if (order_in(c, st) != HQ_OK) return g_err;
   hq_geom G{A, TAU, T, MT, NT, nb, ik, A, NT, prec};
+  G.ncol = ncol;
   if (reserve_scratch(c, G) != HQ_OK) return g_err;
   if (G.CE) HQ_HIP(hq_launch_colscale(A, MT, NT, nb, (int *)G.CE, st));
   if (serial) {
@@ -1060,7 +1061,8 @@ int hq_tile_qr_mn_dev(int m, int n, int nb, int ib, double *A, double *TAU, doub
   // padding entries are zero, so every reflector is zero in the padded rows
   // and the padded columns get tau = 0 (their reflector is the identity)
   if (m % nb || n % nb) HQ_HIP(hq_launch_pad_zero(A, MT, NT, nb, m, n, st));
-  return tile_qr_dev_impl(MT, NT, nb, ib, A, TAU, T, stream, false);
+  // tasks wholly inside the padding columns only write zeros (hq_geom::ncol)
+  return tile_qr_dev_impl(MT, NT, nb, ib, A, TAU, T, stream, false, nullptr, 0, n);
 }
 
 int hq_tile_qr_mixed_dev(int MT, int NT, int nb, int ib, double *A, double *TAU, double *T,
