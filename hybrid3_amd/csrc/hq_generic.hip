@@ -5,20 +5,18 @@
 // The task set, the DAG and the results are those of the specialised kernels
 // (flat-tree GEQRT / TSQRT / LARFB / SSRFB, PLASMA's T layout with ld = ib,
 // Hybrid_tile_QR.c:960-1363); only the data movement differs.  At these sizes
-// the nb x 16 strip of a 64-column slab does not fit a wave's registers next
-// to its operands (512 rows x 16 columns is 128 doubles per lane), so:
+// the nb x 16 strip of a 64-column slab does not fit one wave's registers next
+// to its operands (512 rows x 16 columns is 128 doubles per lane), and the
+// specialised kernels' LDS layout does not fit either, so:
 //   * a panel block (<= 32 columns of <= 512 rows) is factored in LDS: one
 //     pass per column computes the column's squared norm and its dot products
-//     with every column right of it at once (8 row groups per column), and one
+//     with every column right of it at once (16 row groups per column), and one
 //     pass applies the reflector;
-//   * a block reflector is applied to a slab of <= 64 columns by two sweeps
-//     over 64-row chunks: W = V^T X (X chunk and V chunk staged in LDS, each
-//     thread 4 reflectors of one column), then W2 = T^T W in LDS, then
-//     X -= V W2 (each thread one row and 8 columns); each sweep keeps the
-//     next chunk's loads in flight.  Workgroups of 8 waves (two per SIMD).
-//     The slab makes three passes over memory per ib block instead of
-//     staying in registers, so these sizes run well below the specialised
-//     ones (16384^2: about 9 TFLOP/s at nb = 384 / 512 against 46.7 at 256).
+//   * a block reflector is applied on the matrix cores with the slab held in
+//     registers for the whole task, two waves per 16-column strip (one per
+//     row half, 64 doubles per lane each) whose partial V^T X products are
+//     exchanged in LDS once per 16-reflector sub-block (update_strips).
+//     Workgroups of 8 waves, two per SIMD.
 // FP64 everywhere (reduced-precision updates exist for nb = 128 and 256).
 #include <hip/hip_runtime.h>
 
@@ -30,25 +28,18 @@ constexpr int NBMAX = 512;
 constexpr int IBM = HQ_IBMAX;  // reflectors per block (the run's ib <= 32)
 constexpr int LDP = NBMAX + 1; // panel column stride
 constexpr int LDT = IBM + 1;   // T / Gram / triangle column stride
-constexpr int CH = 64;         // rows per chunk of a block application
-constexpr int LDX = CH + 1;    // X chunk column stride
-constexpr int LDVC = IBM + 2;  // V chunk row stride (16-byte aligned rows)
-constexpr int NT = 512;        // threads per workgroup: 8 waves, two per SIMD, to keep more loads in flight
+constexpr int NT = 512;        // threads per workgroup: 8 waves, two per SIMD
 constexpr int NW = NT / 64;    // waves
-constexpr int PR = IBM / NW;   // reflectors per thread in W = V^T X
-constexpr int QC = HQ_SLAB / NW; // columns per thread in X -= V W2
 constexpr int RPT = (NBMAX + NT - 1) / NT;  // panel rows per thread
-static_assert(PR * NW == IBM && QC * NW == HQ_SLAB && PR % 2 == 0 && QC % 2 == 0, "thread mappings");
-static_assert(CH == 64, "chunk rows are the lanes of a wave");
+constexpr int LDV2 = NBMAX + 2;  // column stride of a staged 16-reflector sub-block of V
 
 struct Smem {
   union {
     double P[IBM * LDP];  // panel block: column c at P + c*LDP, row i = panel row i
     struct {
-      alignas(16) double V[CH * LDVC];     // V chunk: V(i, p) at V[i*LDVC + p]
-      alignas(16) double X[HQ_SLAB * LDX]; // X chunk: X(i, q) at X[q*LDX + i]
-      alignas(16) double W[IBM * HQ_SLAB]; // W / W2: W(p, q) at W[p*HQ_SLAB + q]
-    } u;
+      alignas(32) double Vs[16 * LDV2];   // sub-block of V: V(r, c) at Vs[c*LDV2 + r]
+      alignas(32) double Wx[2 * 4 * 256]; // partial W of each (row half, strip), MFMA layout
+    } m;
   };
   double R[IBM * LDT];   // TSQRT: triangle block of R(k,k), R(p, c) at R[c*LDT + p]
   double G[IBM * LDT];   // Gram v_p^T v_c at G[c*LDT + p]
@@ -218,157 +209,167 @@ __device__ void panel(Smem &S, double *A, double *A1, double *T, double *TAU, in
   __syncthreads();
 }
 
-// Entry (row, p) of the block's reflector matrix V (rows of the panel, i.e.
-// tile row r0 + row): the stored tile value, or the unit lower trapezoid's 1
-// and 0 for GEQRT / LARFB.
-template <bool TS>
-__device__ __forceinline__ double v_at(const double *Vt, int nb, int ii, int r0, int p, int row) {
-  if (TS || row > p) return Vt[(size_t)(ii + p) * nb + r0 + row];
-  return row == p ? 1.0 : 0.0;
-}
+typedef double d4 __attribute__((ext_vector_type(4)));
 
-// X <- H^T X for the block reflector H = I - V T V^T of the block at column ii
-// (kb reflectors of tile Vt, T block Tb with ld ib), on X's columns
-// c0 .. c1-1 (c1 - c0 <= 64):
-//   !TS: V unit lower trapezoidal (rows ii.. of Vt), X rows ii.. of tile X;
-//   TS:  [X1 rows ii..ii+kb; X] with V = [I; Vt columns], X all rows.
-// Each sweep over the 64-row chunks keeps the next chunk's global loads in
-// flight (registers) while the current chunk is computed from LDS.
+__device__ __forceinline__ d4 mfma(double a, double b, d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+// v_mfma_f64_16x16x4f64 lane layout (gfx950): A(i,k) at lane i + 16k,
+// B(k,j) at lane j + 16k, D(i,j) at lane j + 16*(i%4), register i/4.  An
+// A-operand row i fed from matrix row arow(i) = 4*(i%4) + i/4 makes register
+// v of lane group g hold matrix row 4g + v of the output, so a column strip
+// in accumulator form is four consecutive rows per lane, and the same
+// registers are the B operand of the next product with k index g.
+__device__ __forceinline__ int arow(int i) { return 4 * (i & 3) + (i >> 2); }
+
+// Rows of the strip a wave owns: half h of the tile's rows, in 16-row blocks.
+__device__ __forceinline__ int half_rows(int nb) { return (((nb + 1) >> 1) + 15) & ~15; }
+
+// X <- H_b^T X for the blocks b = b0 .. b1-1 in order (H_b = I - V_b T_b
+// V_b^T, V_b the reflectors at columns b*ib .. of tile Vt, T_b at
+// T + b*ib*ib with ld ib), on X's columns c0 .. c1-1 (c1 - c0 <= 64):
+//   !TS: V_b unit lower trapezoidal (diagonal at row b*ib + c), X the tile;
+//   TS:  [X1 rows of the block; X] with V_b = [I; Vt columns].
+// Register-resident, on the matrix cores: wave w owns the 16-column strip
+// w % 4 of the slab over row half w / 4, as MFMA accumulators for the whole
+// call.  Each block is applied as sub-blocks of 16 reflectors with the 16x16
+// diagonal blocks of T_b (the compact WY form of a product of two blocks),
+// V of a sub-block staged in LDS (the next one's loads in flight in
+// registers meanwhile):
+//   W = V^T X: each wave its row half; the two halves' partials exchanged in
+//     LDS and added (both waves of a strip hold the same W);
+//   W2 = T^T W (+ X1 rows first, TS; then X1 -= W2 by the upper half);
+//   X -= V W2 on the wave's row blocks.
 template <bool TS>
-__device__ void apply_block(Smem &S, const double *Vt, const double *Tb, int ib, double *X, double *X1, int nb,
-                            int ii, int kb, int c0, int c1) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int r0 = TS ? 0 : ii;
-  const int h = nb - r0;
-  const int w = c1 - c0;
-  for (int e = tid; e < IBM * IBM; e += NT) {
-    const int c = e / IBM, p = e - c * IBM;
-    S.TT[c * LDT + p] = (c < kb && p <= c) ? Tb[(size_t)c * ib + p] : 0.0;
-  }
-  // chunk loads: V(row, p) for p = wave + NW k (k < PR) and X(row, qq) for
-  // qq = wave + NW k (k < QC), row = b0 + lane; zero outside the block
-  double vreg[PR], xreg[QC];
-  auto load_v = [&](int b0) {
-    const int row = b0 + lane;
+__device__ void update_strips(Smem &S, const double *Vt, const double *T, int ib, double *X, double *X1, int nb,
+                              int b0, int b1, int c0, int c1) {
+  const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, g = lane >> 4;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int s = wave & 3, h = wave >> 2;
+  const int RH = half_rows(nb), rbase = h * RH;
+  const int nrb = max(0, min(RH, nb - rbase) + 15) >> 4;  // 16-row blocks of this half
+  const int col = c0 + 16 * s + m;
+  const bool colok = col < c1;
+  // !TS: rows above the first block's diagonal are neither read nor written
+  // (V is zero there; they hold R rows of earlier blocks, which tasks of the
+  // same step may be updating meanwhile)
+  const int row_lo = TS ? 0 : b0 * ib;
+  // the strip, rows rbase + 16 rb + 4g + v at acc[rb][v]
+  d4 acc[NBMAX / 32];
 #pragma unroll
-    for (int k = 0; k < PR; k++) {
-      const int p = wave + NW * k;
-      vreg[k] = (p < kb && row < h) ? v_at<TS>(Vt, nb, ii, r0, p, row) : 0.0;
-    }
-  };
-  auto load_x = [&](int b0) {
-    const int row = b0 + lane;
+  for (int rb = 0; rb < NBMAX / 32; rb++) {
+    acc[rb] = d4{0.0, 0.0, 0.0, 0.0};
+    if (rb < nrb && colok) {
+      const int r = rbase + 16 * rb + 4 * g;
+      const double *xp = X + (size_t)col * nb + r;
+      if (r >= row_lo && r + 3 < nb) {
+        acc[rb] = *reinterpret_cast<const d4 *>(xp);
+      } else {
 #pragma unroll
-    for (int k = 0; k < QC; k++) {
-      const int qq = wave + NW * k;
-      xreg[k] = (qq < w && row < h) ? X[(size_t)(c0 + qq) * nb + r0 + row] : 0.0;
-    }
-  };
-  // W = V^T X (+ X1): thread (q, pg) sums reflectors PR pg .. PR pg + PR-1 of
-  // column q
-  const int q = lane, pg = wave;
-  double acc[PR];
-#pragma unroll
-  for (int pp = 0; pp < PR; pp++) acc[pp] = 0.0;
-  load_v(0);
-  load_x(0);
-  for (int b0 = 0; b0 < h; b0 += CH) {
-    __syncthreads();  // the previous chunk has been read
-#pragma unroll
-    for (int k = 0; k < PR; k++) S.u.V[lane * LDVC + wave + NW * k] = vreg[k];
-#pragma unroll
-    for (int k = 0; k < QC; k++) S.u.X[(wave + NW * k) * LDX + lane] = xreg[k];
-    __syncthreads();
-    if (b0 + CH < h) {
-      load_v(b0 + CH);
-      load_x(b0 + CH);
-    }
-    // rows past the block were staged as zeros: a fixed trip count
-#pragma unroll 8
-    for (int i = 0; i < CH; i++) {
-      const double x = S.u.X[q * LDX + i];
-      const double2 *vr = reinterpret_cast<const double2 *>(S.u.V + i * LDVC + PR * pg);
-#pragma unroll
-      for (int pp = 0; pp < PR / 2; pp++) {
-        const double2 v2 = vr[pp];
-        acc[2 * pp] += v2.x * x;
-        acc[2 * pp + 1] += v2.y * x;
+        for (int v = 0; v < 4; v++)
+          if (r + v >= row_lo && r + v < nb) acc[rb][v] = xp[v];
       }
     }
   }
-  if (TS && q < w) {
+  // sub-blocks: (block b, first reflector q0 of the block) in order
+  const int nsb_per = (ib + 15) / 16;
+  const int nsb = (b1 - b0) * nsb_per;
+  // V of a sub-block: 16 columns of LDV2 rows (zero past nb), NT threads x
+  // VPT doubles
+  constexpr int VPT = (16 * LDV2 + NT - 1) / NT;
+  double vpre[VPT];
+  auto load_v = [&](int sbi) {
+    const int b = b0 + sbi / nsb_per, q0 = (sbi % nsb_per) * 16;
+    const int ii = b * ib, kb = min(ib, nb - ii);
 #pragma unroll
-    for (int pp = 0; pp < PR; pp++) {
-      const int p = PR * pg + pp;
-      if (p < kb) acc[pp] += X1[(size_t)(c0 + q) * nb + ii + p];
+    for (int k = 0; k < VPT; k++) {
+      const int e = tid + k * NT, c = e / LDV2, r = e - c * LDV2;
+      const int q = q0 + c, d = ii + q;  // reflector index in the block, its diagonal row
+      double val = 0.0;
+      if (e < 16 * LDV2 && q < kb && r < nb) {
+        if (TS || r > d)
+          val = Vt[(size_t)d * nb + r];
+        else
+          val = r == d ? 1.0 : 0.0;
+      }
+      vpre[k] = val;
     }
-  }
-  __syncthreads();  // the last chunk has been read
-#pragma unroll
-  for (int pp = 0; pp < PR; pp++) S.u.W[(PR * pg + pp) * HQ_SLAB + q] = acc[pp];
-  __syncthreads();
-  // W2 = T^T W (W2(p, q) = sum_{r <= p} T(r, p) W(r, q)); TS: X1 -= W2
-  double w2[PR];
-#pragma unroll
-  for (int pp = 0; pp < PR; pp++) {
-    const int p = PR * pg + pp;
-    double s = 0.0;
-    for (int r = 0; r <= p; r++) s += S.TT[p * LDT + r] * S.u.W[r * HQ_SLAB + q];
-    w2[pp] = s;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int pp = 0; pp < PR; pp++) {
-    const int p = PR * pg + pp;
-    S.u.W[p * HQ_SLAB + q] = w2[pp];
-    if (TS && q < w && p < kb) X1[(size_t)(c0 + q) * nb + ii + p] -= w2[pp];
-  }
-  // X -= V W2: thread (row i of the chunk, QC-column group qg) with its QC
-  // entries of X in registers (independent FMA chains); the V chunk staged
-  // column-major in LDS (lane-consecutive reads), W2 read as broadcast
-  // 16-byte LDS loads.  The next chunk's V and X loads are in flight during
-  // the arithmetic.
-  const int i = lane, qg = wave;
-  const int nq = min(QC, w - QC * qg);  // this thread's live columns (may be <= 0)
-  double *VT = S.u.X;                   // V chunk, V(i, p) at VT[p*LDX + i]
-  double xn[QC];
-  auto load_rows = [&](int b0) {
-    const int row = b0 + i;
-#pragma unroll
-    for (int qq = 0; qq < QC; qq++)
-      xn[qq] = (qq < nq && row < h) ? X[(size_t)(c0 + QC * qg + qq) * nb + r0 + row] : 0.0;
   };
-  load_v(0);
-  load_rows(0);
-  for (int b0 = 0; b0 < h; b0 += CH) {
-    __syncthreads();  // W2 written / the previous chunk's V read
-#pragma unroll
-    for (int k = 0; k < PR; k++) VT[(wave + NW * k) * LDX + lane] = vreg[k];
-    double x[QC];
-#pragma unroll
-    for (int qq = 0; qq < QC; qq++) x[qq] = xn[qq];
-    __syncthreads();
-    const bool live = b0 + i < h && nq > 0;
-    if (b0 + CH < h) {
-      load_v(b0 + CH);
-      load_rows(b0 + CH);
+  if (nsb > 0) load_v(0);
+  for (int sbi = 0; sbi < nsb; sbi++) {
+    const int b = b0 + sbi / nsb_per, q0 = (sbi % nsb_per) * 16;
+    const int ii = b * ib, kb = min(ib, nb - ii);
+    const int nq = min(16, kb - q0);  // reflectors of this sub-block
+    if (nq <= 0) {                    // past the last (narrower) block's end; uniform
+      if (sbi + 1 < nsb) load_v(sbi + 1);
+      continue;
     }
-#pragma unroll 4
-    for (int p = 0; p < IBM; p++) {
-      const double v = VT[p * LDX + i];
-      const double2 *wr = reinterpret_cast<const double2 *>(S.u.W + p * HQ_SLAB + QC * qg);
+    __syncthreads();                  // the previous sub-block's V and W are read
 #pragma unroll
-      for (int qq = 0; qq < QC / 2; qq++) {
-        const double2 w2v = wr[qq];
-        x[2 * qq] -= v * w2v.x;
-        x[2 * qq + 1] -= v * w2v.y;
+    for (int k = 0; k < VPT; k++)
+      if (tid + k * NT < 16 * LDV2) S.m.Vs[tid + k * NT] = vpre[k];
+    for (int e = tid; e < 16 * 16; e += NT) {
+      const int c = e >> 4, p = e & 15;
+      S.TT[c * LDT + p] = (c < nq && p <= c) ? T[(size_t)ii * ib + (size_t)(q0 + c) * ib + q0 + p] : 0.0;
+    }
+    __syncthreads();
+    if (sbi + 1 < nsb) load_v(sbi + 1);  // in flight during the products below
+    const double *Vs = S.m.Vs;
+    // W partial over this half's rows: A(i, k) = V(row_k, i), B(k, j) = X(row_k, j)
+    d4 wp = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int rb = 0; rb < NBMAX / 32; rb++) {
+      if (rb < nrb) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) wp = mfma(Vs[m * LDV2 + rbase + 16 * rb + 4 * g + v], acc[rb][v], wp);
       }
     }
-    if (live) {
-      double *xrow = X + (size_t)(c0 + QC * qg) * nb + r0 + b0 + i;
+    // TS: the block's rows of X1 (read before the exchange barrier; the upper
+    // half writes them after it)
+    d4 x1 = d4{0.0, 0.0, 0.0, 0.0};
+    if (TS && colok) {
 #pragma unroll
-      for (int qq = 0; qq < QC; qq++)
-        if (qq < nq) xrow[(size_t)qq * nb] = x[qq];
+      for (int v = 0; v < 4; v++)
+        if (4 * v + g < nq) x1[v] = X1[(size_t)col * nb + ii + q0 + 4 * v + g];
+    }
+    *reinterpret_cast<d4 *>(S.m.Wx + (h * 4 + s) * 256 + 4 * lane) = wp;
+    __syncthreads();
+    const d4 wo = *reinterpret_cast<const d4 *>(S.m.Wx + ((1 - h) * 4 + s) * 256 + 4 * lane);
+    d4 w = wp + wo;
+    if (TS) w += x1;
+    // W2 = T^T W: A(i, k) = T(k, i), B = W (D layout = B layout per k step)
+    d4 w2 = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) w2 = mfma(S.TT[m * LDT + 4 * kk + g], w[kk], w2);
+    if (TS && h == 0 && colok) {
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        if (4 * v + g < nq) X1[(size_t)col * nb + ii + q0 + 4 * v + g] = x1[v] - w2[v];
+    }
+    const d4 nw2 = -w2;
+    // X -= V W2 on this half's row blocks: A(i, k) = V(rbase + 16 rb + arow(i), k)
+#pragma unroll
+    for (int rb = 0; rb < NBMAX / 32; rb++) {
+      if (rb < nrb) {
+#pragma unroll
+        for (int kk = 0; kk < 4; kk++)
+          acc[rb] = mfma(Vs[(4 * kk + g) * LDV2 + rbase + 16 * rb + arow(m)], nw2[kk], acc[rb]);
+      }
+    }
+  }
+  // store the strip
+#pragma unroll
+  for (int rb = 0; rb < NBMAX / 32; rb++) {
+    if (rb < nrb && colok) {
+      const int r = rbase + 16 * rb + 4 * g;
+      double *xp = X + (size_t)col * nb + r;
+      if (r >= row_lo && r + 3 < nb) {
+        *reinterpret_cast<d4 *>(xp) = acc[rb];
+      } else {
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+          if (r + v >= row_lo && r + v < nb) xp[v] = acc[rb][v];
+      }
     }
   }
   __syncthreads();
@@ -380,7 +381,7 @@ template <bool TS>
 __device__ void panel_rest(Smem &S, double *A, double *A1, const double *T, int nb, int ib, int ii, int live) {
   const int kb = min(ib, nb - ii);
   for (int c0 = ii + kb; c0 < live; c0 += HQ_SLAB)
-    apply_block<TS>(S, A, T + (size_t)ii * ib, ib, A, A1, nb, ii, kb, c0, min(live, c0 + HQ_SLAB));
+    update_strips<TS>(S, A, T, ib, A, A1, nb, ii / ib, ii / ib + 1, c0, min(live, c0 + HQ_SLAB));
 }
 
 // Columns of tile column j that hold matrix entries (hq_geom::ncol).
@@ -443,7 +444,7 @@ __device__ void run_task(Smem &S, const hq_geom &G, const hq_task &t) {
       const int ii = t.j * ib, kb = min(ib, nb - ii), live = live_cols(G, t.k);
       const int c0 = max(t.slab * HQ_SLAB, ii + kb), c1 = min(live, (t.slab + 1) * HQ_SLAB);
       if (ii < live && c0 < c1)
-        apply_block<false>(S, A, t_ptr(G, t.k, t.k) + (size_t)ii * ib, ib, A, nullptr, nb, ii, kb, c0, c1);
+        update_strips<false>(S, A, t_ptr(G, t.k, t.k), ib, A, nullptr, nb, t.j, t.j + 1, c0, c1);
       break;
     }
     case HQ_TSQRT_U: {
@@ -451,25 +452,21 @@ __device__ void run_task(Smem &S, const hq_geom &G, const hq_task &t) {
       const int ii = t.j * ib, kb = min(ib, nb - ii), live = live_cols(G, t.k);
       const int c0 = max(t.slab * HQ_SLAB, ii + kb), c1 = min(live, (t.slab + 1) * HQ_SLAB);
       if (ii < live && c0 < c1)
-        apply_block<true>(S, A2, t_ptr(G, t.r, t.k) + (size_t)ii * ib, ib, A2, A1, nb, ii, kb, c0, c1);
+        update_strips<true>(S, A2, t_ptr(G, t.r, t.k), ib, A2, A1, nb, t.j, t.j + 1, c0, c1);
       break;
     }
     case HQ_LARFB: {
       const double *V = tile_ptr(G, t.k, t.k), *T = t_ptr(G, t.k, t.k);
       double *X = xtile_ptr(G, t.k, t.j);
       const int uw = hq_upd_slab(nb, 0), c0 = t.slab * uw, c1 = min(live_cols(G, t.j), c0 + uw);
-      if (c0 < c1)
-        for (int ii = 0; ii < nb; ii += ib)
-          apply_block<false>(S, V, T + (size_t)ii * ib, ib, X, nullptr, nb, ii, min(ib, nb - ii), c0, c1);
+      if (c0 < c1) update_strips<false>(S, V, T, ib, X, nullptr, nb, 0, (nb + ib - 1) / ib, c0, c1);
       break;
     }
     case HQ_SSRFB: {
       const double *V = tile_ptr(G, t.r, t.k), *T = t_ptr(G, t.r, t.k);
       double *X1 = xtile_ptr(G, t.k, t.j), *X2 = xtile_ptr(G, t.r, t.j);
       const int uw = hq_upd_slab(nb, 0), c0 = t.slab * uw, c1 = min(live_cols(G, t.j), c0 + uw);
-      if (c0 < c1)
-        for (int ii = 0; ii < nb; ii += ib)
-          apply_block<true>(S, V, T + (size_t)ii * ib, ib, X2, X1, nb, ii, min(ib, nb - ii), c0, c1);
+      if (c0 < c1) update_strips<true>(S, V, T, ib, X2, X1, nb, 0, (nb + ib - 1) / ib, c0, c1);
       break;
     }
     default:
@@ -579,15 +576,13 @@ __global__ __launch_bounds__(NT) void tsqrt_kernel(double *A1, double *A2, doubl
 __global__ __launch_bounds__(NT) void larfb_kernel(const double *V, const double *T, double *C, int nb, int ib) {
   __shared__ Smem S;
   const int c0 = blockIdx.x * HQ_SLAB, c1 = min(nb, c0 + HQ_SLAB);
-  for (int ii = 0; ii < nb; ii += ib)
-    apply_block<false>(S, V, T + (size_t)ii * ib, ib, C, nullptr, nb, ii, min(ib, nb - ii), c0, c1);
+  update_strips<false>(S, V, T, ib, C, nullptr, nb, 0, (nb + ib - 1) / ib, c0, c1);
 }
 __global__ __launch_bounds__(NT) void ssrfb_kernel(double *A1, double *A2, const double *V, const double *T, int nb,
                                                    int ib) {
   __shared__ Smem S;
   const int c0 = blockIdx.x * HQ_SLAB, c1 = min(nb, c0 + HQ_SLAB);
-  for (int ii = 0; ii < nb; ii += ib)
-    apply_block<true>(S, V, T + (size_t)ii * ib, ib, A2, A1, nb, ii, min(ib, nb - ii), c0, c1);
+  update_strips<true>(S, V, T, ib, A2, A1, nb, 0, (nb + ib - 1) / ib, c0, c1);
 }
 
 // T of one block of kb <= ib reflectors (any ib <= nb) rebuilt from the
