@@ -50,6 +50,7 @@ def parse():
                     help="precision of the trailing updates of the timed factorisation (the headline is fp64)")
     ap.add_argument("--no-panels", action="store_true", help="skip the whole-chip panel-kernel measurement")
     ap.add_argument("--no-library", action="store_true", help="skip the rocSOLVER / host LAPACK reference lines")
+    ap.add_argument("--no-homegrown", action="store_true", help="skip the tile_kernel.h per-call timings")
     ap.add_argument("--no-tile-sizes", action="store_true",
                     help="skip the factorisation with nb = 384 / 512 (generic kernels)")
     ap.add_argument("--only-factor", action="store_true",
@@ -283,6 +284,52 @@ def batched_bench(count, retune=False, check=True):
             "max_orthogonality": max(o for _, o in res32),
         },
     }
+
+
+def homegrown_bench(n=256, reps=5):
+    """The homegrown tile_kernel.h calls (host arrays, as the reference's
+    callers pass them) on n x n tiles: wall time per call, best of `reps`
+    after a warm-up, the copies to and from the device and the stream
+    synchronisation included (one workgroup per call)."""
+    import ctypes as C
+
+    import numpy as np
+
+    import hybrid3_amd as hq
+    L = hq.lib()
+    rng = np.random.default_rng(7)
+
+    def P(a):
+        return a.ctypes.data_as(C.POINTER(C.c_double))
+
+    def F(a):
+        return np.array(a, dtype=np.float64, order="F", copy=True)
+
+    A0 = F(rng.uniform(-1, 1, (n, n)))
+    R0 = F(np.triu(rng.uniform(-1, 1, (n, n))))
+    B0 = F(rng.uniform(-1, 1, (n, n)))
+    TAU, Y, T = np.zeros(n), F(np.zeros((n, n))), F(np.zeros((n, n)))
+    calls = {
+        "tile_dgeqrt": lambda A: L.tile_dgeqrt(n, n, n, P(A), n, P(TAU), P(Y), P(T), n, None),
+        "tile_dtsqrt": lambda A: L.tile_dtsqrt(n, n, n, P(R0.copy(order="F")), n, P(A), n, P(TAU), P(T), n, None),
+        "tile_dlarft": lambda A: L.tile_dlarft(n, n, 1, P(Y), n, P(TAU), P(T), n, None),
+        "tile_dlarfb": lambda A: L.tile_dlarfb(n, n, P(A), n, P(Y), P(T), n, None),
+        "tile_dssrfb": lambda A: L.tile_dssrfb(n, n, P(A), P(B0.copy(order="F")), n, P(Y), P(T), n, None),
+    }
+    flops = {"tile_dgeqrt": 4.0 / 3.0 * n ** 3, "tile_dtsqrt": 2.0 * n ** 3, "tile_dlarft": n ** 3 / 3.0,
+             "tile_dlarfb": 2.0 * n ** 3, "tile_dssrfb": 4.0 * n ** 3}
+    out = {"n": n}
+    for name, fn in calls.items():
+        best = float("inf")
+        for _ in range(reps + 1):
+            A = A0.copy(order="F")
+            t0 = time.perf_counter()
+            fn(A)
+            best = min(best, time.perf_counter() - t0)
+        if hq.lib().hq_last_error() != 0:
+            raise RuntimeError(f"{name} failed")
+        out[name] = {"ms": round(best * 1e3, 3), "gflops": round(flops[name] / best / 1e9, 1)}
+    return out
 
 
 def tall_bench(m=65536, n=4096, nrhs=64, nb=256, ib=32, seed=20151130, reps=3):
@@ -602,6 +649,11 @@ def main():
         print(json.dumps({"error": "batched factorisation check failed", "batched": batched}))
         sys.exit(3)
 
+    homegrown = None if args.no_homegrown else homegrown_bench()
+    if homegrown is not None:
+        note("tile_kernel.h calls at 256: " + ", ".join(f"{k} {v['ms']} ms" for k, v in homegrown.items()
+                                                          if k != "n"))
+
     tall = None
     if not args.no_tall:
         tall = tall_bench(nb=nb, ib=ib)
@@ -652,6 +704,7 @@ def main():
         "cpu_baseline": cpu,
         "library_reference": library,
         "other_tile_sizes": tile_sizes,
+        "homegrown_api": homegrown,
         "panels": panels,
         "check": check,
         "batched": batched,

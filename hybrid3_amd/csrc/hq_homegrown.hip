@@ -303,12 +303,6 @@ __global__ __launch_bounds__(NT) void dlarfb_kernel(int row, int col, double *A,
   sub_dlarfb(row, col, col, Y, lda, T, ldt, A, lda, work, work + (size_t)row * col);
 }
 
-// tile_kernel.c:121-136 (tile_dssrfb)
-__global__ __launch_bounds__(NT) void dssrfb_kernel(int row, int col, double *A1, double *A2,
-                                                   int lda, const double *Y, const double *T,
-                                                   int ldt, double *work) {
-  sub_dssrfb(row, row, col, col, Y, lda, T, ldt, A1, lda, A2, lda, work, work + (size_t)row * col);
-}
 
 // tile_kernel.c:368-437 (tile_dlarft): block < 2 -> full recursion;
 // 2 <= block < col -> combine the diagonal blocks already in T with the
@@ -419,12 +413,6 @@ hipError_t hqh_launch_dlarfb(int row, int col, double *A, int lda, const double 
                              const double *T, int ldt, double *work, hipStream_t st) {
   hipLaunchKernelGGL(hqh::dlarfb_kernel, dim3(1), dim3(hqh::NT), 0, st, row, col, A, lda, Y, T,
                      ldt, work);
-  return hipGetLastError();
-}
-hipError_t hqh_launch_dssrfb(int row, int col, double *A1, double *A2, int lda, const double *Y,
-                             const double *T, int ldt, double *work, hipStream_t st) {
-  hipLaunchKernelGGL(hqh::dssrfb_kernel, dim3(1), dim3(hqh::NT), 0, st, row, col, A1, A2, lda, Y,
-                     T, ldt, work);
   return hipGetLastError();
 }
 hipError_t hqh_launch_dlarft(int row, int col, int block, const double *Y, int lda,

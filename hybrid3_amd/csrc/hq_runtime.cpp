@@ -13,7 +13,10 @@
 #include <mutex>
 #include <queue>
 #include <tuple>
+#include <type_traits>
 #include <vector>
+
+#include <rocblas/rocblas.h>
 
 #include "../../include/Hybrid_tile_QR.h"
 #include "../../include/core_blas.h"
@@ -53,8 +56,6 @@ hipError_t hqh_launch_dtsqrt(int row, int col, int block, double *A1, int lda1, 
                              int lda2, double *TAU, double *T, int ldt, double *work,
                              hipStream_t st);
 hipError_t hqh_launch_dlarfb(int row, int col, double *A, int lda, const double *Y,
-                             const double *T, int ldt, double *work, hipStream_t st);
-hipError_t hqh_launch_dssrfb(int row, int col, double *A1, double *A2, int lda, const double *Y,
                              const double *T, int ldt, double *work, hipStream_t st);
 hipError_t hqh_launch_dlarft(int row, int col, int block, const double *Y, int lda,
                              const double *TAU, double *T, int ldt, double *work, hipStream_t st);
@@ -369,6 +370,7 @@ struct CachedSched {
 struct Lane {
   hipStream_t stream = nullptr;
   DevBuf buf;
+  rocblas_handle blas = nullptr;  // made on first use, bound to `stream`
 };
 
 struct Context {
@@ -1606,13 +1608,23 @@ int homegrown_call(const char *fn_name, std::vector<HArr> arrs, size_t work_elem
       HQ_HIP(hipMemcpyAsync(a.d, a.h, a.n() * sizeof(double), hipMemcpyHostToDevice, st));
   }
   double *work = base + off;
-  HQ_HIP(fn(arrs, work, st));
+  if constexpr (std::is_invocable_v<Fn, std::vector<HArr> &, double *, hipStream_t, Lane *>)
+    HQ_HIP(fn(arrs, work, st, lane));
+  else
+    HQ_HIP(fn(arrs, work, st));
   for (auto &a : arrs)
     if (a.out && a.n())
       HQ_HIP(hipMemcpyAsync(a.h, a.d, a.n() * sizeof(double), hipMemcpyDeviceToHost, st));
   HQ_HIP(hipStreamSynchronize(st));
   (void)fn_name;
   return ok();
+}
+
+// The lane's rocBLAS handle on the lane's stream (nullptr if it cannot be made).
+rocblas_handle lane_blas(Lane *l) {
+  if (!l->blas && rocblas_create_handle(&l->blas) != rocblas_status_success) l->blas = nullptr;
+  if (l->blas && rocblas_set_stream(l->blas, l->stream) != rocblas_status_success) return nullptr;
+  return l->blas;
 }
 
 bool hg_dims_ok(int row, int col, int ld) {
@@ -1671,11 +1683,77 @@ void tile_dtsqrt(const int row, const int col, const int block, double *A1, cons
   report("tile_dtsqrt");
 }
 
+}  // extern "C"
+
+// tile_dlarfb on square tiles and tile_dssrfb: the reference's BLAS call
+// sequence (tile_kernel.c:273-298 with row = col, and tile_sub_dssrfb,
+// tile_kernel.c:313-363, with over_row = under_row = row, Y_col = A_col =
+// col) as rocBLAS products on the lane's stream, so any T the caller passes
+// is applied as written (the products are the formula, not a compact-WY
+// shortcut), and rows of A1 past col come back zero as in the reference.
+hipError_t hg_dlarfb_blas(Lane *ln, int n, double *A, int lda, const double *Y, const double *T, int ldt,
+                          double *w) {
+  rocblas_handle h = lane_blas(ln);
+  if (!h) return hipErrorUnknown;
+  const double one = 1.0, mone = -1.0;
+  double *D = w, *D2 = w + (size_t)n * n;
+  // D = tril(Y)^T A; D2 = tril(T) D; A -= Y D2
+  if (rocblas_dtrmm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_transpose,
+                    rocblas_diagonal_non_unit, n, n, &one, Y, lda, A, lda, D, n) != rocblas_status_success ||
+      rocblas_dtrmm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_non_unit,
+                    n, n, &one, T, ldt, D, n, D2, n) != rocblas_status_success ||
+      rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n, n, n, &mone, Y, lda, D2, n, &one, A,
+                    lda) != rocblas_status_success)
+    return hipErrorUnknown;
+  return hipSuccess;
+}
+
+hipError_t hg_dssrfb_blas(Lane *ln, int row, int col, double *A1, double *A2, int lda, const double *Y,
+                          const double *T, int ldt, double *w, hipStream_t st) {
+  rocblas_handle h = lane_blas(ln);
+  if (!h) return hipErrorUnknown;
+  const double one = 1.0, mone = -1.0;
+  double *D = w, *D2 = w + (size_t)row * col;
+  // D = A1 (row rows); D[:col] += Y^T A2; D2 = tril(T) D[:col]
+  hipError_t e = hipMemcpy2DAsync(D, sizeof(double) * row, A1, sizeof(double) * lda, sizeof(double) * row, col,
+                                  hipMemcpyDeviceToDevice, st);
+  if (e != hipSuccess) return e;
+  if (rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, col, col, row, &one, Y, lda, A2, lda,
+                    &one, D, row) != rocblas_status_success ||
+      rocblas_dtrmm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_non_unit,
+                    col, col, &one, T, ldt, D, row, D2, col) != rocblas_status_success)
+    return hipErrorUnknown;
+  // A1 -= D: rows < col take D2, rows col..row-1 (D = A1 there) become zero
+  if (rocblas_dgeam(h, rocblas_operation_none, rocblas_operation_none, col, col, &one, A1, lda, &mone, D2, col, A1,
+                    lda) != rocblas_status_success)
+    return hipErrorUnknown;
+  if (row > col && (e = hipMemset2DAsync(A1 + col, sizeof(double) * lda, 0, sizeof(double) * (row - col), col,
+                                         st)) != hipSuccess)
+    return e;
+  // A2 -= Y D2
+  if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, row, col, col, &mone, Y, lda, D2, col, &one,
+                    A2, lda) != rocblas_status_success)
+    return hipErrorUnknown;
+  return hipSuccess;
+}
+
+extern "C" {
+
 void tile_dlarfb(const int row, const int col, double *A, const int lda, const double *Y,
                  const double *T, const int ldt, double *work) {
   (void)work;
   if (!hg_dims_ok(row, col, lda) || ldt < col) {
     if (g_err == HQ_OK) set_error(HQ_ERR_ARG, "tile_dlarfb: bad ldt");
+    report("tile_dlarfb");
+    return;
+  }
+  if (row == col) {
+    homegrown_call("tile_dlarfb",
+                   {{A, row, col, lda, true, true}, {(double *)Y, row, col, lda, true, false},
+                    {(double *)T, col, col, ldt, true, false}},
+                   2 * (size_t)col * col, [&](std::vector<HArr> &a, double *w, hipStream_t, Lane *ln) {
+                     return hg_dlarfb_blas(ln, col, a[0].d, lda, a[1].d, a[2].d, ldt, w);
+                   });
     report("tile_dlarfb");
     return;
   }
@@ -1698,11 +1776,9 @@ void tile_dssrfb(const int row, const int col, double *A1, double *A2, const int
   }
   homegrown_call("tile_dssrfb",
                  {{A1, row, col, lda, true, true}, {A2, row, col, lda, true, true},
-                  {(double *)Y, row, col, lda, true, false},
-                  {(double *)T, col, col, ldt, true, false}},
-                 hg_work(row, col), [&](std::vector<HArr> &a, double *w, hipStream_t st) {
-                   return hqh_launch_dssrfb(row, col, a[0].d, a[1].d, lda, a[2].d, a[3].d, ldt, w,
-                                            st);
+                  {(double *)Y, row, col, lda, true, false}, {(double *)T, col, col, ldt, true, false}},
+                 (size_t)row * col + (size_t)col * col, [&](std::vector<HArr> &a, double *w, hipStream_t st, Lane *ln) {
+                   return hg_dssrfb_blas(ln, row, col, a[0].d, a[1].d, lda, a[2].d, a[3].d, ldt, w, st);
                  });
   report("tile_dssrfb");
 }

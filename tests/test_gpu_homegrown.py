@@ -59,7 +59,7 @@ def test_tile_dtsqrt(hq, orc, n, block):
     assert np.abs(TAU - TAUo).max() < TOL
 
 
-@pytest.mark.parametrize("n", [16, 64, 128])
+@pytest.mark.parametrize("n", [16, 64, 128, 256])
 def test_tile_dlarft_dlarfb_dssrfb(hq, orc, n):
     rng = np.random.default_rng(3 * n)
     A = F(rng.uniform(-1, 1, (n, n)))
@@ -89,6 +89,39 @@ def test_tile_dlarft_dlarfb_dssrfb(hq, orc, n):
     g1, g2 = F(B1), F(B2)
     hq.lib().tile_dssrfb(n, n, P(g1), P(g2), n, P(V2), P(T2), n, None)
     assert rel(g1, o1) < TOL and rel(g2, o2) < TOL
+
+
+@pytest.mark.parametrize("n", [128, 256])
+def test_tile_dlarfb_zero_reflector_column(hq, orc, n):
+    """tile_dgeqrt leaves Y's last column zero; tile_dlarfb then ignores the
+    matching row and column of T whatever they hold (here a random lower T),
+    on the matrix-core path too."""
+    rng = np.random.default_rng(5 * n)
+    _, _, Y, _ = orc.tile_dgeqrt(F(rng.uniform(-1, 1, (n, n))), n)
+    assert np.all(Y[:, -1] == 0.0)
+    T = F(np.tril(rng.uniform(-1, 1, (n, n))))
+    Cm = F(rng.uniform(-1, 1, (n, n)))
+    Co = orc.tile_dlarfb(Cm, Y, T)
+    Cg = F(Cm)
+    print(f"tile_dlarfb zero column n={n}", flush=True)
+    hq.lib().tile_dlarfb(n, n, P(Cg), n, P(Y), P(T), n, None)
+    assert rel(Cg, Co) < TOL
+
+
+@pytest.mark.parametrize("row,col", [(24, 16), (40, 32), (96, 64)])
+def test_tile_dssrfb_row_above_col(hq, orc, row, col):
+    """tile_dssrfb with more rows than reflectors (tile_kernel.c:121-136):
+    A1's rows past col come back zero, as in the reference."""
+    rng = np.random.default_rng(row + col)
+    Y = F(rng.uniform(-1, 1, (row, col)))
+    T = F(np.tril(rng.uniform(-1, 1, (col, col))))
+    B1, B2 = F(rng.uniform(-1, 1, (row, col))), F(rng.uniform(-1, 1, (row, col)))
+    o1, o2 = orc.tile_dssrfb(B1, B2, Y, T)
+    g1, g2 = F(B1), F(B2)
+    print(f"tile_dssrfb row={row} col={col}", flush=True)
+    hq.lib().tile_dssrfb(row, col, P(g1), P(g2), row, P(Y), P(T), col, None)
+    assert rel(g1, o1) < TOL and rel(g2, o2) < TOL
+    assert np.all(g1[col:] == 0.0)
 
 
 def test_make_i_and_dgemm(hq, orc):

@@ -123,3 +123,21 @@ def test_tile_dlarft_blocked_keeps_reference_loop_bound(orc):
     # block >= col computes nothing
     T0 = orc.tile_dlarft(Y, TAU, n, T=Tdiag.copy())
     assert np.all(T0 == Tdiag)
+
+
+def test_tile_dssrfb_rows_past_col_are_zeroed(orc):
+    """tile_kernel.c:121-136,313-363 with row > col: the over-block copy of A1
+    has `row` rows, but only its first col rows receive Y^T A2 and the T
+    product, so after A1 -= that copy, A1's rows col..row-1 are zero; the
+    first col rows and A2 follow the block reflector (closed form)."""
+    rng = np.random.default_rng(17)
+    row, col = 12, 8
+    Y = rng.uniform(-1, 1, (row, col))
+    T = np.tril(rng.uniform(-1, 1, (col, col)))
+    A1 = rng.uniform(-1, 1, (row, col))
+    A2 = rng.uniform(-1, 1, (row, col))
+    o1, o2 = orc.tile_dssrfb(A1, A2, Y, T)
+    W = T @ (A1[:col] + Y.T @ A2)
+    assert np.abs(o1[:col] - (A1[:col] - W)).max() < 1e-13
+    assert np.all(o1[col:] == 0.0)
+    assert np.abs(o2 - (A2 - Y @ W)).max() < 1e-13
