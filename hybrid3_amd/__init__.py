@@ -72,6 +72,9 @@ SIGNATURES = {
     "hq_wall_clock_khz": (C.c_int, [C.POINTER(C.c_int)]),
     "hq_tile_build_t_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp]),
     "hq_tile_ormqr_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp, C.c_int, _vp]),
+    "hq_tile_ormqr_cols_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp]),
+    "hq_tile_ormqr_cols_traced_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int,
+                                                                _vp, _vp]),
     "hq_tile_orgqr_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp, _vp]),
     "hq_tile_make_q_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp]),
     "hq_dag_host": (C.c_int, [C.c_int] * 6 + [C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int,
@@ -349,6 +352,13 @@ def tile_ormqr_dev(MT, NT, nb, ib, RY, TAU, X, XNT, T=None, stream=None):
     """X <- Q^T X on the device (X has MT x XNT tiles)."""
     check(lib().hq_tile_ormqr_dev(MT, NT, nb, ib, _ptr(RY), _ptr(TAU), _ptr(T), _ptr(X), XNT,
                                   _stream(stream)))
+
+
+def tile_ormqr_cols_dev(MT, NT, nb, ib, RY, TAU, X, XNT, ncol, T=None, width=0, stream=None):
+    """X <- Q^T X on the device for an X whose first `ncol` columns hold
+    entries and whose other columns hold zeros; width 16 / 64 / 0 (pick)."""
+    check(lib().hq_tile_ormqr_cols_dev(MT, NT, nb, ib, _ptr(RY), _ptr(TAU), _ptr(T), _ptr(X), XNT, ncol, width,
+                                       _stream(stream)))
 
 
 def tile_orgqr_dev(MT, NT, nb, ib, RY, TAU, Q, T=None, stream=None):

@@ -3136,7 +3136,7 @@ hipError_t hq_launch_sched(const hq_geom &G, const hq_geom *geoms, const hq_task
                            int *flags, int epoch, int *counter, int grid, long long *trace, hipStream_t st,
                            int *epoch_dev, unsigned wait_limit) {
   hipLaunchKernelGGL(hq::reset_counter_kernel, dim3(1), dim3(64), 0, st, counter, epoch_dev);
-  if (!hq_tuned_nb(G.nb))
+  if (!hq_tuned_nb(G.nb) || G.uw)
     return hqg_launch_sched(G, geoms, tasks, ntasks, flags, epoch, counter, grid, trace, st, epoch_dev, wait_limit);
   return hq_dispatch(G.nb, G.ib, [&](auto nbc, auto wbc) {
     HQ_NBWB

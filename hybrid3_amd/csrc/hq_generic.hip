@@ -38,7 +38,7 @@ struct Smem {
     double P[IBM * LDP];  // panel block: column c at P + c*LDP, row i = panel row i
     struct {
       alignas(32) double Vs[16 * LDV2];   // sub-block of V: V(r, c) at Vs[c*LDV2 + r]
-      alignas(32) double Wx[2 * 4 * 256]; // partial W of each (row half, strip), MFMA layout
+      alignas(32) double Wx[NW * 256];   // partial W of each (row group, strip), MFMA layout
     } m;
   };
   double R[IBM * LDT];   // TSQRT: triangle block of R(k,k), R(p, c) at R[c*LDT + p]
@@ -222,32 +222,39 @@ __device__ __forceinline__ d4 mfma(double a, double b, d4 c) {
 // registers are the B operand of the next product with k index g.
 __device__ __forceinline__ int arow(int i) { return 4 * (i & 3) + (i >> 2); }
 
-// Rows of the strip a wave owns: half h of the tile's rows, in 16-row blocks.
-__device__ __forceinline__ int half_rows(int nb) { return (((nb + 1) >> 1) + 15) & ~15; }
+// Rows of the strip a wave owns: one of R row groups of the tile, in whole
+// 16-row blocks.
+template <int R>
+__device__ __forceinline__ int group_rows(int nb) { return (((nb + R - 1) / R) + 15) & ~15; }
 
 // X <- H_b^T X for the blocks b = b0 .. b1-1 in order (H_b = I - V_b T_b
 // V_b^T, V_b the reflectors at columns b*ib .. of tile Vt, T_b at
-// T + b*ib*ib with ld ib), on X's columns c0 .. c1-1 (c1 - c0 <= 64):
+// T + b*ib*ib with ld ib), on X's columns c0 .. c1-1 (c1 - c0 <= 16 SN):
 //   !TS: V_b unit lower trapezoidal (diagonal at row b*ib + c), X the tile;
 //   TS:  [X1 rows of the block; X] with V_b = [I; Vt columns].
 // Register-resident, on the matrix cores: wave w owns the 16-column strip
-// w % 4 of the slab over row half w / 4, as MFMA accumulators for the whole
-// call.  Each block is applied as sub-blocks of 16 reflectors with the 16x16
-// diagonal blocks of T_b (the compact WY form of a product of two blocks),
-// V of a sub-block staged in LDS (the next one's loads in flight in
-// registers meanwhile):
-//   W = V^T X: each wave its row half; the two halves' partials exchanged in
-//     LDS and added (both waves of a strip hold the same W);
-//   W2 = T^T W (+ X1 rows first, TS; then X1 -= W2 by the upper half);
+// w % SN of the slab over row group w / SN (of NW / SN groups), as MFMA
+// accumulators for the whole call.  Each block is applied as sub-blocks of
+// 16 reflectors with the 16x16 diagonal blocks of T_b (the compact WY form
+// of a product of two blocks), V of a sub-block staged in LDS (the next
+// one's loads in flight in registers meanwhile):
+//   W = V^T X: each wave its row group; the groups' partials exchanged in
+//     LDS and summed in group order (every wave of a strip holds the same W);
+//   W2 = T^T W (+ X1 rows first, TS; then X1 -= W2 by row group 0);
 //   X -= V W2 on the wave's row blocks.
-template <bool TS>
+// SN = 4 (64-column slabs, two row halves) for the factorisation; SN = 1
+// (one 16-column strip over 8 row groups) for a narrow X, whose chain of
+// tasks along the flat tree then runs on a fraction of the work each.
+template <bool TS, int SN = 4>
 __device__ void update_strips(Smem &S, const double *Vt, const double *T, int ib, double *X, double *X1, int nb,
                               int b0, int b1, int c0, int c1) {
+  constexpr int RG = NW / SN;              // row groups
+  constexpr int RBM = NBMAX / 16 / RG;     // 16-row blocks of a group at most
   const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, g = lane >> 4;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int s = wave & 3, h = wave >> 2;
-  const int RH = half_rows(nb), rbase = h * RH;
-  const int nrb = max(0, min(RH, nb - rbase) + 15) >> 4;  // 16-row blocks of this half
+  const int s = wave % SN, h = wave / SN;
+  const int RH = group_rows<RG>(nb), rbase = h * RH;
+  const int nrb = max(0, min(RH, nb - rbase) + 15) >> 4;  // 16-row blocks of this group
   const int col = c0 + 16 * s + m;
   const bool colok = col < c1;
   // !TS: rows above the first block's diagonal are neither read nor written
@@ -255,9 +262,9 @@ __device__ void update_strips(Smem &S, const double *Vt, const double *T, int ib
   // same step may be updating meanwhile)
   const int row_lo = TS ? 0 : b0 * ib;
   // the strip, rows rbase + 16 rb + 4g + v at acc[rb][v]
-  d4 acc[NBMAX / 32];
+  d4 acc[RBM];
 #pragma unroll
-  for (int rb = 0; rb < NBMAX / 32; rb++) {
+  for (int rb = 0; rb < RBM; rb++) {
     acc[rb] = d4{0.0, 0.0, 0.0, 0.0};
     if (rb < nrb && colok) {
       const int r = rbase + 16 * rb + 4 * g;
@@ -318,7 +325,7 @@ __device__ void update_strips(Smem &S, const double *Vt, const double *T, int ib
     // W partial over this half's rows: A(i, k) = V(row_k, i), B(k, j) = X(row_k, j)
     d4 wp = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int rb = 0; rb < NBMAX / 32; rb++) {
+    for (int rb = 0; rb < RBM; rb++) {
       if (rb < nrb) {
 #pragma unroll
         for (int v = 0; v < 4; v++) wp = mfma(Vs[m * LDV2 + rbase + 16 * rb + 4 * g + v], acc[rb][v], wp);
@@ -332,10 +339,17 @@ __device__ void update_strips(Smem &S, const double *Vt, const double *T, int ib
       for (int v = 0; v < 4; v++)
         if (4 * v + g < nq) x1[v] = X1[(size_t)col * nb + ii + q0 + 4 * v + g];
     }
-    *reinterpret_cast<d4 *>(S.m.Wx + (h * 4 + s) * 256 + 4 * lane) = wp;
+    *reinterpret_cast<d4 *>(S.m.Wx + (h * SN + s) * 256 + 4 * lane) = wp;
     __syncthreads();
-    const d4 wo = *reinterpret_cast<const d4 *>(S.m.Wx + ((1 - h) * 4 + s) * 256 + 4 * lane);
-    d4 w = wp + wo;
+    d4 w;
+    if constexpr (RG == 2) {
+      // own + other: the same sum in both waves (addition commutes)
+      w = wp + *reinterpret_cast<const d4 *>(S.m.Wx + ((1 - h) * SN + s) * 256 + 4 * lane);
+    } else {
+      w = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int q = 0; q < RG; q++) w += *reinterpret_cast<const d4 *>(S.m.Wx + (q * SN + s) * 256 + 4 * lane);
+    }
     if (TS) w += x1;
     // W2 = T^T W: A(i, k) = T(k, i), B = W (D layout = B layout per k step)
     d4 w2 = d4{0.0, 0.0, 0.0, 0.0};
@@ -347,9 +361,9 @@ __device__ void update_strips(Smem &S, const double *Vt, const double *T, int ib
         if (4 * v + g < nq) X1[(size_t)col * nb + ii + q0 + 4 * v + g] = x1[v] - w2[v];
     }
     const d4 nw2 = -w2;
-    // X -= V W2 on this half's row blocks: A(i, k) = V(rbase + 16 rb + arow(i), k)
+    // X -= V W2 on this group's row blocks: A(i, k) = V(rbase + 16 rb + arow(i), k)
 #pragma unroll
-    for (int rb = 0; rb < NBMAX / 32; rb++) {
+    for (int rb = 0; rb < RBM; rb++) {
       if (rb < nrb) {
 #pragma unroll
         for (int kk = 0; kk < 4; kk++)
@@ -359,7 +373,7 @@ __device__ void update_strips(Smem &S, const double *Vt, const double *T, int ib
   }
   // store the strip
 #pragma unroll
-  for (int rb = 0; rb < NBMAX / 32; rb++) {
+  for (int rb = 0; rb < RBM; rb++) {
     if (rb < nrb && colok) {
       const int r = rbase + 16 * rb + 4 * g;
       double *xp = X + (size_t)col * nb + r;
@@ -369,6 +383,168 @@ __device__ void update_strips(Smem &S, const double *Vt, const double *T, int ib
 #pragma unroll
         for (int v = 0; v < 4; v++)
           if (r + v >= row_lo && r + v < nb) xp[v] = acc[rb][v];
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// X <- H^T X for all blocks of tile Vt (as update_strips with b0 = 0, b1 =
+// the block count) on one 16-column strip of X (c1 - c0 <= 16): wave w owns
+// row group w of NW.  No two waves need the same rows of V, so each wave
+// reads its own rows of each 16-reflector sub-block (and the sub-block's
+// 16 x 16 block of T) with whole-line loads -- lane l: reflector l / 4, four
+// consecutive rows -- PD sub-blocks ahead in registers, and rearranges them
+// into the MFMA operand layouts through an LDS area of its own (no barrier:
+// a wave's LDS accesses complete in order).  The groups' partial W are summed
+// in group order through LDS, double-buffered, one barrier per sub-block.
+// RB: 16-row blocks a group holds at most.  Out of line, so that the
+// apply-only kernel allocates registers for each of its two bodies on its
+// own; its pointers are then generic, and are cast to the global address
+// space (flat loads would also count against the LDS counter, so every wait
+// for an LDS read would wait for the loads in flight as well).
+typedef __attribute__((address_space(1))) double gdouble;
+constexpr int NLD = 17;  // column stride of a wave's LDS copy of V / T (odd: fewer bank conflicts)
+template <bool TS, int RB, int PD>
+__device__ __noinline__ void update_narrow(Smem &S, const double *Vt_, const double *T_, int ib, double *X_,
+                                           double *X1_, int nb, int c0, int c1) {
+  const gdouble *Vt = (const gdouble *)Vt_, *T = (const gdouble *)T_;
+  gdouble *X = (gdouble *)X_, *X1 = (gdouble *)X1_;
+  const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, g = lane >> 4;
+  const int h = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int RH = group_rows<NW>(nb), rbase = h * RH;
+  const int nrb = max(0, min(RH, nb - rbase) + 15) >> 4;
+  const int col = c0 + m;
+  const bool colok = col < c1;
+  // LDS: partial W (2 x NW x 256), then per wave RB blocks of V (16 x NLD
+  // each) and the T block (16 x NLD)
+  static_assert(2 * NW * 256 + NW * (RB + 1) * 16 * NLD <= sizeof(S.P) / sizeof(double), "LDS areas");
+  double *Wx = S.P;
+  double *Vw = S.P + 2 * NW * 256 + h * (RB + 1) * 16 * NLD, *Tw = Vw + RB * 16 * NLD;
+  // the strip, rows rbase + 16 rb + 4g + v at acc[rb][v]
+  d4 acc[RB];
+#pragma unroll
+  for (int rb = 0; rb < RB; rb++) {
+    acc[rb] = d4{0.0, 0.0, 0.0, 0.0};
+    if (rb < nrb && colok) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int r = rbase + 16 * rb + 4 * g + v;
+        if (r < nb) acc[rb][v] = X[(size_t)col * nb + r];
+      }
+    }
+  }
+  const int nsb_per = (ib + 15) / 16, nsb = ((nb + ib - 1) / ib) * nsb_per;
+  // loaded layout: lane l holds reflector lc = l / 4, rows 4 (l % 4) + e
+  const int lc = lane >> 2, lr = 4 * (lane & 3);
+  double vr[PD][RB][4], tr[PD][4];
+  auto load = [&](int p, int sbi) {
+    const bool live = sbi < nsb;
+    const int b = sbi / nsb_per, q0 = (sbi % nsb_per) * 16, ii = b * ib, kb = min(ib, nb - ii);
+    const int q = q0 + lc, d = ii + q;  // reflector of this lane, its diagonal row
+    const bool qok = live && q < kb;
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++) {
+      const int r0 = rbase + 16 * rb + lr;
+      const bool whole = qok && rb < nrb && r0 + 3 < nb && (nb & 3) == 0;
+      d4 x = d4{0.0, 0.0, 0.0, 0.0};
+      if (whole) {
+        x = *reinterpret_cast<const __attribute__((address_space(1))) d4 *>(Vt + (size_t)d * nb + r0);
+      } else if (qok && rb < nrb) {
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+          if (r0 + e < nb) x[e] = Vt[(size_t)d * nb + r0 + e];
+      }
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int r = r0 + e;
+        // !TS: unit lower trapezoidal (R above the diagonal is not V)
+        vr[p][rb][e] = TS || r > d ? x[e] : (r == d && qok ? 1.0 : 0.0);
+      }
+    }
+    // T(q0 + lr + e, q0 + lc): upper triangle of the sub-block's diagonal block
+    const int nq = min(16, kb - q0);
+    d4 t = d4{0.0, 0.0, 0.0, 0.0};
+    if (live && lc < nq) {
+      const gdouble *tp = T + (size_t)ii * ib + (size_t)(q0 + lc) * ib + q0 + lr;
+#pragma unroll
+      for (int e = 0; e < 4; e++)
+        if (lr + e <= lc) t[e] = tp[e];
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++) tr[p][e] = t[e];
+  };
+  auto apply = [&](int p, int sbi) {
+    const int b = sbi / nsb_per, q0 = (sbi % nsb_per) * 16, ii = b * ib, kb = min(ib, nb - ii);
+    const int nq = min(16, kb - q0);
+    if (nq <= 0) return;  // past the last (narrower) block's end; uniform
+    // this wave's copy: V(row 16 rb + r, reflector c) at Vw[rb * 16 * NLD + c * NLD + r]
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) Vw[rb * 16 * NLD + lc * NLD + lr + e] = vr[p][rb][e];
+#pragma unroll
+    for (int e = 0; e < 4; e++) Tw[lc * NLD + lr + e] = tr[p][e];
+    // W partial over this group's rows: A(i, k) = V(row 4k + v, i), B = the strip
+    d4 wp = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++)
+      if (rb < nrb) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) wp = mfma(Vw[rb * 16 * NLD + m * NLD + 4 * g + v], acc[rb][v], wp);
+      }
+    // TS: the sub-block's rows of X1 (read before the barrier; group 0
+    // writes them after it)
+    d4 x1 = d4{0.0, 0.0, 0.0, 0.0};
+    if (TS && colok) {
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        if (4 * v + g < nq) x1[v] = X1[(size_t)col * nb + ii + q0 + 4 * v + g];
+    }
+    double *Wb = Wx + (sbi & 1) * (NW * 256);
+    *reinterpret_cast<d4 *>(Wb + h * 256 + 4 * lane) = wp;
+    __syncthreads();
+    d4 w = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int qg = 0; qg < NW; qg++) w += *reinterpret_cast<const d4 *>(Wb + qg * 256 + 4 * lane);
+    if (TS) w += x1;
+    // W2 = T^T W: A(i, k) = T(k, i)
+    d4 w2 = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) w2 = mfma(Tw[m * NLD + 4 * kk + g], w[kk], w2);
+    if (TS && h == 0 && colok) {
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        if (4 * v + g < nq) X1[(size_t)col * nb + ii + q0 + 4 * v + g] = x1[v] - w2[v];
+    }
+    const d4 nw2 = -w2;
+    // X -= V W2: A(i, k) = V(row arow(i) of block rb, reflector k)
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++)
+      if (rb < nrb) {
+#pragma unroll
+        for (int kk = 0; kk < 4; kk++)
+          acc[rb] = mfma(Vw[rb * 16 * NLD + (4 * kk + g) * NLD + arow(m)], nw2[kk], acc[rb]);
+      }
+  };
+#pragma unroll
+  for (int p = 0; p < PD; p++) load(p, p);
+  for (int s0 = 0; s0 < nsb; s0 += PD) {
+#pragma unroll
+    for (int p = 0; p < PD; p++) {
+      if (s0 + p < nsb) {
+        apply(p, s0 + p);
+        load(p, s0 + p + PD);
+      }
+    }
+  }
+#pragma unroll
+  for (int rb = 0; rb < RB; rb++) {
+    if (rb < nrb && colok) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int r = rbase + 16 * rb + 4 * g + v;
+        if (r < nb) X[(size_t)col * nb + r] = acc[rb][v];
       }
     }
   }
@@ -399,8 +575,16 @@ __device__ void pad_panel_blocks(double *T, double *TAU, int nb, int ib, int ii)
 // One task of the DAG (the task kinds and fields of hq_task).  Tasks wholly
 // inside a batch matrix's zero padding write only the zeros their outputs
 // hold, or nothing; updates stop at the last column that holds entries.
+// NRB > 0: an application of Q^T only (LARFB / SSRFB on 16-column slabs,
+// hq_geom::uw; update_narrow with NRB row blocks per group), a kernel of its
+// own.
+template <int NRB = 0>
 __device__ void run_task(Smem &S, const hq_geom &G, const hq_task &t) {
+  constexpr bool NARROW = NRB > 0;
+  constexpr int PD = NRB <= 2 ? 4 : 3;  // sub-blocks in flight (registers)
   const int nb = G.nb, ib = G.ib;
+  if constexpr (NARROW)
+    if (t.kind != HQ_LARFB && t.kind != HQ_SSRFB) return;
   switch (t.kind) {
     case HQ_GEQRT: {
       double *A = tile_ptr(G, t.k, t.k), *T = t_ptr(G, t.k, t.k), *TAU = tau_ptr(G, t.k, t.k);
@@ -458,15 +642,23 @@ __device__ void run_task(Smem &S, const hq_geom &G, const hq_task &t) {
     case HQ_LARFB: {
       const double *V = tile_ptr(G, t.k, t.k), *T = t_ptr(G, t.k, t.k);
       double *X = xtile_ptr(G, t.k, t.j);
-      const int uw = hq_upd_slab(nb, 0), c0 = t.slab * uw, c1 = min(live_cols(G, t.j), c0 + uw);
-      if (c0 < c1) update_strips<false>(S, V, T, ib, X, nullptr, nb, 0, (nb + ib - 1) / ib, c0, c1);
+      const int uw = NARROW ? 16 : hq_upd_slab(nb, 0), c0 = t.slab * uw, c1 = min(live_cols(G, t.j), c0 + uw);
+      if (c0 >= c1) break;
+      if constexpr (NARROW)
+        update_narrow<false, (NARROW ? NRB : 1), PD>(S, V, T, ib, X, nullptr, nb, c0, c1);
+      else
+        update_strips<false>(S, V, T, ib, X, nullptr, nb, 0, (nb + ib - 1) / ib, c0, c1);
       break;
     }
     case HQ_SSRFB: {
       const double *V = tile_ptr(G, t.r, t.k), *T = t_ptr(G, t.r, t.k);
       double *X1 = xtile_ptr(G, t.k, t.j), *X2 = xtile_ptr(G, t.r, t.j);
-      const int uw = hq_upd_slab(nb, 0), c0 = t.slab * uw, c1 = min(live_cols(G, t.j), c0 + uw);
-      if (c0 < c1) update_strips<true>(S, V, T, ib, X2, X1, nb, 0, (nb + ib - 1) / ib, c0, c1);
+      const int uw = NARROW ? 16 : hq_upd_slab(nb, 0), c0 = t.slab * uw, c1 = min(live_cols(G, t.j), c0 + uw);
+      if (c0 >= c1) break;
+      if constexpr (NARROW)
+        update_narrow<true, (NARROW ? NRB : 1), PD>(S, V, T, ib, X2, X1, nb, c0, c1);
+      else
+        update_strips<true>(S, V, T, ib, X2, X1, nb, 0, (nb + ib - 1) / ib, c0, c1);
       break;
     }
     default:
@@ -479,6 +671,7 @@ __device__ void run_task(Smem &S, const hq_geom &G, const hq_task &t) {
 // counter, dependency flags equal to this run's epoch, release before
 // publishing, acquire after the wait, a bounded wait that reports the task in
 // counter[1] and drains the grid), without its look-ahead claim.
+template <int NRB>
 __global__ __launch_bounds__(NT) void sched_kernel(hq_geom G0, const hq_geom *__restrict__ geoms,
                                                    const hq_task *__restrict__ tasks, int ntasks, int *flags,
                                                    int epoch_arg, int *counter, long long *trace,
@@ -542,7 +735,7 @@ __global__ __launch_bounds__(NT) void sched_kernel(hq_geom G0, const hq_geom *__
       trace[4 * t + 1] = wall_clock64();
       trace[4 * t + 3] = blockIdx.x;
     }
-    run_task(S, geoms ? geoms[tk.mat] : G0, tk);
+    run_task<NRB>(S, geoms ? geoms[tk.mat] : G0, tk);
     // every wave's stores have completed before the flag can be published
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
@@ -665,9 +858,17 @@ __global__ __launch_bounds__(NT) void tbuild_tile_kernel(const double *V, const 
 hipError_t hqg_launch_sched(const hq_geom &G, const hq_geom *geoms, const hq_task *tasks, int ntasks, int *flags,
                             int epoch, int *counter, int grid, long long *trace, hipStream_t st, int *epoch_dev,
                             unsigned wait_limit) {
-  if (G.nb > hqg::NBMAX || G.ib > hqg::IBM || G.prec != 0) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(hqg::sched_kernel, dim3(grid), dim3(hqg::NT), 0, st, G, geoms, tasks, ntasks, flags, epoch,
-                     counter, trace, (const int *)epoch_dev, wait_limit);
+  if (G.nb > hqg::NBMAX || G.ib > hqg::IBM || G.prec != 0 || (G.uw && G.uw != 16)) return hipErrorInvalidValue;
+  // 16-column apply: row groups of NB / 8 rows, in 16-row blocks
+  if (G.uw && G.nb <= 256)
+    hipLaunchKernelGGL(hqg::sched_kernel<2>, dim3(grid), dim3(hqg::NT), 0, st, G, geoms, tasks, ntasks, flags,
+                       epoch, counter, trace, (const int *)epoch_dev, wait_limit);
+  else if (G.uw)
+    hipLaunchKernelGGL(hqg::sched_kernel<4>, dim3(grid), dim3(hqg::NT), 0, st, G, geoms, tasks, ntasks, flags,
+                       epoch, counter, trace, (const int *)epoch_dev, wait_limit);
+  else
+    hipLaunchKernelGGL(hqg::sched_kernel<0>, dim3(grid), dim3(hqg::NT), 0, st, G, geoms, tasks, ntasks, flags,
+                       epoch, counter, trace, (const int *)epoch_dev, wait_limit);
   return hipGetLastError();
 }
 hipError_t hqg_launch_task(const hq_geom &G, const hq_task &tk, hipStream_t st) {

@@ -78,6 +78,10 @@ struct hq_geom {
   // padding only write those zeros (and the diagonal tile's W = 0 and unit
   // V columns) and skip their arithmetic.
   int ncol;
+  // Column width of the LARFB / SSRFB tasks when it is not the tile size's
+  // default (hq_upd_slab): 16 for the Q^T application to a narrow X, which
+  // runs on the generic kernels with one 16-column strip per workgroup.
+  int uw;
 };
 
 // halves of split images per tile: one image of 64 nb halves per 16-reflector
@@ -96,6 +100,8 @@ struct hq_bmat {
 
 #define HQ_THREADS 256
 #define HQ_SLAB 64
+// Q^T applied to fewer columns than this runs as 16-column update tasks
+#define HQ_NARROW_COLS 256
 
 // Column width of one LARFB / SSRFB task.  FP64 and split updates: 64 (one
 // 16-column strip per wave).  FP32 updates: 128 (two strips per wave), so

@@ -127,6 +127,9 @@ struct Schedule {
 // (0 <= j < XNT) gets
 //   LARFB(k,j,s)   MEM: SSRFB(k-1,j,k,s)
 //   SSRFB(k,j,r,s) LIN: SSRFB(k,j,r-1,s) | LARFB(k,j,s); MEM: SSRFB(k-1,j,r,s)
+// for each slab s of UW columns (uw_ overrides the factorisation's slab
+// width) that holds one of X's first ncol_ columns (0: all); the slabs are
+// independent chains, so a narrow X gets as many chains as it has slabs.
 struct DagBuilder {
   int MT, NT, nb, ib, S, KT;
   bool apply;
@@ -134,6 +137,7 @@ struct DagBuilder {
   int NBLK;  // ib blocks per tile
   int prec;  // precision of the LARFB / SSRFB updates (their slab width and cost)
   int UW, SU;  // LARFB / SSRFB slab width (hq_upd_slab) and slabs per tile
+  int NCX;     // apply mode: X's columns that hold entries (0: all)
   std::vector<hq_task> t;
   std::vector<int> idl, ids;
   // panel tasks of the current step k: (row r, block b); their trailing
@@ -141,11 +145,12 @@ struct DagBuilder {
   std::vector<int> idp, idu;
   bool split;  // GEQRT / TSQRT as panel + trailing tasks (else one task per tile)
   DagBuilder(int MT_, int NT_, int nb_, int ib_, bool apply_ = false, int XNT_ = 0, bool split_ = true,
-             int prec_ = 0)
+             int prec_ = 0, int uw_ = 0, int ncol_ = 0)
       : MT(MT_), NT(NT_), nb(nb_), ib(ib_), apply(apply_), XNT(apply_ ? XNT_ : NT_), prec(apply_ ? 0 : prec_),
         split(split_) {
     S = (nb + HQ_SLAB - 1) / HQ_SLAB;
-    UW = hq_upd_slab(nb, prec);
+    UW = apply && uw_ ? uw_ : hq_upd_slab(nb, prec);
+    NCX = apply ? ncol_ : 0;
     SU = (nb + UW - 1) / UW;
     KT = std::min(MT, NT);
     NBLK = (nb + ib - 1) / ib;
@@ -181,16 +186,19 @@ struct DagBuilder {
     }
     x.dep[x.ndep++] = d;
   }
+  bool live_slab(int j, int s) const { return NCX <= 0 || j * nb + s * UW < NCX; }
   void build_apply() {
     for (int k = 0; k < KT; k++) {
       for (int j = 0; j < XNT; j++)
         for (int s = 0; s < SU; s++) {
+          if (!live_slab(j, s)) continue;
           int l = L(k, j, s) = add(HQ_LARFB, k, k, j, s);
           if (k > 0) dep(l, Sr(k - 1, j, k, s));
         }
       for (int r = k + 1; r < MT; r++)
         for (int j = 0; j < XNT; j++)
           for (int s = 0; s < SU; s++) {
+            if (!live_slab(j, s)) continue;
             int x = Sr(k, j, r, s) = add(HQ_SSRFB, k, r, j, s);
             dep(x, r == k + 1 ? L(k, j, s) : Sr(k, j, r - 1, s));
             if (k > 0) dep(x, Sr(k - 1, j, r, s));
@@ -394,7 +402,7 @@ struct Context {
   std::mutex host_mu;
   std::mutex lane_mu;
   std::vector<std::unique_ptr<Lane>> lanes;  // idle lanes
-  std::map<std::tuple<int, int, int, int, int, int>, std::unique_ptr<CachedSched>> scheds;
+  std::map<std::tuple<int, int, int, int, int, int, int, int>, std::unique_ptr<CachedSched>> scheds;
   std::map<std::pair<int, int>, std::unique_ptr<DevBuf>> tilelists;  // (MT, NT) -> int2 list
   // batches: key = (nb, ib, MT_0, NT_0, MT_1, NT_1, ...)
   std::map<std::vector<int>, std::unique_ptr<CachedSched>> batch_scheds;
@@ -485,17 +493,20 @@ std::vector<hq_task> claim_order(std::vector<hq_task> &&tl, int nb, int ib, int 
   return out;
 }
 
-int get_schedule(Context *c, int MT, int NT, int nb, int ib, CachedSched **out, int XNT = -1, int prec = 0) {
+int get_schedule(Context *c, int MT, int NT, int nb, int ib, CachedSched **out, int XNT = -1, int prec = 0,
+                 int uw = 0, int ncol = 0) {
   // XNT < 0: factorisation DAG (prec: precision of its updates); XNT >= 1:
-  // Q^T application to MT x XNT tiles (FP64)
+  // Q^T application to MT x XNT tiles (FP64) in slabs of uw columns (0: the
+  // factorisation's) over X's first ncol columns (0: all)
   if (XNT >= 1) prec = 0;
-  auto key = std::make_tuple(MT, NT, nb, ib, XNT, prec);
+  else uw = ncol = 0;
+  auto key = std::make_tuple(MT, NT, nb, ib, XNT, prec, uw, ncol);
   auto it = c->scheds.find(key);
   if (it != c->scheds.end()) {
     *out = it->second.get();
     return HQ_OK;
   }
-  DagBuilder b(MT, NT, nb, ib, XNT >= 1, XNT, true, prec);
+  DagBuilder b(MT, NT, nb, ib, XNT >= 1, XNT, true, prec, uw, ncol);
   b.build();
   int grid = c->num_cu > 0 ? c->num_cu : 256;
   std::vector<int> ord = b.order(grid);
@@ -635,8 +646,12 @@ int build_t_impl(Context *c, int MT, int NT, int nb, int ib, const double *RY, c
 // diagonal blocks of T for any ib); without T (or with nb = 64, whose update
 // kernel stages whole ib blocks of at most HQ_IBMAX), T is rebuilt from RY
 // and TAU with the inner block width.
+// ncol > 0: only X's first ncol columns hold entries (the rest hold zeros
+// and still do afterwards); width 16 runs the updates as 16-column tasks on
+// the generic kernels (hq_geom::uw), 0 the factorisation's slabs.
 int apply_qt_impl(int MT, int NT, int nb, int ib, const double *RY, const double *TAU,
-                  const double *T, double *X, int XNT, void *stream) {
+                  const double *T, double *X, int XNT, void *stream, int ncol = 0, int width = 0,
+                  long long *trace = nullptr) {
   if (!geometry_ok(MT, NT, nb, ib)) return g_err;
   if (XNT < 1) return set_error(HQ_ERR_ARG, "X needs at least one tile column");
   if (!RY || !TAU || !X) return set_error(HQ_ERR_ARG, "RY, TAU and X must be device pointers");
@@ -646,18 +661,20 @@ int apply_qt_impl(int MT, int NT, int nb, int ib, const double *RY, const double
   if (order_in(c, st) != HQ_OK) return g_err;
   double *Tw = (double *)T;
   int iq = ib;
-  if (Tw && (nb < 128 || !tuned_nb(nb)) && ib > HQ_IBMAX) Tw = nullptr;
+  if (Tw && (nb < 128 || !tuned_nb(nb) || width == 16) && ib > HQ_IBMAX) Tw = nullptr;
   if (!Tw) {
     iq = ibi(ib);
     if (ensure_T(c, MT, NT, nb, iq, &Tw) != HQ_OK) return g_err;
     if (build_t_impl(c, MT, NT, nb, iq, RY, TAU, Tw, st) != HQ_OK) return g_err;
   }
   CachedSched *cs;
-  if (get_schedule(c, MT, NT, nb, iq, &cs, XNT) != HQ_OK) return g_err;
+  if (get_schedule(c, MT, NT, nb, iq, &cs, XNT, 0, width, ncol) != HQ_OK) return g_err;
   hq_geom G{(double *)RY, (double *)TAU, Tw, MT, NT, nb, iq, X, XNT};
+  G.ncol = ncol;
+  G.uw = width;
   c->epoch++;
   HQ_HIP(hq_launch_sched(G, nullptr, (const hq_task *)cs->tasks.p, cs->ntasks, (int *)c->flags.p,
-                         c->epoch, (int *)c->counter.p, cs->grid, nullptr, st, nullptr, c->wait_limit));
+                         c->epoch, (int *)c->counter.p, cs->grid, trace, st, nullptr, c->wait_limit));
   if (order_out(c, st) != HQ_OK) return g_err;
   return ok();
 }
@@ -1261,6 +1278,33 @@ int hq_tile_build_t_dev(int MT, int NT, int nb, int ib, const double *RY, const 
 int hq_tile_ormqr_dev(int MT, int NT, int nb, int ib, const double *RY, const double *TAU,
                       const double *T, double *X, int XNT, void *stream) {
   return apply_qt_impl(MT, NT, nb, ib, RY, TAU, T, X, XNT, stream);
+}
+
+// Narrow X: width 0 picks 16-column tasks when X holds fewer than
+// HQ_NARROW_COLS columns (a short block of right-hand sides, whose few
+// column chains along the flat tree are the whole critical path), else the
+// factorisation's slabs.
+static int ormqr_cols(int MT, int NT, int nb, int ib, const double *RY, const double *TAU, const double *T,
+                      double *X, int XNT, int ncol, int width, long long *trace, void *stream) {
+  if (XNT < 1) return set_error(HQ_ERR_ARG, "X needs at least one tile column");
+  if (ncol <= (XNT - 1) * nb || ncol > XNT * nb)
+    return set_error(HQ_ERR_ARG, "ncol=%d outside tile column %d of X (nb=%d)", ncol, XNT, nb);
+  if (width != 0 && width != 16 && width != 64)
+    return set_error(HQ_ERR_ARG, "update width %d (0, 16 or 64)", width);
+  if (width == 0) width = ncol < HQ_NARROW_COLS ? 16 : 64;
+  return apply_qt_impl(MT, NT, nb, ib, RY, TAU, T, X, XNT, stream, ncol, width == 16 ? 16 : 0, trace);
+}
+
+int hq_tile_ormqr_cols_dev(int MT, int NT, int nb, int ib, const double *RY, const double *TAU,
+                           const double *T, double *X, int XNT, int ncol, int width, void *stream) {
+  return ormqr_cols(MT, NT, nb, ib, RY, TAU, T, X, XNT, ncol, width, nullptr, stream);
+}
+
+int hq_tile_ormqr_cols_traced_dev(int MT, int NT, int nb, int ib, const double *RY, const double *TAU,
+                                  const double *T, double *X, int XNT, int ncol, int width, long long *trace,
+                                  void *stream) {
+  if (!trace) return set_error(HQ_ERR_ARG, "trace must be a device buffer of 4*ntasks int64");
+  return ormqr_cols(MT, NT, nb, ib, RY, TAU, T, X, XNT, ncol, width, trace, stream);
 }
 
 int hq_tile_orgqr_dev(int MT, int NT, int nb, int ib, const double *RY, const double *TAU,

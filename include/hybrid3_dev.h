@@ -184,6 +184,27 @@ int hq_tile_ormqr_dev(int MT, int NT, int nb, int ib, const double *RY,
                       const double *TAU, const double *T, double *X, int XNT,
                       void *stream);
 
+/* hq_tile_ormqr_dev for an X whose first ncol columns hold entries
+ * ((XNT-1)*nb < ncol <= XNT*nb) and whose other columns hold zeros (they
+ * still do on return): the C argument of Hybrid_dormqr (Hybrid_dormqr.c:
+ * 20-171) when it is a short block of right-hand sides.  Tasks that would
+ * update only zero columns are left out.  width: 16 runs the updates as
+ * 16-column tasks (many short chains along the flat tree for a narrow X),
+ * 64 as the factorisation's update slabs, 0 picks 16 below 256 columns.
+ * Errors: HQ_ERR_ARG for ncol outside X's last tile column or another
+ * width. */
+int hq_tile_ormqr_cols_dev(int MT, int NT, int nb, int ib, const double *RY,
+                           const double *TAU, const double *T, double *X,
+                           int XNT, int ncol, int width, void *stream);
+
+/* hq_tile_ormqr_cols_dev with the per-task timeline of hq_tile_qr_dev_traced
+ * (trace: device, 4*ntasks int64; tasks in claim order). */
+int hq_tile_ormqr_cols_traced_dev(int MT, int NT, int nb, int ib,
+                                  const double *RY, const double *TAU,
+                                  const double *T, double *X, int XNT,
+                                  int ncol, int width, long long *trace,
+                                  void *stream);
+
 /* Q (MT x NT tiles) <- Q^T (the array Hybrid_dorgqr returns, Hybrid_dorgqr.c:
  * 56-138): identity, then hq_tile_ormqr_dev. */
 int hq_tile_orgqr_dev(int MT, int NT, int nb, int ib, const double *RY,
