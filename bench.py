@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--only-panels", action="store_true", help="only the panel-kernel measurement (profiling runs)")
     ap.add_argument("--only-batched", action="store_true",
                     help="only the batched uneven-QR measurement, FP64 and FP32 matrices (profiling runs)")
+    ap.add_argument("--only-tall", action="store_true",
+                    help="only the 65536 x 4096 QR / dorgqr / dormqr measurement (profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only a 2 GiB device copy (calibrates FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--traffic-file",
@@ -544,6 +546,9 @@ def main():
 
     if args.only_panels:
         print(json.dumps(panel_bench(args.nb, args.ib)))
+        return
+    if args.only_tall:
+        print(json.dumps(tall_bench(nb=args.nb, ib=args.ib)))
         return
     if args.only_batched:
         # (--no-check: no per-matrix checks and no per-matrix launches, for

@@ -245,9 +245,16 @@ __device__ __forceinline__ int group_rows(int nb) { return (((nb + R - 1) / R) +
 // SN = 4 (64-column slabs, two row halves) for the factorisation; SN = 1
 // (one 16-column strip over 8 row groups) for a narrow X, whose chain of
 // tasks along the flat tree then runs on a fraction of the work each.
+// Its pointers are cast to the global address space (the compiler keeps
+// this function out of line, where they would be generic: a flat access also
+// counts against the LDS counter, so every wait for an LDS read would wait
+// for the V loads in flight as well).
+typedef __attribute__((address_space(1))) double gdouble;
 template <bool TS, int SN = 4>
-__device__ void update_strips(Smem &S, const double *Vt, const double *T, int ib, double *X, double *X1, int nb,
+__device__ void update_strips(Smem &S, const double *Vt_, const double *T_, int ib, double *X_, double *X1_, int nb,
                               int b0, int b1, int c0, int c1) {
+  const gdouble *Vt = (const gdouble *)Vt_, *T = (const gdouble *)T_;
+  gdouble *X = (gdouble *)X_, *X1 = (gdouble *)X1_;
   constexpr int RG = NW / SN;              // row groups
   constexpr int RBM = NBMAX / 16 / RG;     // 16-row blocks of a group at most
   const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, g = lane >> 4;
@@ -268,9 +275,9 @@ __device__ void update_strips(Smem &S, const double *Vt, const double *T, int ib
     acc[rb] = d4{0.0, 0.0, 0.0, 0.0};
     if (rb < nrb && colok) {
       const int r = rbase + 16 * rb + 4 * g;
-      const double *xp = X + (size_t)col * nb + r;
+      const gdouble *xp = X + (size_t)col * nb + r;
       if (r >= row_lo && r + 3 < nb) {
-        acc[rb] = *reinterpret_cast<const d4 *>(xp);
+        acc[rb] = *reinterpret_cast<const __attribute__((address_space(1))) d4 *>(xp);
       } else {
 #pragma unroll
         for (int v = 0; v < 4; v++)
@@ -376,9 +383,9 @@ __device__ void update_strips(Smem &S, const double *Vt, const double *T, int ib
   for (int rb = 0; rb < RBM; rb++) {
     if (rb < nrb && colok) {
       const int r = rbase + 16 * rb + 4 * g;
-      double *xp = X + (size_t)col * nb + r;
+      gdouble *xp = X + (size_t)col * nb + r;
       if (r >= row_lo && r + 3 < nb) {
-        *reinterpret_cast<d4 *>(xp) = acc[rb];
+        *reinterpret_cast<__attribute__((address_space(1))) d4 *>(xp) = acc[rb];
       } else {
 #pragma unroll
         for (int v = 0; v < 4; v++)
@@ -403,7 +410,6 @@ __device__ void update_strips(Smem &S, const double *Vt, const double *T, int ib
 // own; its pointers are then generic, and are cast to the global address
 // space (flat loads would also count against the LDS counter, so every wait
 // for an LDS read would wait for the loads in flight as well).
-typedef __attribute__((address_space(1))) double gdouble;
 constexpr int NLD = 17;  // column stride of a wave's LDS copy of V / T (odd: fewer bank conflicts)
 template <bool TS, int RB, int PD>
 __device__ __noinline__ void update_narrow(Smem &S, const double *Vt_, const double *T_, int ib, double *X_,

@@ -42,9 +42,11 @@ def narrow_block(orc, MT, NT, nb, XNT, ncol, seed):
     return Xt, full
 
 
-# (MT, NT, nb, ib): specialised tile sizes, generic ones, and ib above the
-# update kernels' block width (T rebuilt with ib = 32)
-GEOMS = [(4, 3, 64, 16), (4, 4, 128, 32), (5, 3, 256, 32), (3, 2, 96, 32), (2, 2, 512, 32), (3, 2, 128, 64)]
+# (MT, NT, nb, ib): specialised tile sizes, generic ones, ib above the update
+# kernels' block width and ib not a multiple of 16 (16-column tasks rebuild
+# T with ib = 32)
+GEOMS = [(4, 3, 64, 16), (4, 4, 128, 32), (5, 3, 256, 32), (3, 2, 96, 32), (2, 2, 512, 32), (3, 2, 128, 64),
+         (3, 2, 128, 24)]
 
 
 @pytest.mark.parametrize("width", [16, 64])

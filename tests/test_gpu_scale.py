@@ -100,6 +100,15 @@ def test_tall_65536x4096_orgqr_ormqr(hq):
     hq.tile_ormqr_dev(MT, NT, nb, ib, A, TAU, C64.view(-1), 1, T=T)
     hq.sync()
     assert (C64[:, :64, :] - qc[:, :64, :]).abs().max().item() / rmax < 1e-12
+    # the same block through the narrow-C entry (16-column tasks): the same
+    # columns to rounding, the zero columns untouched
+    C64n = torch.zeros(MT * nb * nb, dtype=torch.float64, device="cuda").view(MT, nb, nb)
+    C64n[:, :64, :] = pristine[: MT * nb * nb].view(MT, nb, nb)[:, :64, :]
+    hq.tile_ormqr_cols_dev(MT, NT, nb, ib, A, TAU, C64n.view(-1), 1, 64, T=T)
+    hq.sync()
+    assert (C64n[:, :64, :] - qc[:, :64, :]).abs().max().item() / rmax < 1e-12
+    assert C64n[:, 64:, :].abs().max().item() == 0.0
+    del C64n
     # dorgqr: the Q array (Q^T applied to the first n columns of I) has
     # orthonormal columns; a sample of 64 of them
     Q = torch.empty_like(pristine)
