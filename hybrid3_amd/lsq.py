@@ -32,7 +32,7 @@ from __future__ import annotations
 
 import time
 
-from . import (batched_prepare, batched_tile_qr_ex_dev, cm_to_tile_dev, sync, tile_ormqr_dev, tile_qr_dev,
+from . import (batched_prepare, batched_tile_qr_ex_dev, cm_to_tile_dev, sync, tile_ormqr_cols_dev, tile_qr_dev,
                tile_qr_mixed_dev, tile_qr_prepare, tile_qr_split_dev, tile_to_cm_dev)
 
 __all__ = ["TileQR", "lstsq", "batched_lstsq", "backward_indicator", "gaussian_orthonormal", "svd_matrix",
@@ -82,13 +82,16 @@ class TileQR:
         self.R = torch.triu(top.permute(1, 3, 0, 2).reshape(NT * nb, NT * nb)[:n, :n]).contiguous()
 
     def qt(self, Y):
-        """Q^T Y for a column-major m x k device block (k <= nb)."""
+        """Q^T Y for a column-major m x k device block (k <= nb): the k
+        columns in one zero-padded tile column, applied by the narrow-C
+        entry (tasks only on the slabs that hold columns; 16-column tasks
+        below 256 columns)."""
         import torch
         m, k = Y.shape
         X = torch.zeros(self.MT * self.nb * self.nb, dtype=torch.float64, device=Y.device)
         Yc = Y.t().contiguous()  # column-major storage of Y (k x m row-major)
         cm_to_tile_dev(self.nb, [m], [k], [_addr(Yc)], [m], [X])
-        tile_ormqr_dev(self.MT, self.NT, self.nb, self.ib, self.RY, self.TAU, X, 1, T=self.T)
+        tile_ormqr_cols_dev(self.MT, self.NT, self.nb, self.ib, self.RY, self.TAU, X, 1, k, T=self.T)
         out = torch.empty(k * m, dtype=torch.float64, device=Y.device)
         tile_to_cm_dev(self.nb, [m], [k], [X], [out], [m])
         sync()
