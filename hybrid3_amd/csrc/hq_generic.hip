@@ -411,9 +411,27 @@ __device__ void update_strips(Smem &S, const double *Vt_, const double *T_, int 
 // space (flat loads would also count against the LDS counter, so every wait
 // for an LDS read would wait for the loads in flight as well).
 constexpr int NLD = 17;  // column stride of a wave's LDS copy of V / T (odd: fewer bank conflicts)
+// Sub-block pipelining along a chain of SSRFBs on one strip: sub-block s of
+// SSRFB(k, r) reads and writes only rows 16s .. 16s+15 of X1, so it can run
+// once SSRFB(k, r-1) has written those rows.  prog: this task's progress
+// word (published, release-ordered, after every pubg-th sub-block's X1 rows
+// and after the last sub-block's), wait: the predecessor's (or null), both
+// (epoch << 32) | sub-blocks done.  A wait is bounded by wait_limit sleeps;
+// past it the task index goes to err (the scheduler's time-out word) and the
+// task runs on.
+struct Pipe {
+  int pubg;
+  unsigned long long *prog;
+  const unsigned long long *wait;
+  unsigned long long epoch;
+  unsigned wait_limit;
+  int *err;
+  int self;
+};
+
 template <bool TS, int RB, int PD>
 __device__ __noinline__ void update_narrow(Smem &S, const double *Vt_, const double *T_, int ib, double *X_,
-                                           double *X1_, int nb, int c0, int c1) {
+                                           double *X1_, int nb, int c0, int c1, Pipe pp) {
   const gdouble *Vt = (const gdouble *)Vt_, *T = (const gdouble *)T_;
   gdouble *X = (gdouble *)X_, *X1 = (gdouble *)X1_;
   const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, g = lane >> 4;
@@ -441,6 +459,9 @@ __device__ __noinline__ void update_narrow(Smem &S, const double *Vt_, const dou
     }
   }
   const int nsb_per = (ib + 15) / 16, nsb = ((nb + ib - 1) / ib) * nsb_per;
+  // sub-blocks that hold reflectors (a narrower last block has fewer)
+  const int nsb_live = (nb / ib) * nsb_per + (nb % ib + 15) / 16;
+  const int pubg = max(1, pp.pubg);
   // loaded layout: lane l holds reflector lc = l / 4, rows 4 (l % 4) + e
   const int lc = lane >> 2, lr = 4 * (lane & 3);
   double vr[PD][RB][4], tr[PD][4];
@@ -499,17 +520,29 @@ __device__ __noinline__ void update_narrow(Smem &S, const double *Vt_, const dou
 #pragma unroll
         for (int v = 0; v < 4; v++) wp = mfma(Vw[rb * 16 * NLD + m * NLD + 4 * g + v], acc[rb][v], wp);
       }
-    // TS: the sub-block's rows of X1 (read before the barrier; group 0
-    // writes them after it)
+    double *Wb = Wx + (sbi & 1) * (NW * 256);
+    *reinterpret_cast<d4 *>(Wb + h * 256 + 4 * lane) = wp;
+    // TS, pipelined: the predecessor's rows of this sub-block written
+    if (TS && pp.wait && tid == 0) {
+      const unsigned long long need = (pp.epoch << 32) | (unsigned long long)(sbi + 1);
+      unsigned spins = 0;
+      while (__hip_atomic_load(pp.wait, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > pp.wait_limit || __hip_atomic_load(pp.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+          __hip_atomic_store(pp.err, 1 + pp.self, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+    // TS: the sub-block's rows of X1 (group 0 writes them below)
     d4 x1 = d4{0.0, 0.0, 0.0, 0.0};
     if (TS && colok) {
 #pragma unroll
       for (int v = 0; v < 4; v++)
         if (4 * v + g < nq) x1[v] = X1[(size_t)col * nb + ii + q0 + 4 * v + g];
     }
-    double *Wb = Wx + (sbi & 1) * (NW * 256);
-    *reinterpret_cast<d4 *>(Wb + h * 256 + 4 * lane) = wp;
-    __syncthreads();
     d4 w = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int qg = 0; qg < NW; qg++) w += *reinterpret_cast<const d4 *>(Wb + qg * 256 + 4 * lane);
@@ -522,6 +555,14 @@ __device__ __noinline__ void update_narrow(Smem &S, const double *Vt_, const dou
 #pragma unroll
       for (int v = 0; v < 4; v++)
         if (4 * v + g < nq) X1[(size_t)col * nb + ii + q0 + 4 * v + g] = x1[v] - w2[v];
+    }
+    if (TS && pp.prog && h == 0 && ((sbi + 1) % pubg == 0 || sbi + 1 == nsb_live)) {
+      // the X1 rows of the sub-blocks up to this one, visible before the word
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __builtin_amdgcn_s_waitcnt(0);
+      if (lane == 0)
+        __hip_atomic_store(pp.prog, (pp.epoch << 32) | (unsigned long long)(sbi + 1), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
     }
     const d4 nw2 = -w2;
     // X -= V W2: A(i, k) = V(row arow(i) of block rb, reflector k)
@@ -585,7 +626,8 @@ __device__ void pad_panel_blocks(double *T, double *TAU, int nb, int ib, int ii)
 // hq_geom::uw; update_narrow with NRB row blocks per group), a kernel of its
 // own.
 template <int NRB = 0>
-__device__ void run_task(Smem &S, const hq_geom &G, const hq_task &t) {
+__device__ void run_task(Smem &S, const hq_geom &G, const hq_task &t, int self = -1, int epoch = 0,
+                         unsigned wait_limit = 0, int *err = nullptr) {
   constexpr bool NARROW = NRB > 0;
   constexpr int PD = NRB <= 2 ? 4 : 3;  // sub-blocks in flight (registers)
   const int nb = G.nb, ib = G.ib;
@@ -651,7 +693,7 @@ __device__ void run_task(Smem &S, const hq_geom &G, const hq_task &t) {
       const int uw = NARROW ? 16 : hq_upd_slab(nb, 0), c0 = t.slab * uw, c1 = min(live_cols(G, t.j), c0 + uw);
       if (c0 >= c1) break;
       if constexpr (NARROW)
-        update_narrow<false, (NARROW ? NRB : 1), PD>(S, V, T, ib, X, nullptr, nb, c0, c1);
+        update_narrow<false, (NARROW ? NRB : 1), PD>(S, V, T, ib, X, nullptr, nb, c0, c1, Pipe{});
       else
         update_strips<false>(S, V, T, ib, X, nullptr, nb, 0, (nb + ib - 1) / ib, c0, c1);
       break;
@@ -661,9 +703,21 @@ __device__ void run_task(Smem &S, const hq_geom &G, const hq_task &t) {
       double *X1 = xtile_ptr(G, t.k, t.j), *X2 = xtile_ptr(G, t.r, t.j);
       const int uw = NARROW ? 16 : hq_upd_slab(nb, 0), c0 = t.slab * uw, c1 = min(live_cols(G, t.j), c0 + uw);
       if (c0 >= c1) break;
-      if constexpr (NARROW)
-        update_narrow<true, (NARROW ? NRB : 1), PD>(S, V, T, ib, X2, X1, nb, c0, c1);
-      else
+      if constexpr (NARROW) {
+        // dep[0] = -2 - predecessor: the previous SSRFB of the chain, waited
+        // for sub-block by sub-block
+        Pipe pp{};
+        if (G.prog && self >= 0) {
+          pp.prog = G.prog + self;
+          pp.pubg = G.pubg;
+          pp.wait = t.ndep > 0 && t.dep[0] < 0 ? G.prog + (-2 - t.dep[0]) : nullptr;
+          pp.epoch = (unsigned long long)(unsigned)epoch;
+          pp.wait_limit = wait_limit;
+          pp.err = err;
+          pp.self = self;
+        }
+        update_narrow<true, (NARROW ? NRB : 1), PD>(S, V, T, ib, X2, X1, nb, c0, c1, pp);
+      } else
         update_strips<true>(S, V, T, ib, X2, X1, nb, 0, (nb + ib - 1) / ib, c0, c1);
       break;
     }
@@ -707,9 +761,10 @@ __global__ __launch_bounds__(NT) void sched_kernel(hq_geom G0, const hq_geom *__
         for (;;) {
           int f[HQ_MAXDEP];
 #pragma unroll
-          for (int i = 0; i < HQ_MAXDEP; i++)
-            f[i] = i < nd ? __hip_atomic_load(&flags[tk.dep[i]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                          : epoch;
+          for (int i = 0; i < HQ_MAXDEP; i++)  // (a negative dep is waited for inside the task)
+            f[i] = i < nd && tk.dep[i] >= 0
+                       ? __hip_atomic_load(&flags[tk.dep[i]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                       : epoch;
           bool ready = true;
 #pragma unroll
           for (int i = 0; i < HQ_MAXDEP; i++) ready = ready && f[i] == epoch;
@@ -735,13 +790,14 @@ __global__ __launch_bounds__(NT) void sched_kernel(hq_geom G0, const hq_geom *__
     tk.j = __builtin_amdgcn_readfirstlane(S.next.j);
     tk.slab = __builtin_amdgcn_readfirstlane(S.next.slab);
     tk.mat = __builtin_amdgcn_readfirstlane(S.next.mat);
-    tk.ndep = 0;
+    tk.ndep = __builtin_amdgcn_readfirstlane(S.next.ndep);
+    tk.dep[0] = __builtin_amdgcn_readfirstlane(S.next.dep[0]);
     if (trace && threadIdx.x == 0) {
       trace[4 * t + 0] = t_claim;
       trace[4 * t + 1] = wall_clock64();
       trace[4 * t + 3] = blockIdx.x;
     }
-    run_task<NRB>(S, geoms ? geoms[tk.mat] : G0, tk);
+    run_task<NRB>(S, geoms ? geoms[tk.mat] : G0, tk, t, epoch, wait_limit, counter + 1);
     // every wave's stores have completed before the flag can be published
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();

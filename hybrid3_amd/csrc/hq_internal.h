@@ -82,6 +82,14 @@ struct hq_geom {
   // default (hq_upd_slab): 16 for the Q^T application to a narrow X, which
   // runs on the generic kernels with one 16-column strip per workgroup.
   int uw;
+  // With uw = 16: per-task progress words, (epoch << 32) | (sub-blocks whose
+  // X1 rows are written); an SSRFB whose dep[0] is negative (-2 - task) runs
+  // each 16-reflector sub-block once that task has written the same rows of
+  // X1.  nullptr otherwise.
+  unsigned long long *prog;
+  // sub-blocks per published progress word (0 or 1: every sub-block; fewer
+  // words cost each task fewer L2 write-backs, more chains lose less to them)
+  int pubg;
 };
 
 // halves of split images per tile: one image of 64 nb halves per 16-reflector
@@ -101,7 +109,8 @@ struct hq_bmat {
 #define HQ_THREADS 256
 #define HQ_SLAB 64
 // Q^T applied to fewer columns than this runs as 16-column update tasks
-#define HQ_NARROW_COLS 256
+// (their pipelined chains beat the 64-column slabs up to about 9 chains)
+#define HQ_NARROW_COLS 160
 
 // Column width of one LARFB / SSRFB task.  FP64 and split updates: 64 (one
 // 16-column strip per wave).  FP32 updates: 128 (two strips per wave), so

@@ -138,6 +138,9 @@ struct DagBuilder {
   int prec;  // precision of the LARFB / SSRFB updates (their slab width and cost)
   int UW, SU;  // LARFB / SSRFB slab width (hq_upd_slab) and slabs per tile
   int NCX;     // apply mode: X's columns that hold entries (0: all)
+  // pipe[i]: the task that task i follows sub-block by sub-block (its dep[0],
+  // the previous SSRFB of a 16-column chain; -1 otherwise)
+  std::vector<int> pipe;
   std::vector<hq_task> t;
   std::vector<int> idl, ids;
   // panel tasks of the current step k: (row r, block b); their trailing
@@ -175,6 +178,7 @@ struct DagBuilder {
     x.slab = s;
     x.ndep = 0;
     t.push_back(x);
+    pipe.push_back(-1);
     return (int)t.size() - 1;
   }
   void dep(int id, int d) {
@@ -202,6 +206,7 @@ struct DagBuilder {
             int x = Sr(k, j, r, s) = add(HQ_SSRFB, k, r, j, s);
             dep(x, r == k + 1 ? L(k, j, s) : Sr(k, j, r - 1, s));
             if (k > 0) dep(x, Sr(k - 1, j, r, s));
+            if (UW == 16 && r > k + 1) pipe[x] = Sr(k, j, r - 1, s);
           }
     }
   }
@@ -300,21 +305,30 @@ struct DagBuilder {
   // List-scheduling simulation on P workers with bottom-level priorities;
   // the dispatch order is a topological order in which tasks are claimed
   // roughly when they become ready.
+  // A pipelined link (pipe) lets the successor start when its predecessor
+  // starts and end no earlier than one sub-block after the predecessor ends.
   std::vector<int> order(int P) {
     const int n = (int)t.size();
-    std::vector<std::vector<int>> succ(n);
+    const bool piped = (int)pipe.size() == n;
+    const double nsub = std::max(1, nb / 16);
+    std::vector<std::vector<int>> succ(n), psucc(n);
     std::vector<int> indeg(n, 0);
     for (int i = 0; i < n; i++)
       for (int d = 0; d < t[i].ndep; d++) {
-        succ[t[i].dep[d]].push_back(i);
+        if (piped && pipe[i] == t[i].dep[d])
+          psucc[t[i].dep[d]].push_back(i);
+        else
+          succ[t[i].dep[d]].push_back(i);
         indeg[i]++;
       }
     std::vector<double> bl(n, 0.0);
     for (int i = n - 1; i >= 0; i--) {
-      double mx = 0.0;
+      double mx = 0.0, mp = 0.0;
       for (int s : succ[i]) mx = std::max(mx, bl[s]);
-      bl[i] = cost(t[i]) + mx;
+      for (int s : psucc[i]) mp = std::max(mp, cost(t[s]) / nsub + bl[s]);
+      bl[i] = std::max(cost(t[i]) + mx, mp);
     }
+    std::vector<double> fin(n, 0.0);
     typedef std::pair<double, int> PI;
     std::priority_queue<PI> ready;  // max bottom level first
     std::priority_queue<PI, std::vector<PI>, std::greater<PI>> running;  // (finish, id)
@@ -329,8 +343,12 @@ struct DagBuilder {
         int id = -ready.top().second;
         ready.pop();
         out.push_back(id);
-        running.push(PI(now + cost(t[id]), id));
+        fin[id] = now + cost(t[id]);
+        if (piped && pipe[id] >= 0) fin[id] = std::max(fin[id], fin[pipe[id]] + cost(t[id]) / nsub);
+        running.push(PI(fin[id], id));
         busy++;
+        for (int s : psucc[id])
+          if (--indeg[s] == 0) ready.push(PI(bl[s], -s));
       }
       if (running.empty()) break;  // cannot happen for a DAG
       PI f = running.top();
@@ -386,6 +404,7 @@ struct Context {
   int num_cu = 0;
   hipStream_t stream = nullptr;
   DevBuf flags, counter, T, A, TAU, X, T1;
+  DevBuf prog;  // per-task progress words of the 16-column apply (hq_geom::prog)
   DevBuf W, VD;  // W = V T16 and unit lower V of the diagonal tiles (FP64 factorisation)
   DevBuf VH;     // FP16 hi / lo images of the reflector sub-blocks (split factorisation)
   DevBuf CE;     // per-column scale exponents (split factorisation)
@@ -464,12 +483,15 @@ int take_device_faults(int *counter) {
 // entry points pass the library's own stream explicitly.
 hipStream_t pick_stream(Context *, void *s) { return (hipStream_t)s; }
 
-// flags: one int per task, zero-initialised; epochs start at 1
+// flags: one int per task, and the progress words (hq_geom::prog), one
+// 64-bit word per task, zero-initialised together; epochs start at 1 (both
+// hold epochs, so both are cleared whenever the epoch restarts)
 int reserve_flags(Context *c, int n) {
-  size_t fbytes = sizeof(int) * (size_t)n;
-  if (c->flags.bytes < fbytes) {
-    if (c->flags.reserve(fbytes) != HQ_OK) return g_err;
+  size_t fbytes = sizeof(int) * (size_t)n, pbytes = sizeof(unsigned long long) * (size_t)n;
+  if (c->flags.bytes < fbytes || c->prog.bytes < pbytes) {
+    if (c->flags.reserve(fbytes) != HQ_OK || c->prog.reserve(pbytes) != HQ_OK) return g_err;
     HQ_HIP(hipMemset(c->flags.p, 0, c->flags.bytes));
+    HQ_HIP(hipMemset(c->prog.p, 0, c->prog.bytes));
     c->epoch = 0;
   }
   return HQ_OK;
@@ -517,6 +539,10 @@ int get_schedule(Context *c, int MT, int NT, int nb, int ib, CachedSched **out, 
   for (int i = 0; i < n; i++) {
     hq_task x = b.t[ord[i]];
     for (int d = 0; d < x.ndep; d++) x.dep[d] = pos[x.dep[d]];
+    // 16-column apply: an SSRFB below the first waits for the previous
+    // SSRFB of its chain (its dep[0], build_apply) sub-block by sub-block,
+    // inside the task (hq_geom::prog), marked -2 - index
+    if (XNT >= 1 && uw == 16 && x.kind == HQ_SSRFB && x.r > x.k + 1 && x.ndep > 0) x.dep[0] = -2 - x.dep[0];
     tl[i] = x;
   }
   std::unique_ptr<CachedSched> cs(new CachedSched());
@@ -672,6 +698,15 @@ int apply_qt_impl(int MT, int NT, int nb, int ib, const double *RY, const double
   hq_geom G{(double *)RY, (double *)TAU, Tw, MT, NT, nb, iq, X, XNT};
   G.ncol = ncol;
   G.uw = width;
+  G.prog = width == 16 ? (unsigned long long *)c->prog.p : nullptr;
+  // progress words per chain link: every sub-block for one or two chains
+  // (the chain's pace is the run), every fourth for more (the workgroups
+  // are then the limit, and each word costs the producer an L2 write-back);
+  // measured at 65536 x 64 after a 65536 x 4096 factorisation
+  {
+    const int chains = ncol > 0 ? (ncol + 15) / 16 : XNT * nb / 16;
+    G.pubg = chains <= 1 ? 1 : chains <= 2 ? 2 : 4;
+  }
   c->epoch++;
   HQ_HIP(hq_launch_sched(G, nullptr, (const hq_task *)cs->tasks.p, cs->ntasks, (int *)c->flags.p,
                          c->epoch, (int *)c->counter.p, cs->grid, trace, st, nullptr, c->wait_limit));
