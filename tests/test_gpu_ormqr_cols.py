@@ -108,3 +108,30 @@ def test_ormqr_cols_repeat_calls_reuse_schedules(hq, orc):
         hq.sync()
         got = orc.tile_to_cm(dX.cpu().numpy(), MT, 1, nb)
         assert np.abs(got - expect).max() / np.abs(expect).max() < 1e-12, (ncol, width)
+
+
+def test_ormqr_cols_wait_bound_reports_and_recovers(hq, orc):
+    """With the wait bound at one poll, the chain's waits (the scheduler's and
+    the sub-block waits inside a pipelined SSRFB) time out: the launch drains,
+    hq_sync reports it, and the next call with the bound restored is right."""
+    import torch
+    MT, NT, nb, ib = 12, 2, 128, 32
+    dA, dTAU, dT = factor(hq, orc, MT, NT, nb, ib, 16)
+    RY, TAU = dA.cpu().numpy(), dTAU.cpu().numpy()
+    Xt, full = narrow_block(orc, MT, NT, nb, 1, 16, 17)
+    expect = orc.tile_to_cm(orc.dormqr(RY, TAU, full, MT, NT, nb)[: Xt.size], MT, 1, nb)
+    L = hq.lib()
+    print("narrow apply wait bound: 1 poll", flush=True)
+    assert L.hq_set_wait_limit(1) == 0
+    try:
+        dX = torch.from_numpy(Xt.copy()).to("cuda")
+        hq.tile_ormqr_cols_dev(MT, NT, nb, ib, dA, dTAU, dX, 1, 16, T=dT, width=16)
+        with pytest.raises(hq.HQError, match="timed out"):
+            hq.sync()
+    finally:
+        assert L.hq_set_wait_limit(0) == 0
+    dX = torch.from_numpy(Xt.copy()).to("cuda")
+    hq.tile_ormqr_cols_dev(MT, NT, nb, ib, dA, dTAU, dX, 1, 16, T=dT, width=16)
+    hq.sync()
+    got = orc.tile_to_cm(dX.cpu().numpy(), MT, 1, nb)
+    assert np.abs(got - expect).max() / np.abs(expect).max() < 1e-12
