@@ -414,13 +414,13 @@ constexpr int NLD = 17;  // column stride of a wave's LDS copy of V / T (odd: fe
 // Sub-block pipelining along a chain of SSRFBs on one strip: sub-block s of
 // SSRFB(k, r) reads and writes only rows 16s .. 16s+15 of X1, so it can run
 // once SSRFB(k, r-1) has written those rows.  prog: this task's progress
-// word (published, release-ordered, after every pubg-th sub-block's X1 rows
-// and after the last sub-block's), wait: the predecessor's (or null), both
-// (epoch << 32) | sub-blocks done.  A wait is bounded by wait_limit sleeps;
-// past it the task index goes to err (the scheduler's time-out word) and the
-// task runs on.
+// word (published after each sub-block once its X1 rows are stored), wait:
+// the predecessor's (or null), both (epoch << 32) | sub-blocks done.  The X1
+// rows pass between the two tasks as agent-scope atomic stores and loads, so
+// the word needs neither an L2 write-back nor an invalidation.  A wait is
+// bounded by wait_limit sleeps; past it the task index goes to err (the
+// scheduler's time-out word) and the task runs on.
 struct Pipe {
-  int pubg;
   unsigned long long *prog;
   const unsigned long long *wait;
   unsigned long long epoch;
@@ -459,9 +459,6 @@ __device__ __noinline__ void update_narrow(Smem &S, const double *Vt_, const dou
     }
   }
   const int nsb_per = (ib + 15) / 16, nsb = ((nb + ib - 1) / ib) * nsb_per;
-  // sub-blocks that hold reflectors (a narrower last block has fewer)
-  const int nsb_live = (nb / ib) * nsb_per + (nb % ib + 15) / 16;
-  const int pubg = max(1, pp.pubg);
   // loaded layout: lane l holds reflector lc = l / 4, rows 4 (l % 4) + e
   const int lc = lane >> 2, lr = 4 * (lane & 3);
   double vr[PD][RB][4], tr[PD][4];
@@ -533,15 +530,20 @@ __device__ __noinline__ void update_narrow(Smem &S, const double *Vt_, const dou
           break;
         }
       }
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
-    // TS: the sub-block's rows of X1 (group 0 writes them below)
+    // TS: the sub-block's rows of X1 (group 0 writes them below).  With a
+    // pipelined chain they are read and written as agent-scope atomics
+    // (coherent across the XCDs' L2s without write-back or invalidation),
+    // which is all that passes between the two tasks
     d4 x1 = d4{0.0, 0.0, 0.0, 0.0};
     if (TS && colok) {
 #pragma unroll
       for (int v = 0; v < 4; v++)
-        if (4 * v + g < nq) x1[v] = X1[(size_t)col * nb + ii + q0 + 4 * v + g];
+        if (4 * v + g < nq) {
+          double *a = (double *)&X1[(size_t)col * nb + ii + q0 + 4 * v + g];
+          x1[v] = pp.prog ? __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *a;
+        }
     }
     d4 w = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -554,12 +556,19 @@ __device__ __noinline__ void update_narrow(Smem &S, const double *Vt_, const dou
     if (TS && h == 0 && colok) {
 #pragma unroll
       for (int v = 0; v < 4; v++)
-        if (4 * v + g < nq) X1[(size_t)col * nb + ii + q0 + 4 * v + g] = x1[v] - w2[v];
+        if (4 * v + g < nq) {
+          double *a = (double *)&X1[(size_t)col * nb + ii + q0 + 4 * v + g];
+          if (pp.prog)
+            __hip_atomic_store(a, x1[v] - w2[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          else
+            *a = x1[v] - w2[v];
+        }
     }
-    if (TS && pp.prog && h == 0 && ((sbi + 1) % pubg == 0 || sbi + 1 == nsb_live)) {
-      // the X1 rows of the sub-blocks up to this one, visible before the word
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (TS && pp.prog && h == 0) {
+      // this sub-block's X1 rows (atomic stores) complete before the word
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
       __builtin_amdgcn_s_waitcnt(0);
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
       if (lane == 0)
         __hip_atomic_store(pp.prog, (pp.epoch << 32) | (unsigned long long)(sbi + 1), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
@@ -709,7 +718,6 @@ __device__ void run_task(Smem &S, const hq_geom &G, const hq_task &t, int self =
         Pipe pp{};
         if (G.prog && self >= 0) {
           pp.prog = G.prog + self;
-          pp.pubg = G.pubg;
           pp.wait = t.ndep > 0 && t.dep[0] < 0 ? G.prog + (-2 - t.dep[0]) : nullptr;
           pp.epoch = (unsigned long long)(unsigned)epoch;
           pp.wait_limit = wait_limit;

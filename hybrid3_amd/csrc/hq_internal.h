@@ -87,9 +87,6 @@ struct hq_geom {
   // each 16-reflector sub-block once that task has written the same rows of
   // X1.  nullptr otherwise.
   unsigned long long *prog;
-  // sub-blocks per published progress word (0 or 1: every sub-block; fewer
-  // words cost each task fewer L2 write-backs, more chains lose less to them)
-  int pubg;
 };
 
 // halves of split images per tile: one image of 64 nb halves per 16-reflector
@@ -109,8 +106,8 @@ struct hq_bmat {
 #define HQ_THREADS 256
 #define HQ_SLAB 64
 // Q^T applied to fewer columns than this runs as 16-column update tasks
-// (their pipelined chains beat the 64-column slabs up to about 9 chains)
-#define HQ_NARROW_COLS 160
+// (their pipelined chains beat the 64-column slabs up to a 256-column tile)
+#define HQ_NARROW_COLS 257
 
 // Column width of one LARFB / SSRFB task.  FP64 and split updates: 64 (one
 // 16-column strip per wave).  FP32 updates: 128 (two strips per wave), so

@@ -699,14 +699,6 @@ int apply_qt_impl(int MT, int NT, int nb, int ib, const double *RY, const double
   G.ncol = ncol;
   G.uw = width;
   G.prog = width == 16 ? (unsigned long long *)c->prog.p : nullptr;
-  // progress words per chain link: every sub-block for one or two chains
-  // (the chain's pace is the run), every fourth for more (the workgroups
-  // are then the limit, and each word costs the producer an L2 write-back);
-  // measured at 65536 x 64 after a 65536 x 4096 factorisation
-  {
-    const int chains = ncol > 0 ? (ncol + 15) / 16 : XNT * nb / 16;
-    G.pubg = chains <= 1 ? 1 : chains <= 2 ? 2 : 4;
-  }
   c->epoch++;
   HQ_HIP(hq_launch_sched(G, nullptr, (const hq_task *)cs->tasks.p, cs->ntasks, (int *)c->flags.p,
                          c->epoch, (int *)c->counter.p, cs->grid, trace, st, nullptr, c->wait_limit));

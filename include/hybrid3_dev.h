@@ -191,7 +191,7 @@ int hq_tile_ormqr_dev(int MT, int NT, int nb, int ib, const double *RY,
  * update only zero columns are left out.  width: 16 runs the updates as
  * 16-column tasks (one chain per 16 columns along the flat tree, each SSRFB
  * following the previous one 16 reflectors at a time), 64 as the
- * factorisation's update slabs, 0 picks 16 below 160 columns.  Errors:
+ * factorisation's update slabs, 0 picks 16 up to 256 columns.  Errors:
  * HQ_ERR_ARG for ncol outside X's last tile column or another width. */
 int hq_tile_ormqr_cols_dev(int MT, int NT, int nb, int ib, const double *RY,
                            const double *TAU, const double *T, double *X,

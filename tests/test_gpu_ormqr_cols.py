@@ -72,7 +72,7 @@ def test_ormqr_cols_matches_oracle(hq, orc, MT, NT, nb, ib, width):
 
 @pytest.mark.parametrize("MT,NT,nb,ib", [(6, 2, 256, 32), (4, 4, 128, 32)])
 def test_ormqr_cols_widths_agree_and_auto_is_narrow(hq, orc, MT, NT, nb, ib):
-    """16-column and slab-wide updates agree to rounding; width 0 below 160
+    """16-column and slab-wide updates agree to rounding; width 0 up to 256
     columns is the 16-column run, bit for bit; the whole-X entry agrees."""
     import torch
     dA, dTAU, dT = factor(hq, orc, MT, NT, nb, ib, 12)
