@@ -681,6 +681,9 @@ int apply_qt_impl(int MT, int NT, int nb, int ib, const double *RY, const double
   if (!geometry_ok(MT, NT, nb, ib)) return g_err;
   if (XNT < 1) return set_error(HQ_ERR_ARG, "X needs at least one tile column");
   if (!RY || !TAU || !X) return set_error(HQ_ERR_ARG, "RY, TAU and X must be device pointers");
+  // a whole X of at most HQ_NARROW_COLS - 1 columns (dormqr on one tile
+  // column, dorgqr of a single tile column): 16-column tasks too
+  if (width == 0 && ncol == 0 && XNT * nb < HQ_NARROW_COLS && nb % 16 == 0) width = 16;
   Context *c;
   if (get_ctx(&c) != HQ_OK) return g_err;
   hipStream_t st = pick_stream(c, stream);

@@ -179,7 +179,9 @@ int hq_tile_build_t_dev(int MT, int NT, int nb, int ib, const double *RY,
 
 /* X (MT x XNT tiles) <- Q^T X with Q = H(0,0) H(1,0) ... from the tile QR
  * in RY / TAU (MT x NT tiles); T may be NULL (rebuilt).  The sweep of
- * Hybrid_dormqr.c:64-167 on the device DAG scheduler. */
+ * Hybrid_dormqr.c:64-167 on the device DAG scheduler.  An X of at most 256
+ * columns (nb a multiple of 16) runs as the 16-column tasks of
+ * hq_tile_ormqr_cols_dev. */
 int hq_tile_ormqr_dev(int MT, int NT, int nb, int ib, const double *RY,
                       const double *TAU, const double *T, double *X, int XNT,
                       void *stream);
