@@ -338,8 +338,8 @@ def tall_bench(m=65536, n=4096, nrhs=64, nb=256, ib=32, seed=20151130, reps=3):
     """Tall FP64 tile QR (m x n), the Q^T array of dorgqr (Q^T applied to the
     first n columns of I), and dormqr applying Q^T to an m x nrhs block (the
     first nrhs columns of one tile column, the rest zeros: the narrow-X entry
-    with its default 16-column tasks; the whole tile column through the
-    plain entry beside it), each timed with HIP events, best of `reps`.  Rates use the algorithmic counts
+    with its default task width; the whole tile column through the plain
+    entry beside it), each timed with HIP events, best of `reps`.  Rates use the algorithmic counts
     2mn^2 - 2n^3/3, 4mnk - 2(m+n)k^2 + 4k^3/3 (k = n) and 4 m nrhs k - 2 nrhs k^2.
     Check: Q^T A reproduces R in the top rows and zeros below."""
     import torch
@@ -394,7 +394,8 @@ def tall_bench(m=65536, n=4096, nrhs=64, nb=256, ib=32, seed=20151130, reps=3):
            "qr_ms": round(t_qr * 1e3, 3), "qr_tflops": round(f_qr / t_qr / 1e12, 3),
            "orgqr_ms": round(t_q * 1e3, 3), "orgqr_tflops": round(f_q / t_q / 1e12, 3),
            "ormqr_ms": round(t_c * 1e3, 3), "ormqr_tflops_algorithmic": round(f_c / t_c / 1e12, 3),
-           "ormqr_note": f"{nrhs} columns of one tile column, 16-column tasks (hq_tile_ormqr_cols_dev)",
+           "ormqr_note": f"{nrhs} columns of one tile column, hq_tile_ormqr_cols_dev with width 0 "
+                         "(32-column tasks, pipelined chains)",
            "ormqr_whole_tile_ms": round(t_cw * 1e3, 3),
            "check_QtA_top_rel": top_err, "check_QtA_below_rel": below, "check_zero_columns_kept": pad_zero}
     del pristine, A, TAU, T, Q, C, c0n

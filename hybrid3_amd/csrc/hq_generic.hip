@@ -429,7 +429,7 @@ struct Pipe {
   int self;
 };
 
-template <bool TS, int RB, int PD>
+template <bool TS, int RB, int PD, int NS = 1>
 __device__ __noinline__ void update_narrow(Smem &S, const double *Vt_, const double *T_, int ib, double *X_,
                                            double *X1_, int nb, int c0, int c1, Pipe pp) {
   const gdouble *Vt = (const gdouble *)Vt_, *T = (const gdouble *)T_;
@@ -438,26 +438,34 @@ __device__ __noinline__ void update_narrow(Smem &S, const double *Vt_, const dou
   const int h = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int RH = group_rows<NW>(nb), rbase = h * RH;
   const int nrb = max(0, min(RH, nb - rbase) + 15) >> 4;
-  const int col = c0 + m;
-  const bool colok = col < c1;
-  // LDS: partial W (2 x NW x 256), then per wave RB blocks of V (16 x NLD
-  // each) and the T block (16 x NLD)
-  static_assert(2 * NW * 256 + NW * (RB + 1) * 16 * NLD <= sizeof(S.P) / sizeof(double), "LDS areas");
+  // NS strips of 16 columns: strip q is column c0 + 16 q + m of this lane
+  int col[NS];
+  bool colok[NS];
+#pragma unroll
+  for (int q = 0; q < NS; q++) {
+    col[q] = c0 + 16 * q + m;
+    colok[q] = col[q] < c1;
+  }
+  // LDS: partial W (2 x NW x NS x 256), then per wave RB blocks of V (16 x
+  // NLD each) and the T block (16 x NLD)
+  static_assert(2 * NW * NS * 256 + NW * (RB + 1) * 16 * NLD <= sizeof(S.P) / sizeof(double), "LDS areas");
   double *Wx = S.P;
-  double *Vw = S.P + 2 * NW * 256 + h * (RB + 1) * 16 * NLD, *Tw = Vw + RB * 16 * NLD;
-  // the strip, rows rbase + 16 rb + 4g + v at acc[rb][v]
-  d4 acc[RB];
+  double *Vw = S.P + 2 * NW * NS * 256 + h * (RB + 1) * 16 * NLD, *Tw = Vw + RB * 16 * NLD;
+  // the strips, rows rbase + 16 rb + 4g + v at acc[q][rb][v]
+  d4 acc[NS][RB];
 #pragma unroll
-  for (int rb = 0; rb < RB; rb++) {
-    acc[rb] = d4{0.0, 0.0, 0.0, 0.0};
-    if (rb < nrb && colok) {
+  for (int q = 0; q < NS; q++)
 #pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int r = rbase + 16 * rb + 4 * g + v;
-        if (r < nb) acc[rb][v] = X[(size_t)col * nb + r];
+    for (int rb = 0; rb < RB; rb++) {
+      acc[q][rb] = d4{0.0, 0.0, 0.0, 0.0};
+      if (rb < nrb && colok[q]) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int r = rbase + 16 * rb + 4 * g + v;
+          if (r < nb) acc[q][rb][v] = X[(size_t)col[q] * nb + r];
+        }
       }
     }
-  }
   const int nsb_per = (ib + 15) / 16, nsb = ((nb + ib - 1) / ib) * nsb_per;
   // loaded layout: lane l holds reflector lc = l / 4, rows 4 (l % 4) + e
   const int lc = lane >> 2, lr = 4 * (lane & 3);
@@ -509,16 +517,24 @@ __device__ __noinline__ void update_narrow(Smem &S, const double *Vt_, const dou
       for (int e = 0; e < 4; e++) Vw[rb * 16 * NLD + lc * NLD + lr + e] = vr[p][rb][e];
 #pragma unroll
     for (int e = 0; e < 4; e++) Tw[lc * NLD + lr + e] = tr[p][e];
-    // W partial over this group's rows: A(i, k) = V(row 4k + v, i), B = the strip
-    d4 wp = d4{0.0, 0.0, 0.0, 0.0};
+    // W partials over this group's rows: A(i, k) = V(row 4k + v, i) (read
+    // once for all strips), B = the strip
+    d4 wp[NS];
+#pragma unroll
+    for (int q = 0; q < NS; q++) wp[q] = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int rb = 0; rb < RB; rb++)
       if (rb < nrb) {
 #pragma unroll
-        for (int v = 0; v < 4; v++) wp = mfma(Vw[rb * 16 * NLD + m * NLD + 4 * g + v], acc[rb][v], wp);
+        for (int v = 0; v < 4; v++) {
+          const double a = Vw[rb * 16 * NLD + m * NLD + 4 * g + v];
+#pragma unroll
+          for (int q = 0; q < NS; q++) wp[q] = mfma(a, acc[q][rb][v], wp[q]);
+        }
       }
-    double *Wb = Wx + (sbi & 1) * (NW * 256);
-    *reinterpret_cast<d4 *>(Wb + h * 256 + 4 * lane) = wp;
+    double *Wb = Wx + (sbi & 1) * (NW * NS * 256);
+#pragma unroll
+    for (int q = 0; q < NS; q++) *reinterpret_cast<d4 *>(Wb + (h * NS + q) * 256 + 4 * lane) = wp[q];
     // TS, pipelined: the predecessor's rows of this sub-block written
     if (TS && pp.wait && tid == 0) {
       const unsigned long long need = (pp.epoch << 32) | (unsigned long long)(sbi + 1);
@@ -536,32 +552,48 @@ __device__ __noinline__ void update_narrow(Smem &S, const double *Vt_, const dou
     // pipelined chain they are read and written as agent-scope atomics
     // (coherent across the XCDs' L2s without write-back or invalidation),
     // which is all that passes between the two tasks
-    d4 x1 = d4{0.0, 0.0, 0.0, 0.0};
-    if (TS && colok) {
+    d4 x1[NS];
 #pragma unroll
-      for (int v = 0; v < 4; v++)
-        if (4 * v + g < nq) {
-          double *a = (double *)&X1[(size_t)col * nb + ii + q0 + 4 * v + g];
-          x1[v] = pp.prog ? __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *a;
-        }
+    for (int q = 0; q < NS; q++) {
+      x1[q] = d4{0.0, 0.0, 0.0, 0.0};
+      if (TS && colok[q]) {
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+          if (4 * v + g < nq) {
+            double *a = (double *)&X1[(size_t)col[q] * nb + ii + q0 + 4 * v + g];
+            x1[q][v] = pp.prog ? __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *a;
+          }
+      }
     }
-    d4 w = d4{0.0, 0.0, 0.0, 0.0};
+    d4 w[NS], w2[NS];
 #pragma unroll
-    for (int qg = 0; qg < NW; qg++) w += *reinterpret_cast<const d4 *>(Wb + qg * 256 + 4 * lane);
-    if (TS) w += x1;
+    for (int q = 0; q < NS; q++) {
+      w[q] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int qg = 0; qg < NW; qg++) w[q] += *reinterpret_cast<const d4 *>(Wb + (qg * NS + q) * 256 + 4 * lane);
+      if (TS) w[q] += x1[q];
+      w2[q] = d4{0.0, 0.0, 0.0, 0.0};
+    }
     // W2 = T^T W: A(i, k) = T(k, i)
-    d4 w2 = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int kk = 0; kk < 4; kk++) w2 = mfma(Tw[m * NLD + 4 * kk + g], w[kk], w2);
-    if (TS && h == 0 && colok) {
+    for (int kk = 0; kk < 4; kk++) {
+      const double t = Tw[m * NLD + 4 * kk + g];
 #pragma unroll
-      for (int v = 0; v < 4; v++)
-        if (4 * v + g < nq) {
-          double *a = (double *)&X1[(size_t)col * nb + ii + q0 + 4 * v + g];
-          if (pp.prog)
-            __hip_atomic_store(a, x1[v] - w2[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          else
-            *a = x1[v] - w2[v];
+      for (int q = 0; q < NS; q++) w2[q] = mfma(t, w[q][kk], w2[q]);
+    }
+    if (TS && h == 0) {
+#pragma unroll
+      for (int q = 0; q < NS; q++)
+        if (colok[q]) {
+#pragma unroll
+          for (int v = 0; v < 4; v++)
+            if (4 * v + g < nq) {
+              double *a = (double *)&X1[(size_t)col[q] * nb + ii + q0 + 4 * v + g];
+              if (pp.prog)
+                __hip_atomic_store(a, x1[q][v] - w2[q][v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              else
+                *a = x1[q][v] - w2[q][v];
+            }
         }
     }
     if (TS && pp.prog && h == 0) {
@@ -573,14 +605,17 @@ __device__ __noinline__ void update_narrow(Smem &S, const double *Vt_, const dou
         __hip_atomic_store(pp.prog, (pp.epoch << 32) | (unsigned long long)(sbi + 1), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     }
-    const d4 nw2 = -w2;
-    // X -= V W2: A(i, k) = V(row arow(i) of block rb, reflector k)
+    // X -= V W2: A(i, k) = V(row arow(i) of block rb, reflector k), read once
+    // for all strips
 #pragma unroll
     for (int rb = 0; rb < RB; rb++)
       if (rb < nrb) {
 #pragma unroll
-        for (int kk = 0; kk < 4; kk++)
-          acc[rb] = mfma(Vw[rb * 16 * NLD + (4 * kk + g) * NLD + arow(m)], nw2[kk], acc[rb]);
+        for (int kk = 0; kk < 4; kk++) {
+          const double a = Vw[rb * 16 * NLD + (4 * kk + g) * NLD + arow(m)];
+#pragma unroll
+          for (int q = 0; q < NS; q++) acc[q][rb] = mfma(a, -w2[q][kk], acc[q][rb]);
+        }
       }
   };
 #pragma unroll
@@ -595,15 +630,17 @@ __device__ __noinline__ void update_narrow(Smem &S, const double *Vt_, const dou
     }
   }
 #pragma unroll
-  for (int rb = 0; rb < RB; rb++) {
-    if (rb < nrb && colok) {
+  for (int q = 0; q < NS; q++)
 #pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int r = rbase + 16 * rb + 4 * g + v;
-        if (r < nb) X[(size_t)col * nb + r] = acc[rb][v];
+    for (int rb = 0; rb < RB; rb++) {
+      if (rb < nrb && colok[q]) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int r = rbase + 16 * rb + 4 * g + v;
+          if (r < nb) X[(size_t)col[q] * nb + r] = acc[q][rb][v];
+        }
       }
     }
-  }
   __syncthreads();
 }
 
@@ -634,11 +671,11 @@ __device__ void pad_panel_blocks(double *T, double *TAU, int nb, int ib, int ii)
 // NRB > 0: an application of Q^T only (LARFB / SSRFB on 16-column slabs,
 // hq_geom::uw; update_narrow with NRB row blocks per group), a kernel of its
 // own.
-template <int NRB = 0>
+template <int NRB = 0, int NS = 1>
 __device__ void run_task(Smem &S, const hq_geom &G, const hq_task &t, int self = -1, int epoch = 0,
                          unsigned wait_limit = 0, int *err = nullptr) {
   constexpr bool NARROW = NRB > 0;
-  constexpr int PD = NRB <= 2 ? 4 : 3;  // sub-blocks in flight (registers)
+  constexpr int PD = NS > 1 ? 3 : NRB <= 2 ? 4 : 3;  // sub-blocks in flight (registers)
   const int nb = G.nb, ib = G.ib;
   if constexpr (NARROW)
     if (t.kind != HQ_LARFB && t.kind != HQ_SSRFB) return;
@@ -699,10 +736,10 @@ __device__ void run_task(Smem &S, const hq_geom &G, const hq_task &t, int self =
     case HQ_LARFB: {
       const double *V = tile_ptr(G, t.k, t.k), *T = t_ptr(G, t.k, t.k);
       double *X = xtile_ptr(G, t.k, t.j);
-      const int uw = NARROW ? 16 : hq_upd_slab(nb, 0), c0 = t.slab * uw, c1 = min(live_cols(G, t.j), c0 + uw);
+      const int uw = NARROW ? 16 * NS : hq_upd_slab(nb, 0), c0 = t.slab * uw, c1 = min(live_cols(G, t.j), c0 + uw);
       if (c0 >= c1) break;
       if constexpr (NARROW)
-        update_narrow<false, (NARROW ? NRB : 1), PD>(S, V, T, ib, X, nullptr, nb, c0, c1, Pipe{});
+        update_narrow<false, (NARROW ? NRB : 1), PD, NS>(S, V, T, ib, X, nullptr, nb, c0, c1, Pipe{});
       else
         update_strips<false>(S, V, T, ib, X, nullptr, nb, 0, (nb + ib - 1) / ib, c0, c1);
       break;
@@ -710,7 +747,7 @@ __device__ void run_task(Smem &S, const hq_geom &G, const hq_task &t, int self =
     case HQ_SSRFB: {
       const double *V = tile_ptr(G, t.r, t.k), *T = t_ptr(G, t.r, t.k);
       double *X1 = xtile_ptr(G, t.k, t.j), *X2 = xtile_ptr(G, t.r, t.j);
-      const int uw = NARROW ? 16 : hq_upd_slab(nb, 0), c0 = t.slab * uw, c1 = min(live_cols(G, t.j), c0 + uw);
+      const int uw = NARROW ? 16 * NS : hq_upd_slab(nb, 0), c0 = t.slab * uw, c1 = min(live_cols(G, t.j), c0 + uw);
       if (c0 >= c1) break;
       if constexpr (NARROW) {
         // dep[0] = -2 - predecessor: the previous SSRFB of the chain, waited
@@ -724,7 +761,7 @@ __device__ void run_task(Smem &S, const hq_geom &G, const hq_task &t, int self =
           pp.err = err;
           pp.self = self;
         }
-        update_narrow<true, (NARROW ? NRB : 1), PD>(S, V, T, ib, X2, X1, nb, c0, c1, pp);
+        update_narrow<true, (NARROW ? NRB : 1), PD, NS>(S, V, T, ib, X2, X1, nb, c0, c1, pp);
       } else
         update_strips<true>(S, V, T, ib, X2, X1, nb, 0, (nb + ib - 1) / ib, c0, c1);
       break;
@@ -739,7 +776,7 @@ __device__ void run_task(Smem &S, const hq_geom &G, const hq_task &t, int self =
 // counter, dependency flags equal to this run's epoch, release before
 // publishing, acquire after the wait, a bounded wait that reports the task in
 // counter[1] and drains the grid), without its look-ahead claim.
-template <int NRB>
+template <int NRB, int NS = 1>
 __global__ __launch_bounds__(NT) void sched_kernel(hq_geom G0, const hq_geom *__restrict__ geoms,
                                                    const hq_task *__restrict__ tasks, int ntasks, int *flags,
                                                    int epoch_arg, int *counter, long long *trace,
@@ -805,7 +842,7 @@ __global__ __launch_bounds__(NT) void sched_kernel(hq_geom G0, const hq_geom *__
       trace[4 * t + 1] = wall_clock64();
       trace[4 * t + 3] = blockIdx.x;
     }
-    run_task<NRB>(S, geoms ? geoms[tk.mat] : G0, tk, t, epoch, wait_limit, counter + 1);
+    run_task<NRB, NS>(S, geoms ? geoms[tk.mat] : G0, tk, t, epoch, wait_limit, counter + 1);
     // every wave's stores have completed before the flag can be published
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
@@ -928,9 +965,14 @@ __global__ __launch_bounds__(NT) void tbuild_tile_kernel(const double *V, const 
 hipError_t hqg_launch_sched(const hq_geom &G, const hq_geom *geoms, const hq_task *tasks, int ntasks, int *flags,
                             int epoch, int *counter, int grid, long long *trace, hipStream_t st, int *epoch_dev,
                             unsigned wait_limit) {
-  if (G.nb > hqg::NBMAX || G.ib > hqg::IBM || G.prec != 0 || (G.uw && G.uw != 16)) return hipErrorInvalidValue;
-  // 16-column apply: row groups of NB / 8 rows, in 16-row blocks
-  if (G.uw && G.nb <= 256)
+  if (G.nb > hqg::NBMAX || G.ib > hqg::IBM || G.prec != 0 || (G.uw && G.uw != 16 && G.uw != 32) ||
+      (G.uw == 32 && G.nb > 256))
+    return hipErrorInvalidValue;
+  // 16- / 32-column apply: row groups of NB / 8 rows, in 16-row blocks
+  if (G.uw == 32)
+    hipLaunchKernelGGL((hqg::sched_kernel<2, 2>), dim3(grid), dim3(hqg::NT), 0, st, G, geoms, tasks, ntasks, flags,
+                       epoch, counter, trace, (const int *)epoch_dev, wait_limit);
+  else if (G.uw && G.nb <= 256)
     hipLaunchKernelGGL(hqg::sched_kernel<2>, dim3(grid), dim3(hqg::NT), 0, st, G, geoms, tasks, ntasks, flags,
                        epoch, counter, trace, (const int *)epoch_dev, wait_limit);
   else if (G.uw)

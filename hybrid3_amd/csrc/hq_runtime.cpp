@@ -206,7 +206,7 @@ struct DagBuilder {
             int x = Sr(k, j, r, s) = add(HQ_SSRFB, k, r, j, s);
             dep(x, r == k + 1 ? L(k, j, s) : Sr(k, j, r - 1, s));
             if (k > 0) dep(x, Sr(k - 1, j, r, s));
-            if (UW == 16 && r > k + 1) pipe[x] = Sr(k, j, r - 1, s);
+            if (UW <= 32 && r > k + 1) pipe[x] = Sr(k, j, r - 1, s);
           }
     }
   }
@@ -542,7 +542,8 @@ int get_schedule(Context *c, int MT, int NT, int nb, int ib, CachedSched **out, 
     // 16-column apply: an SSRFB below the first waits for the previous
     // SSRFB of its chain (its dep[0], build_apply) sub-block by sub-block,
     // inside the task (hq_geom::prog), marked -2 - index
-    if (XNT >= 1 && uw == 16 && x.kind == HQ_SSRFB && x.r > x.k + 1 && x.ndep > 0) x.dep[0] = -2 - x.dep[0];
+    if (XNT >= 1 && (uw == 16 || uw == 32) && x.kind == HQ_SSRFB && x.r > x.k + 1 && x.ndep > 0)
+      x.dep[0] = -2 - x.dep[0];
     tl[i] = x;
   }
   std::unique_ptr<CachedSched> cs(new CachedSched());
@@ -667,6 +668,14 @@ int build_t_impl(Context *c, int MT, int NT, int nb, int ib, const double *RY, c
   return HQ_OK;
 }
 
+// Task width of a Q^T application to ncol columns on the narrow tasks: one
+// 16-column strip per chain up to 32 columns (the most chains, the chain's
+// pace is the run), two strips sharing each V and T operand above that (the
+// workgroups are then the limit: 64 columns 4.7 against 5.3 ms, 256 columns
+// 13.6 against 18.3 ms at 65536 rows after a 65536 x 4096 factorisation);
+// two strips need nb <= 256.
+int narrow_width(int ncol, int nb) { return ncol > 32 && nb <= 256 ? 32 : 16; }
+
 // X (MT x XNT tiles) <- Q^T X, Q from the reflectors of RY (MT x NT tiles).
 // A caller's T is applied with its own ib (the update kernels take the 16 x 16
 // diagonal blocks of T for any ib); without T (or with nb = 64, whose update
@@ -682,15 +691,16 @@ int apply_qt_impl(int MT, int NT, int nb, int ib, const double *RY, const double
   if (XNT < 1) return set_error(HQ_ERR_ARG, "X needs at least one tile column");
   if (!RY || !TAU || !X) return set_error(HQ_ERR_ARG, "RY, TAU and X must be device pointers");
   // a whole X of at most HQ_NARROW_COLS - 1 columns (dormqr on one tile
-  // column, dorgqr of a single tile column): 16-column tasks too
-  if (width == 0 && ncol == 0 && XNT * nb < HQ_NARROW_COLS && nb % 16 == 0) width = 16;
+  // column, dorgqr of a single tile column): the narrow tasks too
+  if (width == 0 && ncol == 0 && XNT * nb < HQ_NARROW_COLS && nb % 16 == 0) width = narrow_width(XNT * nb, nb);
   Context *c;
   if (get_ctx(&c) != HQ_OK) return g_err;
   hipStream_t st = pick_stream(c, stream);
   if (order_in(c, st) != HQ_OK) return g_err;
   double *Tw = (double *)T;
   int iq = ib;
-  if (Tw && (nb < 128 || !tuned_nb(nb) || width == 16) && ib > HQ_IBMAX) Tw = nullptr;
+  const bool narrow = width == 16 || width == 32;
+  if (Tw && (nb < 128 || !tuned_nb(nb) || narrow) && ib > HQ_IBMAX) Tw = nullptr;
   if (!Tw) {
     iq = ibi(ib);
     if (ensure_T(c, MT, NT, nb, iq, &Tw) != HQ_OK) return g_err;
@@ -701,7 +711,7 @@ int apply_qt_impl(int MT, int NT, int nb, int ib, const double *RY, const double
   hq_geom G{(double *)RY, (double *)TAU, Tw, MT, NT, nb, iq, X, XNT};
   G.ncol = ncol;
   G.uw = width;
-  G.prog = width == 16 ? (unsigned long long *)c->prog.p : nullptr;
+  G.prog = narrow ? (unsigned long long *)c->prog.p : nullptr;
   c->epoch++;
   HQ_HIP(hq_launch_sched(G, nullptr, (const hq_task *)cs->tasks.p, cs->ntasks, (int *)c->flags.p,
                          c->epoch, (int *)c->counter.p, cs->grid, trace, st, nullptr, c->wait_limit));
@@ -1310,19 +1320,20 @@ int hq_tile_ormqr_dev(int MT, int NT, int nb, int ib, const double *RY, const do
   return apply_qt_impl(MT, NT, nb, ib, RY, TAU, T, X, XNT, stream);
 }
 
-// Narrow X: width 0 picks 16-column tasks when X holds fewer than
-// HQ_NARROW_COLS columns (a short block of right-hand sides, whose few
-// column chains along the flat tree are the whole critical path), else the
-// factorisation's slabs.
+// Narrow X: width 0 picks 16- or 32-column tasks (narrow_width) when X
+// holds fewer than HQ_NARROW_COLS columns (a short block of right-hand
+// sides, whose few column chains along the flat tree are the whole critical
+// path), else the factorisation's slabs.
 static int ormqr_cols(int MT, int NT, int nb, int ib, const double *RY, const double *TAU, const double *T,
                       double *X, int XNT, int ncol, int width, long long *trace, void *stream) {
   if (XNT < 1) return set_error(HQ_ERR_ARG, "X needs at least one tile column");
   if (ncol <= (XNT - 1) * nb || ncol > XNT * nb)
     return set_error(HQ_ERR_ARG, "ncol=%d outside tile column %d of X (nb=%d)", ncol, XNT, nb);
-  if (width != 0 && width != 16 && width != 64)
-    return set_error(HQ_ERR_ARG, "update width %d (0, 16 or 64)", width);
-  if (width == 0) width = ncol < HQ_NARROW_COLS ? 16 : 64;
-  return apply_qt_impl(MT, NT, nb, ib, RY, TAU, T, X, XNT, stream, ncol, width == 16 ? 16 : 0, trace);
+  if (width != 0 && width != 16 && width != 32 && width != 64)
+    return set_error(HQ_ERR_ARG, "update width %d (0, 16, 32 or 64)", width);
+  if (width == 32 && nb > 256) return set_error(HQ_ERR_ARG, "32-column updates need nb <= 256 (nb=%d)", nb);
+  if (width == 0) width = ncol < HQ_NARROW_COLS ? narrow_width(ncol, nb) : 64;
+  return apply_qt_impl(MT, NT, nb, ib, RY, TAU, T, X, XNT, stream, ncol, width == 64 ? 0 : width, trace);
 }
 
 int hq_tile_ormqr_cols_dev(int MT, int NT, int nb, int ib, const double *RY, const double *TAU,

@@ -180,8 +180,8 @@ int hq_tile_build_t_dev(int MT, int NT, int nb, int ib, const double *RY,
 /* X (MT x XNT tiles) <- Q^T X with Q = H(0,0) H(1,0) ... from the tile QR
  * in RY / TAU (MT x NT tiles); T may be NULL (rebuilt).  The sweep of
  * Hybrid_dormqr.c:64-167 on the device DAG scheduler.  An X of at most 256
- * columns (nb a multiple of 16) runs as the 16-column tasks of
- * hq_tile_ormqr_cols_dev. */
+ * columns (nb a multiple of 16) runs as the narrow tasks of
+ * hq_tile_ormqr_cols_dev (width 0). */
 int hq_tile_ormqr_dev(int MT, int NT, int nb, int ib, const double *RY,
                       const double *TAU, const double *T, double *X, int XNT,
                       void *stream);
@@ -192,9 +192,11 @@ int hq_tile_ormqr_dev(int MT, int NT, int nb, int ib, const double *RY,
  * 20-171) when it is a short block of right-hand sides.  Tasks that would
  * update only zero columns are left out.  width: 16 runs the updates as
  * 16-column tasks (one chain per 16 columns along the flat tree, each SSRFB
- * following the previous one 16 reflectors at a time), 64 as the
- * factorisation's update slabs, 0 picks 16 up to 256 columns.  Errors:
- * HQ_ERR_ARG for ncol outside X's last tile column or another width. */
+ * following the previous one 16 reflectors at a time), 32 as 32-column
+ * tasks of the same kind (nb <= 256), 64 as the factorisation's update
+ * slabs; 0 picks 16 up to 32 columns, 32 up to 256, else 64.  Errors:
+ * HQ_ERR_ARG for ncol outside X's last tile column, another width, or 32
+ * with nb above 256. */
 int hq_tile_ormqr_cols_dev(int MT, int NT, int nb, int ib, const double *RY,
                            const double *TAU, const double *T, double *X,
                            int XNT, int ncol, int width, void *stream);

@@ -92,12 +92,15 @@ def test_device_entry_points_validate_before_launch(hq):
     assert L.hq_tile_qr_dev(0, 4, 64, 16, None, None, None, None) == 1
     assert L.hq_dgeqrt_dev(64, 0, None, None, None, None) == 1
     assert L.hq_tile_ormqr_dev(4, 4, 64, 16, None, None, None, None, 0, None) == 1
-    # ncol outside X's last tile column, an update width other than 0 / 16 / 64
+    # ncol outside X's last tile column, an update width other than 0 / 16 /
+    # 32 / 64, 32-column updates above nb = 256
     assert L.hq_tile_ormqr_cols_dev(4, 4, 64, 16, None, None, None, None, 1, 65, 0, None) == 1
     assert L.hq_tile_ormqr_cols_dev(4, 4, 64, 16, None, None, None, None, 2, 64, 0, None) == 1
-    assert L.hq_tile_ormqr_cols_dev(4, 4, 64, 16, None, None, None, None, 1, 64, 32, None) == 1
+    assert L.hq_tile_ormqr_cols_dev(4, 4, 64, 16, None, None, None, None, 1, 64, 48, None) == 1
     code, msg = hq.last_error()
-    assert code == 1 and "width 32" in msg
+    assert code == 1 and "width 48" in msg
+    assert L.hq_tile_ormqr_cols_dev(2, 2, 384, 16, None, None, None, None, 1, 64, 32, None) == 1
+    assert "nb <= 256" in hq.last_error()[1]
 
 
 def test_tile_init_allocates_three_tiles_of_zeros(hq):
