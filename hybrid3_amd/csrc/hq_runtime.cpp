@@ -138,6 +138,7 @@ struct DagBuilder {
   int prec;  // precision of the LARFB / SSRFB updates (their slab width and cost)
   int UW, SU;  // LARFB / SSRFB slab width (hq_upd_slab) and slabs per tile
   int NCX;     // apply mode: X's columns that hold entries (0: all)
+  bool NARROWX;  // apply mode on the narrow (16- / 32-column) tasks, whose SSRFB chains are pipelined
   // pipe[i]: the task that task i follows sub-block by sub-block (its dep[0],
   // the previous SSRFB of a 16-column chain; -1 otherwise)
   std::vector<int> pipe;
@@ -153,6 +154,7 @@ struct DagBuilder {
         split(split_) {
     S = (nb + HQ_SLAB - 1) / HQ_SLAB;
     UW = apply && uw_ ? uw_ : hq_upd_slab(nb, prec);
+    NARROWX = apply && uw_ > 0;
     NCX = apply ? ncol_ : 0;
     SU = (nb + UW - 1) / UW;
     KT = std::min(MT, NT);
@@ -206,7 +208,7 @@ struct DagBuilder {
             int x = Sr(k, j, r, s) = add(HQ_SSRFB, k, r, j, s);
             dep(x, r == k + 1 ? L(k, j, s) : Sr(k, j, r - 1, s));
             if (k > 0) dep(x, Sr(k - 1, j, r, s));
-            if (UW <= 32 && r > k + 1) pipe[x] = Sr(k, j, r - 1, s);
+            if (NARROWX && r > k + 1) pipe[x] = Sr(k, j, r - 1, s);
           }
     }
   }
