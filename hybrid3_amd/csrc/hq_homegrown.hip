@@ -6,7 +6,9 @@
 // factorisation's hot path (the reference calls them from Hybrid_dorgqr /
 // Hybrid_dormqr only), so each is one 256-thread workgroup working in global
 // memory: every loop of the reference becomes a parallel loop between
-// barriers, reductions are wave butterflies.
+// barriers, reductions are wave butterflies.  A thread's dot-product loops
+// are unrolled (one accumulator, the same summation order), so that their
+// loads are in flight together instead of one memory latency per term.
 #include "hq_internal.h"
 
 namespace hqh {
@@ -118,6 +120,7 @@ __device__ void sub_dlarft(int row, int col, const double *Y, int lda, const dou
     const double ti = -IX(T, ldt, i, i);
     for (int r = tid; r < i; r += NT) {
       double s = 0.0;
+#pragma unroll 8
       for (int c = 0; c < i; c++) s += IX(T, ldt, c, r) * YtY[c + (size_t)i * col];
       IX(T, ldt, i, r) = ti * s;
     }
@@ -133,6 +136,7 @@ __device__ void sub_dlarfb(int row, int Y_col, int A_col, const double *Y, int l
   for (int e = tid; e < Y_col * A_col; e += NT) {
     const int r = e % Y_col, j = e / Y_col;
     double s = 0.0;
+#pragma unroll 8
     for (int c = r; c < row; c++) s += IX(Y, ldy, c, r) * IX(A, lda, c, j);
     W[e] = s;
   }
@@ -140,6 +144,7 @@ __device__ void sub_dlarfb(int row, int Y_col, int A_col, const double *Y, int l
   for (int e = tid; e < Y_col * A_col; e += NT) {
     const int r = e % Y_col, j = e / Y_col;
     double s = 0.0;
+#pragma unroll 8
     for (int c = 0; c <= r; c++) s += IX(T, ldt, r, c) * W[c + (size_t)j * Y_col];
     W2[e] = s;
   }
@@ -147,6 +152,7 @@ __device__ void sub_dlarfb(int row, int Y_col, int A_col, const double *Y, int l
   for (int e = tid; e < row * A_col; e += NT) {
     const int r = e % row, j = e / row;
     double s = 0.0;
+#pragma unroll 8
     for (int c = 0; c < Y_col; c++) s += IX(Y, ldy, r, c) * W2[c + (size_t)j * Y_col];
     IX(A, lda, r, j) -= s;
   }
@@ -176,6 +182,7 @@ __device__ void sub_dssrfb(int over_row, int under_row, int Y_col, int A_col, co
     double s = W[e];
     if (r < Y_col) {
       s = 0.0;
+#pragma unroll 8
       for (int c = 0; c <= r; c++) s += IX(T, ldt, r, c) * W[c + (size_t)j * over_row];
     }
     W2[e] = s;
@@ -188,6 +195,7 @@ __device__ void sub_dssrfb(int over_row, int under_row, int Y_col, int A_col, co
   for (int e = tid; e < under_row * A_col; e += NT) {
     const int r = e % under_row, j = e / under_row;
     double s = 0.0;
+#pragma unroll 8
     for (int c = 0; c < Y_col; c++) s += IX(Y, ldy, r, c) * W2[c + (size_t)j * over_row];
     IX(A2, lda2, r, j) -= s;
   }
@@ -322,6 +330,7 @@ __global__ __launch_bounds__(NT) void dlarft_kernel(int row, int col, int block,
   for (int e = tid; e < col * col; e += NT) {
     const int r = e / col, c = e % col;
     double s = 0.0;
+#pragma unroll 8
     for (int p = 0; p < row; p++) s += IX(Y, lda, p, r) * IX(Y, lda, p, c);
     YtY[(size_t)r * ldyty + c] = s;
   }
@@ -330,6 +339,7 @@ __global__ __launch_bounds__(NT) void dlarft_kernel(int row, int col, int block,
     for (int e = tid; e < i * block; e += NT) {
       const int r = e / block, c = e % block;
       double s = 0.0;
+#pragma unroll 8
       for (int p = 0; p < i; p++) s += IX(T, ldt, p, r) * YtY[(size_t)p * ldyty + i + c];
       T1YtY[(size_t)r * block + c] = s;
     }
@@ -337,6 +347,7 @@ __global__ __launch_bounds__(NT) void dlarft_kernel(int row, int col, int block,
     for (int e = tid; e < i * block; e += NT) {
       const int r = e / block, c = e % block;
       double s = 0.0;
+#pragma unroll 8
       for (int p = 0; p < block; p++) s += T1YtY[(size_t)r * block + p] * IX(T, ldt, i + c, i + p);
       IX(T, ldt, i + c, r) = -s;
     }
@@ -346,6 +357,7 @@ __global__ __launch_bounds__(NT) void dlarft_kernel(int row, int col, int block,
     const double ti = -IX(T, ldt, i, i);
     for (int r = tid; r < i; r += NT) {
       double s = 0.0;
+#pragma unroll 8
       for (int c = 0; c < i; c++) s += IX(T, ldt, c, r) * YtY[(size_t)c * ldyty + i];
       IX(T, ldt, i, r) = ti * s;
     }
