@@ -1,6 +1,6 @@
 """GPU parity of Q generation / application (Hybrid_dorgqr.c, Hybrid_dormqr.c,
 Hybrid_make_Q.c semantics: the Q array holds Q^T in tile layout) against the
-CPU oracle, through the C-ABI, plus the north-star norms."""
+CPU oracle, through the C-ABI, plus the residual and orthogonality norms."""
 import ctypes as C
 
 import numpy as np
@@ -104,7 +104,7 @@ def test_ormqr_on_narrow_block(hq, orc):
     assert np.allclose(np.linalg.norm(out, axis=0), np.linalg.norm(Cm, axis=0), rtol=1e-13)
 
 
-def test_config1_north_star_norms(hq, orc):
+def test_2048_nb128_residual_and_orthogonality_norms(hq, orc):
     """2048^2, nb=128, ib=32: ||A-QR||_F/||A||_F and ||I-Q^T Q||_F with Q from
     the GPU dorgqr path."""
     import torch
