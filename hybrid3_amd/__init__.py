@@ -79,6 +79,8 @@ SIGNATURES = {
     "hq_tile_make_q_dev": (C.c_int, [C.c_int] * 4 + [_vp, _vp, _vp, _vp]),
     "hq_dag_host": (C.c_int, [C.c_int] * 6 + [C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int,
                                              C.POINTER(C.c_int)]),
+    "hq_dag_host_apply": (C.c_int, [C.c_int] * 8 + [C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int,
+                                                 C.POINTER(C.c_int)]),
     "hq_dag_host_prec": (C.c_int, [C.c_int] * 7 + [C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int,
                                                   C.POINTER(C.c_int)]),
     "hq_batched_tile_qr_dev": (C.c_int, [C.c_int] * 3 + [C.POINTER(C.c_int)] * 2 + [C.POINTER(_vp)] * 3 + [_vp]),
@@ -386,4 +388,19 @@ def dag_host(MT, NT, nb, ib, XNT=0, P=256, prec=0):
     ip = C.POINTER(C.c_int)
     check(L.hq_dag_host_prec(MT, NT, nb, ib, prec, XNT, P, tasks.ctypes.data_as(ip), deps.ctypes.data_as(ip),
                              n.value, C.byref(n)))
+    return tasks.reshape(-1, 5), deps.reshape(-1, 6)
+
+
+def dag_host_apply(MT, NT, nb, ib, XNT=1, ncol=0, width=16, P=256):
+    """Task list of a Q^T application in claim order as the device receives it
+    (no GPU): (tasks[n,5], deps[n,6]); a pipelined predecessor is -2 - index,
+    -1 pads."""
+    L = lib()
+    n = C.c_int(0)
+    check(L.hq_dag_host_apply(MT, NT, nb, ib, XNT, ncol, width, P, None, None, 0, C.byref(n)))
+    tasks = np.zeros(5 * n.value, dtype=np.int32)
+    deps = np.zeros(6 * n.value, dtype=np.int32)
+    ip = C.POINTER(C.c_int)
+    check(L.hq_dag_host_apply(MT, NT, nb, ib, XNT, ncol, width, P, tasks.ctypes.data_as(ip),
+                              deps.ctypes.data_as(ip), n.value, C.byref(n)))
     return tasks.reshape(-1, 5), deps.reshape(-1, 6)

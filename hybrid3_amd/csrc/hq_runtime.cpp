@@ -517,6 +517,29 @@ std::vector<hq_task> claim_order(std::vector<hq_task> &&tl, int nb, int ib, int 
   return out;
 }
 
+// The task list the device scheduler runs, in claim order (list scheduling
+// on P workers), dependencies as claim indices; in a 16- / 32-column apply an
+// SSRFB below the first waits for the previous SSRFB of its chain (its
+// dep[0], build_apply) sub-block by sub-block, inside the task
+// (hq_geom::prog): that dependency is marked -2 - index.
+std::vector<hq_task> claim_list(int MT, int NT, int nb, int ib, int XNT, int prec, int uw, int ncol, int P) {
+  DagBuilder b(MT, NT, nb, ib, XNT >= 1, XNT, true, prec, uw, ncol);
+  b.build();
+  std::vector<int> ord = b.order(P);
+  const int n = (int)b.t.size();
+  std::vector<int> pos(n);
+  for (int i = 0; i < n; i++) pos[ord[i]] = i;
+  std::vector<hq_task> tl(n);
+  for (int i = 0; i < n; i++) {
+    hq_task x = b.t[ord[i]];
+    for (int d = 0; d < x.ndep; d++) x.dep[d] = pos[x.dep[d]];
+    if (XNT >= 1 && (uw == 16 || uw == 32) && x.kind == HQ_SSRFB && x.r > x.k + 1 && x.ndep > 0)
+      x.dep[0] = -2 - x.dep[0];
+    tl[i] = x;
+  }
+  return tl;
+}
+
 int get_schedule(Context *c, int MT, int NT, int nb, int ib, CachedSched **out, int XNT = -1, int prec = 0,
                  int uw = 0, int ncol = 0) {
   // XNT < 0: factorisation DAG (prec: precision of its updates); XNT >= 1:
@@ -530,24 +553,9 @@ int get_schedule(Context *c, int MT, int NT, int nb, int ib, CachedSched **out, 
     *out = it->second.get();
     return HQ_OK;
   }
-  DagBuilder b(MT, NT, nb, ib, XNT >= 1, XNT, true, prec, uw, ncol);
-  b.build();
-  int grid = c->num_cu > 0 ? c->num_cu : 256;
-  std::vector<int> ord = b.order(grid);
-  const int n = (int)b.t.size();
-  std::vector<int> pos(n);
-  for (int i = 0; i < n; i++) pos[ord[i]] = i;
-  std::vector<hq_task> tl(n);
-  for (int i = 0; i < n; i++) {
-    hq_task x = b.t[ord[i]];
-    for (int d = 0; d < x.ndep; d++) x.dep[d] = pos[x.dep[d]];
-    // 16-column apply: an SSRFB below the first waits for the previous
-    // SSRFB of its chain (its dep[0], build_apply) sub-block by sub-block,
-    // inside the task (hq_geom::prog), marked -2 - index
-    if (XNT >= 1 && (uw == 16 || uw == 32) && x.kind == HQ_SSRFB && x.r > x.k + 1 && x.ndep > 0)
-      x.dep[0] = -2 - x.dep[0];
-    tl[i] = x;
-  }
+  const int grid = c->num_cu > 0 ? c->num_cu : 256;
+  std::vector<hq_task> tl = claim_list(MT, NT, nb, ib, XNT, prec, uw, ncol, grid);
+  const int n = (int)tl.size();
   std::unique_ptr<CachedSched> cs(new CachedSched());
   if (cs->tasks.reserve(sizeof(hq_task) * (size_t)n) != HQ_OK) return g_err;
   HQ_HIP(hipMemcpy(cs->tasks.p, tl.data(), sizeof(hq_task) * (size_t)n, hipMemcpyHostToDevice));
@@ -1210,6 +1218,30 @@ int hq_dag_host_prec(int MT, int NT, int nb, int ib, int prec, int XNT, int P, i
 int hq_dag_host(int MT, int NT, int nb, int ib, int XNT, int P, int *tasks, int *deps,
                 int capacity, int *ntasks) {
   return hq_dag_host_prec(MT, NT, nb, ib, 0, XNT, P, tasks, deps, capacity, ntasks);
+}
+
+int hq_dag_host_apply(int MT, int NT, int nb, int ib, int XNT, int ncol, int width, int P, int *tasks, int *deps,
+                      int capacity, int *ntasks) {
+  if (!geometry_ok(MT, NT, nb, ib)) return g_err;
+  if (XNT < 1) return set_error(HQ_ERR_ARG, "X needs at least one tile column");
+  if (ncol < 0 || ncol > XNT * nb) return set_error(HQ_ERR_ARG, "ncol=%d outside X (%d columns)", ncol, XNT * nb);
+  if (width != 0 && width != 16 && width != 32) return set_error(HQ_ERR_ARG, "width %d (0, 16 or 32)", width);
+  if (P < 1) return set_error(HQ_ERR_ARG, "P must be >= 1");
+  const std::vector<hq_task> tl = claim_list(MT, NT, nb, ibi(ib), XNT, 0, width, ncol, P);
+  const int n = (int)tl.size();
+  if (ntasks) *ntasks = n;
+  if (!tasks || !deps) return ok();
+  if (capacity < n) return set_error(HQ_ERR_ARG, "capacity %d < %d tasks", capacity, n);
+  for (int i = 0; i < n; i++) {
+    const hq_task &x = tl[i];
+    tasks[5 * i + 0] = x.kind;
+    tasks[5 * i + 1] = x.k;
+    tasks[5 * i + 2] = x.r;
+    tasks[5 * i + 3] = x.j;
+    tasks[5 * i + 4] = x.slab;
+    for (int d = 0; d < HQ_MAXDEP; d++) deps[HQ_MAXDEP * i + d] = d < x.ndep ? x.dep[d] : -1;
+  }
+  return ok();
 }
 
 int hq_wall_clock_khz(int *khz) {

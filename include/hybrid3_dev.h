@@ -143,6 +143,16 @@ int hq_schedule_tasks_prec(int MT, int NT, int nb, int ib, int prec, int *out, i
  * only *ntasks is set. */
 int hq_dag_host(int MT, int NT, int nb, int ib, int XNT, int P, int *tasks,
                 int *deps, int capacity, int *ntasks);
+/* Host-only: the task list of a Q^T application to MT x XNT tiles on X's
+ * first ncol columns (0: all) with width 16 / 32 (the narrow tasks of
+ * hq_tile_ormqr_cols_dev) or 0 (the factorisation's slabs), in claim order
+ * for P workers, exactly as the device scheduler receives it: tasks as
+ * hq_dag_host; deps[6*i..] = claim indices of the dependencies, and a
+ * pipelined predecessor (the previous SSRFB of a narrow chain, waited for
+ * sub-block by sub-block inside the task) as -2 - its index; -1 padded. */
+int hq_dag_host_apply(int MT, int NT, int nb, int ib, int XNT, int ncol, int width, int P,
+                      int *tasks, int *deps, int capacity, int *ntasks);
+
 /* hq_dag_host of the factorisation with updates in precision prec. */
 int hq_dag_host_prec(int MT, int NT, int nb, int ib, int prec, int XNT, int P, int *tasks,
                      int *deps, int capacity, int *ntasks);
